@@ -1,0 +1,942 @@
+// Host side of libds2d.so: handle, batch layout and the per-step schedules.
+//
+// The handle owns every device buffer (as FD2DGPU_ELASTIC does, FD2DGPU.cu:
+// 1350-1801) and one HIP stream.  A batch packs many shot windows ("slots")
+// into shared field arenas so each reference kernel runs once per step for
+// all shots (the MPI master/worker shot loop of main.cpp:896-1036 becomes
+// one launch per step).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ds2d.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace ds2d;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(x)                                                                                   \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return fail(DS2D_EDEVICE, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                       \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct ds2d_ctx {
+    ds2d_geom g{};
+    hipStream_t st = nullptr;
+    std::vector<float> wav;
+    float* d_wav = nullptr;
+    float *d_vp = nullptr, *d_vs = nullptr, *d_rho = nullptr;  // global model
+    float *d_ipp = nullptr, *d_ips = nullptr;                  // global image
+    float* d_grad7 = nullptr;                                  // accumulated gradients
+    bool have_model = false, have_image = false, have_shot = false;
+    // current single shot (ds2d_set_shot)
+    ds2d_shot cur{};
+    std::vector<int> cur_rix;
+    std::vector<float> cur_oz, cur_ox;
+    std::map<std::string, DevBuf> ws;  // grow-only workspace
+    hipEvent_t ev[4]{};
+    ds2d_stats last{};
+
+    ~ds2d_ctx() {
+        for (auto& kv : ws)
+            if (kv.second.p) (void)hipFree(kv.second.p);
+        void* ps[] = {d_wav, d_vp, d_vs, d_rho, d_ipp, d_ips, d_grad7};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    // Grow-only named device buffer.
+    int buf(const char* name, size_t bytes, void** out) {
+        DevBuf& b = ws[name];
+        if (b.bytes < bytes) {
+            if (b.p) (void)hipFree(b.p);
+            b.p = nullptr;
+            b.bytes = 0;
+            hipError_t e = hipMalloc(&b.p, bytes ? bytes : 256);
+            if (e != hipSuccess) {
+                b.p = nullptr;
+                return fail(DS2D_ENOMEM, "hipMalloc(%s, %zu bytes): %s", name, bytes, hipGetErrorString(e));
+            }
+            b.bytes = bytes;
+        }
+        *out = b.p;
+        return 0;
+    }
+    template <typename T>
+    int fbuf(const char* name, size_t n, T** out) {
+        return buf(name, n * sizeof(T), (void**)out);
+    }
+};
+
+namespace {
+
+// Host image of a batch.
+struct Batch {
+    std::vector<SlotDesc> slots;
+    std::vector<int> rix, tbase;
+    BatchShape shape{};
+    long long total_cells = 0, total_rec = 0, total_win = 0, total_edge = 0, total_rmap = 0;
+    int total_traces = 0;
+    bool any_pml = false;
+};
+
+int build_batch(ds2d_ctx* h, int nshots, const ds2d_shot* shots, Batch& B) {
+    const ds2d_geom& g = h->g;
+    if (nshots <= 0 || !shots) return fail(DS2D_EINVAL, "nshots must be > 0");
+    const int pml = g.pml, nzp = g.allnz + 2 * pml;
+    B.slots.resize(nshots);
+    long long off = 0, roff = 0, woff = 0, eoff = 0, moff = 0;
+    int rixo = 0, max_nxp = 0, max_ntr = 0;
+    for (int s = 0; s < nshots; ++s) {
+        const ds2d_shot& q = shots[s];
+        if (q.nx <= 0 || q.ngx_min < 0 || q.ngx_min + q.nx > g.allnx)
+            return fail(DS2D_EINVAL, "shot %d: window [%d, %d) outside model of %d columns", s, q.ngx_min,
+                        q.ngx_min + q.nx, g.allnx);
+        const int nxp = q.nx + 2 * pml;
+        if (q.isx < NOP || q.isx >= nxp - NOP || q.isz < NOP || q.isz >= nzp - NOP)
+            return fail(DS2D_EINVAL, "shot %d: source (%d,%d) outside the update region", s, q.isx, q.isz);
+        if (q.ntr < 0 || (q.ntr > 0 && !q.rec_ix)) return fail(DS2D_EINVAL, "shot %d: bad receiver list", s);
+        if (q.igz < NOP || q.igz >= nzp - NOP) return fail(DS2D_EINVAL, "shot %d: receiver row %d", s, q.igz);
+        for (int r = 0; r < q.ntr; ++r)
+            if (q.rec_ix[r] < NOP || q.rec_ix[r] >= nxp - NOP)
+                return fail(DS2D_EINVAL, "shot %d: receiver %d column %d outside [%d,%d)", s, r, q.rec_ix[r], NOP,
+                            nxp - NOP);
+        SlotDesc& S = B.slots[s];
+        std::memset(&S, 0, sizeof S);
+        S.off = off;
+        S.rec_off = roff;
+        S.win_off = woff;
+        S.edge_off = eoff;
+        S.nxp = nxp;
+        S.nzp = nzp;
+        S.nx = q.nx;
+        S.nz = g.allnz;
+        S.pml = pml;
+        S.col0 = q.ngx_min;
+        S.isx = q.isx;
+        S.isz = q.isz;
+        S.igz = q.igz;
+        S.ntr = q.ntr;
+        S.rix_off = rixo;
+        S.shot_id = q.shot_id;
+        S.boundary = q.boundary;
+        S.rmap_off = (int)moff;
+        const long long n = (long long)nxp * nzp;
+        off += (n + 63) / 64 * 64;
+        roff += (long long)q.ntr * g.nt;
+        woff += (long long)q.nx * g.allnz;
+        if (q.boundary == 0) {
+            B.any_pml = true;
+            eoff += (long long)g.nt * (2LL * (2 * NOP) * q.nx + 2LL * (2 * NOP) * g.allnz);
+        }
+        moff += nxp;
+        B.tbase.push_back(B.total_traces);
+        B.total_traces += q.ntr;
+        for (int r = 0; r < q.ntr; ++r) B.rix.push_back(q.rec_ix[r]);
+        rixo += q.ntr;
+        max_nxp = std::max(max_nxp, nxp);
+        max_ntr = std::max(max_ntr, q.ntr);
+    }
+    B.total_cells = off;
+    B.total_rec = roff;
+    B.total_win = woff;
+    B.total_edge = eoff;
+    B.total_rmap = moff;
+    SlotDesc* d_slots;
+    int* d_rix;
+    int* d_tb;
+    int rc;
+    if ((rc = h->fbuf("slots", B.slots.size(), &d_slots))) return rc;
+    if ((rc = h->fbuf("rix", std::max<size_t>(B.rix.size(), 1), &d_rix))) return rc;
+    if ((rc = h->fbuf("tbase", B.tbase.size(), &d_tb))) return rc;
+    HIP_TRY(hipMemcpyAsync(d_slots, B.slots.data(), B.slots.size() * sizeof(SlotDesc), hipMemcpyHostToDevice, h->st));
+    if (!B.rix.empty())
+        HIP_TRY(hipMemcpyAsync(d_rix, B.rix.data(), B.rix.size() * sizeof(int), hipMemcpyHostToDevice, h->st));
+    HIP_TRY(hipMemcpyAsync(d_tb, B.tbase.data(), B.tbase.size() * sizeof(int), hipMemcpyHostToDevice, h->st));
+    BatchShape& sh = B.shape;
+    sh.slots = d_slots;
+    sh.rix = d_rix;
+    sh.nslots = nshots;
+    sh.max_nxp = max_nxp;
+    sh.nzp = nzp;
+    sh.nz = g.allnz;
+    sh.max_ntr = std::max(max_ntr, 1);
+    sh.nt = g.nt;
+    sh.total_cells = off;
+    sh.dt = g.dt;
+    sh.dx = g.dx;
+    sh.dz = g.dz;
+    return 0;
+}
+
+// Allocate (grow-only) and zero one 15-array elastic wavefield.
+int alloc_efield(ds2d_ctx* h, const char* tag, long long n, EField& f, bool zero = true) {
+    float** ptrs[15] = {&f.vx, &f.vz, &f.txx, &f.tzz, &f.txz, &f.vx_x, &f.vx_z, &f.vz_x, &f.vz_z,
+                        &f.txx_x, &f.txx_z, &f.tzz_x, &f.tzz_z, &f.txz_x, &f.txz_z};
+    float* base;
+    std::string name = std::string("ef_") + tag;
+    int rc = h->fbuf(name.c_str(), (size_t)n * 15, &base);
+    if (rc) return rc;
+    for (int k = 0; k < 15; ++k) *ptrs[k] = base + (size_t)k * n;
+    if (zero) HIP_TRY(hipMemsetAsync(base, 0, (size_t)n * 15 * sizeof(float), h->st));
+    return 0;
+}
+
+int alloc_med(ds2d_ctx* h, long long n, Med& m) {
+    float* base;
+    int rc = h->fbuf("med", (size_t)n * 9, &base);
+    if (rc) return rc;
+    float** ptrs[9] = {&m.vp, &m.vs, &m.rho, &m.lam, &m.mu, &m.dampx, &m.dampz, &m.dmp, &m.dms};
+    for (int k = 0; k < 9; ++k) *ptrs[k] = base + (size_t)k * n;
+    return 0;
+}
+
+SetupArgs setup_args(ds2d_ctx* h, bool with_image) {
+    SetupArgs a{};
+    a.vp_all = h->d_vp;
+    a.vs_all = h->d_vs;
+    a.rho_all = h->d_rho;
+    a.img_pp = with_image && h->have_image ? h->d_ipp : nullptr;
+    a.img_ps = with_image && h->have_image ? h->d_ips : nullptr;
+    a.allnx = h->g.allnx;
+    a.allnz = h->g.allnz;
+    a.pml_rr = h->g.pml_rr > 0 ? h->g.pml_rr : 1e-3f;
+    a.rho_const = h->g.rho_const;
+    a.rb_amp = h->g.rb_amp;
+    a.rb_seed = h->g.rb_seed;
+    return a;
+}
+
+double host_dot(ds2d_ctx* h, const float* x, const float* y, long long n, int* rc) {
+    const int nb = 1024;
+    double* d_part;
+    if ((*rc = h->fbuf("dotpart", nb, &d_part))) return 0;
+    launch_dot(h->st, x, y, n, d_part, nb);
+    std::vector<double> part(nb);
+    if (hipMemcpyAsync(part.data(), d_part, nb * sizeof(double), hipMemcpyDeviceToHost, h->st) != hipSuccess ||
+        hipStreamSynchronize(h->st) != hipSuccess) {
+        *rc = fail(DS2D_EDEVICE, "dot product copy failed");
+        return 0;
+    }
+    double s = 0;
+    for (double v : part) s += v;
+    return s;
+}
+
+void stats_begin(ds2d_ctx* h) { (void)hipEventRecord(h->ev[0], h->st); }
+void stats_loop_begin(ds2d_ctx* h) { (void)hipEventRecord(h->ev[1], h->st); }
+void stats_loop_end(ds2d_ctx* h) { (void)hipEventRecord(h->ev[2], h->st); }
+int stats_end(ds2d_ctx* h, ds2d_stats* out, long long cell_updates, long long launches, double alg_bytes) {
+    HIP_TRY(hipEventRecord(h->ev[3], h->st));
+    HIP_TRY(hipEventSynchronize(h->ev[3]));
+    float a = 0, b = 0;
+    HIP_TRY(hipEventElapsedTime(&a, h->ev[0], h->ev[3]));
+    HIP_TRY(hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
+    h->last.elapsed_ms = a;
+    h->last.stencil_ms = b;
+    h->last.cell_updates = cell_updates;
+    h->last.stencil_launches = launches;
+    h->last.alg_bytes = alg_bytes;
+    if (out) *out = h->last;
+    return 0;
+}
+
+int check_h(ds2d_ctx* h) {
+    if (!h) return fail(DS2D_EINVAL, "null handle");
+    HIP_TRY(hipSetDevice(0));
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char* ds2d_last_error(void) { return g_err.c_str(); }
+
+int ds2d_device_info(char* name, int len, int* n_cu, double* hbm_gb) {
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (n < 1) return fail(DS2D_EDEVICE, "no HIP device");
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, 0));
+    if (name && len > 0) {
+        snprintf(name, len, "%s", p.gcnArchName);
+    }
+    if (n_cu) *n_cu = p.multiProcessorCount;
+    if (hbm_gb) *hbm_gb = (double)p.totalGlobalMem / 1e9;
+    return 0;
+}
+
+int ds2d_create(const ds2d_geom* g, const float* wavelet, ds2d_t** out) {
+    if (!g || !wavelet || !out) return fail(DS2D_EINVAL, "null argument");
+    if (g->nop != NOP) return fail(DS2D_EINVAL, "nop must be %d (8th-order coefficients), got %d", NOP, g->nop);
+    if (g->nt <= 0 || g->allnx <= 0 || g->allnz <= 0 || g->pml < NOP || g->dx <= 0 || g->dz <= 0 || g->dt <= 0)
+        return fail(DS2D_EINVAL, "bad geometry");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (ndev < 1) return fail(DS2D_EDEVICE, "no HIP device");
+    HIP_TRY(hipSetDevice(0));
+    ds2d_ctx* h = new ds2d_ctx();
+    h->g = *g;
+    if (h->g.scale <= 0) h->g.scale = 1;
+    h->wav.assign(wavelet, wavelet + g->nt);
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(DS2D_EDEVICE, "stream creation failed");
+    }
+    for (auto& e : h->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete h;
+            return fail(DS2D_EDEVICE, "event creation failed");
+        }
+    const size_t A = (size_t)g->allnx * g->allnz;
+    if (hipMalloc(&h->d_wav, g->nt * sizeof(float)) != hipSuccess || hipMalloc(&h->d_vp, A * 4) != hipSuccess ||
+        hipMalloc(&h->d_vs, A * 4) != hipSuccess || hipMalloc(&h->d_rho, A * 4) != hipSuccess ||
+        hipMalloc(&h->d_ipp, A * 4) != hipSuccess || hipMalloc(&h->d_ips, A * 4) != hipSuccess ||
+        hipMalloc(&h->d_grad7, 7 * A * 4) != hipSuccess) {
+        delete h;
+        return fail(DS2D_ENOMEM, "model allocation failed");
+    }
+    (void)hipMemcpy(h->d_wav, wavelet, g->nt * sizeof(float), hipMemcpyHostToDevice);
+    (void)hipMemset(h->d_grad7, 0, 7 * A * 4);
+    (void)hipMemset(h->d_ipp, 0, A * 4);
+    (void)hipMemset(h->d_ips, 0, A * 4);
+    *out = h;
+    return 0;
+}
+
+int ds2d_destroy(ds2d_t* h) {
+    if (!h) return 0;
+    (void)hipStreamSynchronize(h->st);
+    delete h;
+    return 0;
+}
+
+int ds2d_set_model(ds2d_t* h, const float* vp, const float* vs, const float* rho) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!vp) return fail(DS2D_EINVAL, "vp is required");
+    const size_t A = (size_t)h->g.allnx * h->g.allnz * 4;
+    HIP_TRY(hipMemcpyAsync(h->d_vp, vp, A, hipMemcpyHostToDevice, h->st));
+    if (vs) HIP_TRY(hipMemcpyAsync(h->d_vs, vs, A, hipMemcpyHostToDevice, h->st));
+    else HIP_TRY(hipMemsetAsync(h->d_vs, 0, A, h->st));
+    if (rho) HIP_TRY(hipMemcpyAsync(h->d_rho, rho, A, hipMemcpyHostToDevice, h->st));
+    else if (h->g.rho_const <= 0) return fail(DS2D_EINVAL, "rho array required when rho_const <= 0");
+    HIP_TRY(hipStreamSynchronize(h->st));
+    h->have_model = true;
+    return 0;
+}
+
+int ds2d_set_image(ds2d_t* h, const float* pp, const float* ps) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    const size_t A = (size_t)h->g.allnx * h->g.allnz * 4;
+    if (pp) HIP_TRY(hipMemcpyAsync(h->d_ipp, pp, A, hipMemcpyHostToDevice, h->st));
+    else HIP_TRY(hipMemsetAsync(h->d_ipp, 0, A, h->st));
+    if (ps) HIP_TRY(hipMemcpyAsync(h->d_ips, ps, A, hipMemcpyHostToDevice, h->st));
+    else HIP_TRY(hipMemsetAsync(h->d_ips, 0, A, h->st));
+    HIP_TRY(hipStreamSynchronize(h->st));
+    h->have_image = true;
+    return 0;
+}
+
+int ds2d_set_shot(ds2d_t* h, const ds2d_shot* s, const float* rec_z, const float* rec_x) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!s) return fail(DS2D_EINVAL, "null shot");
+    h->cur = *s;
+    h->cur_rix.assign(s->rec_ix, s->rec_ix + s->ntr);
+    h->cur.rec_ix = h->cur_rix.data();
+    const size_t nr = (size_t)s->ntr * h->g.nt;
+    if (rec_z) h->cur_oz.assign(rec_z, rec_z + nr); else h->cur_oz.assign(nr, 0.f);
+    if (rec_x) h->cur_ox.assign(rec_x, rec_x + nr); else h->cur_ox.assign(nr, 0.f);
+    h->have_shot = true;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Reference ELSRTM gradient, batched (FD2DGPU.cu:4184-4879 per slot)
+// ---------------------------------------------------------------------------
+int ds2d_batch_gradient(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* obs_z, const float* obs_x,
+                        int iter, double* misfit) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!h->have_model) return fail(DS2D_ESTATE, "no model");
+    if (h->g.physics != 0) return fail(DS2D_EINVAL, "the reference gradient is elastic");
+    if (!obs_z || !obs_x) return fail(DS2D_EINVAL, "observed records required");
+    Batch B;
+    if ((rc = build_batch(h, nshots, shots, B))) return rc;
+    const BatchShape& sh = B.shape;
+    const long long n = B.total_cells;
+    hipStream_t st = h->st;
+    EField bg, b, b2, bk, bb;
+    Med m;
+    Strain sg, sb;
+    if ((rc = alloc_efield(h, "bg", n, bg)) || (rc = alloc_efield(h, "b", n, b)) ||
+        (rc = alloc_efield(h, "b2", n, b2)) || (rc = alloc_efield(h, "bk", n, bk)) ||
+        (rc = alloc_efield(h, "bb", n, bb)) || (rc = alloc_med(h, n, m)))
+        return rc;
+    float* strain;
+    if ((rc = h->fbuf("strain", (size_t)n * 8, &strain))) return rc;
+    HIP_TRY(hipMemsetAsync(strain, 0, (size_t)n * 8 * 4, st));
+    sg = Strain{strain, strain + n, strain + 2 * n, strain + 3 * n};
+    sb = Strain{strain + 4 * n, strain + 5 * n, strain + 6 * n, strain + 7 * n};
+    const size_t R = std::max<long long>(B.total_rec, 1);
+    float *rs, *obs;
+    if ((rc = h->fbuf("recs", R * 4, &rs)) || (rc = h->fbuf("obs", R * 2, &obs))) return rc;
+    float *rs_z = rs, *rs_x = rs + R, *rs2_z = rs + 2 * R, *rs2_x = rs + 3 * R, *ob_z = obs, *ob_x = obs + R;
+    HIP_TRY(hipMemsetAsync(rs, 0, R * 4 * 4, st));
+    HIP_TRY(hipMemcpyAsync(ob_z, obs_z, B.total_rec * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ob_x, obs_x, B.total_rec * 4, hipMemcpyHostToDevice, st));
+    const long long W = std::max<long long>(B.total_win, 1);
+    float* win;
+    if ((rc = h->fbuf("win7", (size_t)W * 7, &win))) return rc;
+    HIP_TRY(hipMemsetAsync(win, 0, (size_t)W * 7 * 4, st));
+    float *e_bvx = nullptr, *e_bvz = nullptr, *e_cvx = nullptr, *e_cvz = nullptr;
+    if (B.any_pml) {
+        float* e;
+        if ((rc = h->fbuf("edges", (size_t)B.total_edge * 4, &e))) return rc;
+        e_bvx = e; e_bvz = e + B.total_edge; e_cvx = e + 2 * B.total_edge; e_cvz = e + 3 * B.total_edge;
+    }
+    launch_setup(st, sh, setup_args(h, true), m);
+    const int nt = h->g.nt;
+    for (int it = 0; it < nt; ++it) {
+        launch_inject_src(st, sh, bg.txx, bg.tzz, h->d_wav, it);
+        launch_ref_v(st, sh, bg, m, 1.f, 0);
+        launch_ref_s(st, sh, bg, m, sg, 1.f, 0);
+        launch_born_src(st, sh, bg.vx, bg.vz, b, m, 1.f);
+        launch_ref_v(st, sh, b, m, 1.f, 0);
+        launch_ref_s(st, sh, b, m, sg, 1.f, 0);
+        launch_born_src(st, sh, b.vx, b.vz, b2, m, 1.f);
+        launch_ref_v(st, sh, b2, m, 1.f, 0);
+        launch_ref_s(st, sh, b2, m, sg, 1.f, 0);
+        launch_record(st, sh, b.vz, b.vx, rs_z, rs_x, b2.vz, b2.vx, rs2_z, rs2_x, it);
+        if (B.any_pml) {
+            launch_edge_io(st, sh, bg.vx, bg.vz, e_bvx, e_bvz, it, 1);
+            launch_edge_io(st, sh, b.vx, b.vz, e_cvx, e_cvz, it, 1);
+        }
+    }
+    // scatter_add (FD2DGPU.cu:3143-3149): s = 1*s + s2
+    launch_axpby(st, rs_z, rs2_z, B.total_rec, 1.f, 1.f);
+    launch_axpby(st, rs_x, rs2_x, B.total_rec, 1.f, 1.f);
+    // NCC (FD2DGPU.cu:4532-4673) on the device, misfit finished on the host
+    double* sums;
+    if ((rc = h->fbuf("nccsum", (size_t)std::max(B.total_traces, 1) * 6, &sums))) return rc;
+    int* d_tb;
+    if ((rc = h->fbuf("tbase", B.tbase.size(), &d_tb))) return rc;
+    launch_ncc_sums(st, sh, rs_z, rs_x, ob_z, ob_x, sums, d_tb);
+    std::vector<double> hs((size_t)B.total_traces * 6);
+    if (!hs.empty()) HIP_TRY(hipMemcpyAsync(hs.data(), sums, hs.size() * 8, hipMemcpyDeviceToHost, st));
+    launch_ncc_apply(st, sh, rs_z, rs_x, ob_z, ob_x, sums, d_tb, iter);
+    HIP_TRY(hipStreamSynchronize(st));
+    if (misfit && iter != 0) {
+        for (int s = 0; s < nshots; ++s) {
+            const int ntr = shots[s].ntr;
+            for (int r = 0; r < ntr; ++r) {
+                const double* q = &hs[(size_t)(B.tbase[s] + r) * 6];
+                double tx = (q[0] > 0 && q[1] > 0) ? q[2] / (std::sqrt(q[0]) * std::sqrt(q[1])) : 0.0;
+                double tz = (q[3] > 0 && q[4] > 0) ? q[5] / (std::sqrt(q[3]) * std::sqrt(q[4])) : 0.0;
+                *misfit -= (double)(1.0f / (2 * ntr)) * (tx + tz);
+            }
+        }
+    }
+    for (int it = nt - 1; it >= 0; --it) {
+        if (B.any_pml) {
+            launch_edge_io(st, sh, bg.vx, bg.vz, e_bvx, e_bvz, it, 0);
+            launch_edge_io(st, sh, b.vx, b.vz, e_cvx, e_cvz, it, 0);
+        }
+        launch_ref_s(st, sh, bg, m, sg, -1.f, 1);
+        launch_ref_v(st, sh, bg, m, -1.f, 1);
+        launch_ref_s(st, sh, b, m, sb, -1.f, 1);
+        launch_born_src(st, sh, bg.vx, bg.vz, b, m, 1.f);
+        launch_ref_v(st, sh, b, m, -1.f, 1);
+        launch_inject_res(st, sh, bk.vz, bk.vx, rs_z, rs_x, it);
+        launch_adj_s(st, sh, bk, m, -1.f);
+        launch_adj_v(st, sh, bk, m, -1.f);
+        launch_adj_v(st, sh, bb, m, -1.f);
+        launch_born_adj(st, sh, bk, bb, m, -1.f);
+        launch_adj_s(st, sh, bb, m, -1.f);
+        launch_image(st, sh, sg, sb, bk, bb, bg, m, win, W);
+    }
+    const long long A = (long long)h->g.allnx * h->g.allnz;
+    launch_gather(st, sh, win, W, h->d_grad7, A, 7, h->g.allnx, h->g.allnz);
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int ds2d_shot_gradient(ds2d_t* h, int iter, double* misfit) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!h->have_shot) return fail(DS2D_ESTATE, "no shot set");
+    return ds2d_batch_gradient(h, 1, &h->cur, h->cur_oz.data(), h->cur_ox.data(), iter, misfit);
+}
+
+int ds2d_take_gradients(ds2d_t* h, float* pp, float* ps, float* il, float* bpp, float* bps, float* b2pp,
+                        float* b2ps) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    const size_t A = (size_t)h->g.allnx * h->g.allnz;
+    float* outs[7] = {pp, ps, il, bpp, bps, b2pp, b2ps};
+    for (int k = 0; k < 7; ++k)
+        if (outs[k]) HIP_TRY(hipMemcpyAsync(outs[k], h->d_grad7 + k * A, A * 4, hipMemcpyDeviceToHost, h->st));
+    HIP_TRY(hipMemsetAsync(h->d_grad7, 0, 7 * A * 4, h->st));
+    HIP_TRY(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Elastic forward modelling
+// ---------------------------------------------------------------------------
+int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec_z, float* rec_x, int mode,
+                       ds2d_stats* stats) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!h->have_model) return fail(DS2D_ESTATE, "no model");
+    if (h->g.physics != 0) return fail(DS2D_EINVAL, "elastic forward needs physics = 0");
+    Batch B;
+    if ((rc = build_batch(h, nshots, shots, B))) return rc;
+    const BatchShape& sh = B.shape;
+    const long long n = B.total_cells;
+    hipStream_t st = h->st;
+    Med m;
+    if ((rc = alloc_med(h, n, m))) return rc;
+    const size_t R = std::max<long long>(B.total_rec, 1);
+    float* rs;
+    if ((rc = h->fbuf("frec", R * 2, &rs))) return rc;
+    float *rz = rs, *rx = rs + R;
+    EField f;
+    if ((rc = alloc_efield(h, "bg", n, f, false))) return rc;
+    const int nt = h->g.nt;
+    long long cu = 0;
+    for (auto& S : B.slots) cu += (long long)S.nxp * S.nzp;
+    cu *= nt;
+    stats_begin(h);
+    launch_setup(st, sh, setup_args(h, false), m);
+    HIP_TRY(hipMemsetAsync(f.vx, 0, (size_t)n * 15 * 4, st));
+    double bytes = 0;
+    long long launches = 0;
+    if (mode == 0) {
+        Strain none{nullptr, nullptr, nullptr, nullptr};
+        stats_loop_begin(h);
+        for (int it = 0; it < nt; ++it) {
+            launch_inject_src(st, sh, f.txx, f.tzz, h->d_wav, it);
+            launch_ref_v(st, sh, f, m, 1.f, 0);
+            launch_ref_s(st, sh, f, m, none, 1.f, 0);
+            launch_record(st, sh, f.vz, f.vx, rz, rx, nullptr, nullptr, nullptr, nullptr, it);
+        }
+        stats_loop_end(h);
+        launches = (long long)nt;
+        bytes = (double)n * 21 * 4;  // split-layout stress kernel: 12 read + 9 written per cell
+    } else {
+        float* par;
+        int* rmap;
+        if ((rc = h->fbuf("fpar", (size_t)n * 3, &par)) || (rc = h->fbuf("rmap", B.total_rmap, &rmap))) return rc;
+        launch_fast_params(st, m.rho, m.lam, m.mu, par, par + n, par + 2 * n, n, h->g.dt);
+        HIP_TRY(hipMemsetAsync(rmap, 0xff, B.total_rmap * 4, st));
+        launch_fast_rmap(st, sh, rmap);
+        FastArgs a{};
+        a.vx = f.vx; a.vz = f.vz; a.txx = f.txx; a.tzz = f.tzz; a.txz = f.txz;
+        a.vx_x = f.vx_x; a.vx_z = f.vx_z; a.vz_x = f.vz_x; a.vz_z = f.vz_z;
+        a.txx_x = f.txx_x; a.txx_z = f.txx_z; a.tzz_x = f.tzz_x; a.tzz_z = f.tzz_z; a.txz_x = f.txz_x;
+        a.txz_z = f.txz_z;
+        a.bx = par; a.lam = par + n; a.mu = par + 2 * n;
+        a.dampx = m.dampx; a.dampz = m.dampz;
+        a.wav = h->d_wav;
+        a.rec_z = rz; a.rec_x = rx;
+        stats_loop_begin(h);
+        for (int it = 0; it < nt; ++it) {
+            launch_fast_v(st, sh, a, rmap, it);
+            launch_fast_s(st, sh, a);
+        }
+        stats_loop_end(h);
+        launches = (long long)nt;
+        bytes = (double)n * 10 * 4;  // unsplit stress kernel: 7 read + 3 written per cell
+    }
+    if ((rc = stats_end(h, stats, cu, launches, bytes))) return rc;
+    if (rec_z) HIP_TRY(hipMemcpyAsync(rec_z, rz, B.total_rec * 4, hipMemcpyDeviceToHost, st));
+    if (rec_x) HIP_TRY(hipMemcpyAsync(rec_x, rx, B.total_rec * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int ds2d_forward(ds2d_t* h, float* rec_z, float* rec_x) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!h->have_shot) return fail(DS2D_ESTATE, "no shot set");
+    return ds2d_batch_forward(h, 1, &h->cur, rec_z, rec_x, 1, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Reference acoustic forward (FD() of FD2DGPU.cu:554-584)
+// ---------------------------------------------------------------------------
+int ds2d_acoustic_ref_forward(ds2d_t* h, int sxnum, int ngx_min, float* rec_p) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!h->have_model) return fail(DS2D_ESTATE, "no model");
+    const ds2d_geom& g = h->g;
+    const int nx = g.nxpml - 2 * g.pml;
+    if (sxnum < 0 || sxnum >= nx) return fail(DS2D_EINVAL, "sxnum %d outside window of %d", sxnum, nx);
+    int rix = NOP;
+    ds2d_shot s{};
+    s.ngx_min = ngx_min;
+    s.nx = nx;
+    s.isx = g.pml + sxnum;
+    s.isz = g.pml;
+    s.igz = g.pml;
+    s.ntr = 0;
+    s.rec_ix = &rix;
+    s.boundary = 0;
+    Batch B;
+    if ((rc = build_batch(h, 1, &s, B))) return rc;
+    const BatchShape& sh = B.shape;
+    const long long n = B.total_cells;
+    Med m;
+    if ((rc = alloc_med(h, n, m))) return rc;
+    float* ab;
+    if ((rc = h->fbuf("aref", (size_t)n * 9, &ab))) return rc;
+    HIP_TRY(hipMemsetAsync(ab, 0, (size_t)n * 9 * 4, h->st));
+    float *px = ab, *pz = ab + n, *p = ab + 2 * n, *vx = ab + 3 * n, *vz = ab + 4 * n, *bu = ab + 5 * n,
+          *kk = ab + 6 * n, *refl = ab + 7 * n, *damp = ab + 8 * n;
+    SetupArgs a = setup_args(h, false);
+    a.pml_rr = g.pml_rr > 0 ? g.pml_rr : 1e-6f;
+    launch_setup_acoustic(h->st, sh, a, m, bu, kk, refl);
+    launch_combine_damp(h->st, sh, m.dampx, m.dampz, damp);
+    float* drec;
+    if ((rc = h->fbuf("arec", (size_t)nx * g.nt, &drec))) return rc;
+    for (int it = 0; it < g.nt; ++it)
+        launch_acou_ref_step(h->st, sh, px, pz, p, vx, vz, kk, m.rho, damp, h->d_wav, sxnum, drec, it);
+    HIP_TRY(hipMemcpyAsync(rec_p, drec, (size_t)nx * g.nt * 4, hipMemcpyDeviceToHost, h->st));
+    HIP_TRY(hipStreamSynchronize(h->st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Plain-L2 acoustic Born operator, adjoint and CG
+// ---------------------------------------------------------------------------
+namespace {
+
+struct AcouWS {
+    AcouField bg, bn, lm;
+    AcouMed m;
+    float *grad_pad = nullptr;  // per-slot padded gradient (n)
+    float *win = nullptr;       // per-slot window images
+    float *rec = nullptr;       // R floats: bg records
+    float *rec2 = nullptr;      // R floats: born records / residual
+    long long n = 0, R = 0, W = 0;
+};
+
+int acou_prepare(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouWS& w) {
+    int rc;
+    const long long n = B.total_cells;
+    w.n = n;
+    w.R = std::max<long long>(B.total_rec, 1);
+    w.W = std::max<long long>(B.total_win, 1);
+    float* base;
+    if ((rc = h->fbuf("acou", (size_t)n * 13, &base))) return rc;
+    w.bg = AcouField{base, base + n, base + 2 * n};
+    w.bn = AcouField{base + 3 * n, base + 4 * n, base + 5 * n};
+    w.lm = AcouField{base + 6 * n, base + 7 * n, base + 8 * n};
+    w.grad_pad = base + 9 * n;
+    float *bu = base + 10 * n, *kk = base + 11 * n, *refl = base + 12 * n;
+    Med m;
+    if ((rc = alloc_med(h, n, m))) return rc;
+    SetupArgs a = setup_args(h, false);
+    a.img_pp = refl_glob_dev;
+    launch_setup_acoustic(h->st, B.shape, a, m, bu, kk, refl);
+    w.m = AcouMed{bu, kk, refl, m.dampx, m.rho};
+    if ((rc = h->fbuf("acrec", (size_t)w.R * 2, &w.rec))) return rc;
+    w.rec2 = w.rec + w.R;
+    if ((rc = h->fbuf("acwin", (size_t)w.W, &w.win))) return rc;
+    return 0;
+}
+
+// Re-window a new reflectivity into the existing workspace (zero frame).
+int acou_set_refl(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouWS& w) {
+    Med m;
+    int rc;
+    if ((rc = alloc_med(h, w.n, m))) return rc;
+    SetupArgs a = setup_args(h, false);
+    a.img_pp = refl_glob_dev;
+    launch_setup_acoustic(h->st, B.shape, a, m, (float*)w.m.b, (float*)w.m.k, (float*)w.m.m);
+    return 0;
+}
+
+// Background + Born forward from rest; records in w.rec (bg) / w.rec2 (born).
+void acou_fwd(ds2d_ctx* h, const Batch& B, AcouWS& w, bool with_born, bool rec_bg) {
+    hipStream_t st = h->st;
+    (void)hipMemsetAsync(w.bg.vx, 0, (size_t)w.n * 6 * 4, st);  // bg + bn
+    for (int it = 0; it < h->g.nt; ++it) {
+        launch_acou_v(st, B.shape, w.bg, w.bn, w.m, h->d_wav, it, 1.f, with_born);
+        launch_acou_p(st, B.shape, w.bg, w.bn, w.m, rec_bg ? w.rec : nullptr, with_born ? w.rec2 : nullptr, it,
+                      with_born);
+    }
+}
+
+// Backward pass from the background state left by acou_fwd: residual in
+// w.rec2 (device), gradient accumulated in w.grad_pad (zeroed here).
+void acou_bwd(ds2d_ctx* h, const Batch& B, AcouWS& w) {
+    hipStream_t st = h->st;
+    (void)hipMemsetAsync(w.lm.vx, 0, (size_t)w.n * 3 * 4, st);
+    (void)hipMemsetAsync(w.grad_pad, 0, (size_t)w.n * 4, st);
+    for (int it = h->g.nt - 1; it >= 0; --it) {
+        launch_acou_adj_p(st, B.shape, w.bg, w.lm, w.m, w.grad_pad, w.rec2, it);
+        launch_acou_adj_v(st, B.shape, w.bg, w.lm, w.m, h->d_wav, it);
+    }
+}
+
+// Padded per-slot gradient -> global (deterministic slot order), overwrite.
+int acou_grad_to_global(ds2d_ctx* h, const Batch& B, AcouWS& w, float* d_glob) {
+    // copy interior of each slot into the window arena, then gather
+    for (size_t s = 0; s < B.slots.size(); ++s) {
+        const SlotDesc& S = B.slots[s];
+        HIP_TRY(hipMemcpy2DAsync(w.win + S.win_off, (size_t)S.nz * 4,
+                                 w.grad_pad + S.off + (size_t)S.pml * S.nzp + S.pml, (size_t)S.nzp * 4,
+                                 (size_t)S.nz * 4, S.nx, hipMemcpyDeviceToDevice, h->st));
+    }
+    const long long A = (long long)h->g.allnx * h->g.allnz;
+    HIP_TRY(hipMemsetAsync(d_glob, 0, A * 4, h->st));
+    launch_gather(h->st, B.shape, w.win, w.W, d_glob, A, 1, h->g.allnx, h->g.allnz);
+    return 0;
+}
+
+int acou_begin(ds2d_ctx* h, int nshots, const ds2d_shot* shots, Batch& B) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!h->have_model) return fail(DS2D_ESTATE, "no model");
+    for (int s = 0; s < nshots; ++s)
+        if (shots[s].boundary != 1)
+            return fail(DS2D_EINVAL, "plain Born/adjoint needs the random boundary (shot %d)", s);
+    return build_batch(h, nshots, shots, B);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ds2d_born_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* refl, float* rec_bg,
+                      float* rec_born, ds2d_stats* stats) {
+    int rc;
+    Batch B;
+    if ((rc = acou_begin(h, nshots, shots, B))) return rc;
+    const size_t A = (size_t)h->g.allnx * h->g.allnz;
+    float* d_refl;
+    if ((rc = h->fbuf("refl", A, &d_refl))) return rc;
+    HIP_TRY(hipMemcpyAsync(d_refl, refl, A * 4, hipMemcpyHostToDevice, h->st));
+    AcouWS w;
+    if ((rc = acou_prepare(h, B, d_refl, w))) return rc;
+    stats_begin(h);
+    stats_loop_begin(h);
+    acou_fwd(h, B, w, true, true);
+    stats_loop_end(h);
+    long long cu = 0;
+    for (auto& S : B.slots) cu += (long long)S.nxp * S.nzp;
+    if ((rc = stats_end(h, stats, cu * h->g.nt * 2, h->g.nt, (double)B.total_cells * 10 * 4))) return rc;
+    if (rec_bg) HIP_TRY(hipMemcpyAsync(rec_bg, w.rec, B.total_rec * 4, hipMemcpyDeviceToHost, h->st));
+    if (rec_born) HIP_TRY(hipMemcpyAsync(rec_born, w.rec2, B.total_rec * 4, hipMemcpyDeviceToHost, h->st));
+    HIP_TRY(hipStreamSynchronize(h->st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int ds2d_born_adjoint(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* res, float* grad,
+                      ds2d_stats* stats) {
+    int rc;
+    Batch B;
+    if ((rc = acou_begin(h, nshots, shots, B))) return rc;
+    AcouWS w;
+    if ((rc = acou_prepare(h, B, nullptr, w))) return rc;
+    stats_begin(h);
+    stats_loop_begin(h);
+    acou_fwd(h, B, w, false, false);
+    HIP_TRY(hipMemcpyAsync(w.rec2, res, B.total_rec * 4, hipMemcpyHostToDevice, h->st));
+    acou_bwd(h, B, w);
+    stats_loop_end(h);
+    const size_t A = (size_t)h->g.allnx * h->g.allnz;
+    float* d_g;
+    if ((rc = h->fbuf("gglob", A, &d_g))) return rc;
+    if ((rc = acou_grad_to_global(h, B, w, d_g))) return rc;
+    long long cu = 0;
+    for (auto& S : B.slots) cu += (long long)S.nxp * S.nzp;
+    if ((rc = stats_end(h, stats, cu * h->g.nt * 3, 2LL * h->g.nt, (double)B.total_cells * 10 * 4))) return rc;
+    HIP_TRY(hipMemcpyAsync(grad, d_g, A * 4, hipMemcpyDeviceToHost, h->st));
+    HIP_TRY(hipStreamSynchronize(h->st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// CGLS on ||L m - d||^2 (Hestenes-Stiefel form).  Each iteration propagates the
+// background with the Born field (q = L p) and then runs the adjoint
+// backward from that background state (s = L^T r).
+int ds2d_lsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_obs, int niter, float* image,
+                  double* resid, ds2d_stats* stats) {
+    int rc;
+    Batch B;
+    if ((rc = acou_begin(h, nshots, shots, B))) return rc;
+    if (niter < 0) return fail(DS2D_EINVAL, "niter < 0");
+    const long long A = (long long)h->g.allnx * h->g.allnz;
+    hipStream_t st = h->st;
+    float *x, *p, *s, *d, *r, *q;
+    if ((rc = h->fbuf("cg_x", A, &x)) || (rc = h->fbuf("cg_p", A, &p)) || (rc = h->fbuf("cg_s", A, &s)))
+        return rc;
+    const long long R = std::max<long long>(B.total_rec, 1);
+    if ((rc = h->fbuf("cg_d", R, &d)) || (rc = h->fbuf("cg_r", R, &r)) || (rc = h->fbuf("cg_q", R, &q))) return rc;
+    HIP_TRY(hipMemcpyAsync(x, image, A * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d, d_obs, B.total_rec * 4, hipMemcpyHostToDevice, st));
+    AcouWS w;
+    if ((rc = acou_prepare(h, B, x, w))) return rc;
+    stats_begin(h);
+    stats_loop_begin(h);
+    // r = d - L x
+    acou_fwd(h, B, w, true, false);
+    HIP_TRY(hipMemcpyAsync(r, d, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
+    launch_axpby(st, r, w.rec2, B.total_rec, -1.f, 1.f);
+    const double dd = host_dot(h, d, d, B.total_rec, &rc);
+    if (rc) return rc;
+    if (dd <= 0) return fail(DS2D_EINVAL, "observed data are all zero");
+    double rr = host_dot(h, r, r, B.total_rec, &rc);
+    if (rc) return rc;
+    if (resid) resid[0] = std::sqrt(rr / dd);
+    // s = L^T r (background state is at t = nt after the forward above)
+    HIP_TRY(hipMemcpyAsync(w.rec2, r, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
+    acou_bwd(h, B, w);
+    if ((rc = acou_grad_to_global(h, B, w, s))) return rc;
+    HIP_TRY(hipMemcpyAsync(p, s, A * 4, hipMemcpyDeviceToDevice, st));
+    double gamma = host_dot(h, s, s, A, &rc);
+    if (rc) return rc;
+    long long props = 3;
+    for (int k = 0; k < niter; ++k) {
+        if (gamma <= 0) {
+            if (resid) for (int j = k; j < niter; ++j) resid[j + 1] = std::sqrt(rr / dd);
+            break;
+        }
+        // q = L p
+        if ((rc = acou_set_refl(h, B, p, w))) return rc;
+        acou_fwd(h, B, w, true, false);
+        HIP_TRY(hipMemcpyAsync(q, w.rec2, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
+        const double qq = host_dot(h, q, q, B.total_rec, &rc);
+        if (rc) return rc;
+        const double alpha = qq > 0 ? gamma / qq : 0.0;
+        launch_axpby(st, x, p, A, (float)alpha, 1.f);
+        launch_axpby(st, r, q, B.total_rec, (float)-alpha, 1.f);
+        rr = host_dot(h, r, r, B.total_rec, &rc);
+        if (rc) return rc;
+        if (resid) resid[k + 1] = std::sqrt(rr / dd);
+        // s = L^T r
+        HIP_TRY(hipMemcpyAsync(w.rec2, r, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
+        acou_bwd(h, B, w);
+        if ((rc = acou_grad_to_global(h, B, w, s))) return rc;
+        const double gnew = host_dot(h, s, s, A, &rc);
+        if (rc) return rc;
+        const double beta = gamma > 0 ? gnew / gamma : 0.0;
+        launch_axpby(st, p, s, A, 1.f, (float)beta);  // p = s + beta p
+        gamma = gnew;
+        props += 4;
+    }
+    stats_loop_end(h);
+    long long cu = 0;
+    for (auto& S : B.slots) cu += (long long)S.nxp * S.nzp;
+    if ((rc = stats_end(h, stats, cu * h->g.nt * props, 0, 0))) return rc;
+    HIP_TRY(hipMemcpyAsync(image, x, A * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// laplace.cpp:82-106: (1/dx^2)(5-point Laplacian) on [2, n-2), zero frame.
+int ds2d_laplace_filter(int nz, int nx, int order, float dx, float* dat) {
+    (void)order;
+    if (!dat || nz <= 0 || nx <= 0 || dx == 0) return fail(DS2D_EINVAL, "bad laplace arguments");
+    std::vector<float> t1(dat, dat + (size_t)nx * nz), t2((size_t)nx * nz, 0.f);
+    for (int ix = 2; ix < nx - 2; ++ix)
+        for (int iz = 2; iz < nz - 2; ++iz) {
+            const size_t c = (size_t)ix * nz + iz;
+            t2[c] = (float)((1.0 / dx / dx) * (t1[c + nz] + t1[c - nz] - 2 * t1[c] + t1[c + 1] + t1[c - 1] - 2 * t1[c]));
+        }
+    std::memcpy(dat, t2.data(), t2.size() * 4);
+    return 0;
+}
+
+// laplace.cpp:3-57 band filter with its stated intent (padded stride nz+4,
+// zero frame): out = Sx(orig - (Sx o Sz)^4 orig), S = (1/4, 1/2, 1/4).
+int ds2d_laplace_band(int nz, int nx, float* dat) {
+    if (!dat || nz <= 0 || nx <= 0) return fail(DS2D_EINVAL, "bad laplace arguments");
+    const int s = nz + 4;
+    std::vector<float> t0((size_t)(nx + 4) * s, 0.f), t1(t0.size(), 0.f), t2(t0.size(), 0.f);
+    auto I = [&](int ix, int iz) { return (size_t)(ix + 2) * s + (iz + 2); };
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iz = 0; iz < nz; ++iz) t0[I(ix, iz)] = t1[I(ix, iz)] = dat[(size_t)ix * nz + iz];
+    for (int k = 0; k < 4; ++k) {
+        for (int ix = 0; ix < nx; ++ix)
+            for (int iz = 0; iz < nz; ++iz)
+                t2[I(ix, iz)] = t1[I(ix, iz)] * 0.5f + t1[I(ix, iz - 1)] * 0.25f + t1[I(ix, iz + 1)] * 0.25f;
+        for (int ix = 0; ix < nx; ++ix)
+            for (int iz = 0; iz < nz; ++iz)
+                t1[I(ix, iz)] = t2[I(ix, iz)] * 0.5f + t2[I(ix - 1, iz)] * 0.25f + t2[I(ix + 1, iz)] * 0.25f;
+    }
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iz = 0; iz < nz; ++iz) t2[I(ix, iz)] = t0[I(ix, iz)] - t1[I(ix, iz)];
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iz = 0; iz < nz; ++iz)
+            t1[I(ix, iz)] = t2[I(ix, iz)] * 0.5f + t2[I(ix - 1, iz)] * 0.25f + t2[I(ix + 1, iz)] * 0.25f;
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iz = 0; iz < nz; ++iz) dat[(size_t)ix * nz + iz] = t1[I(ix, iz)];
+    return 0;
+}
+
+// main.cpp:461-473 ricker1.
+int ds2d_ricker(int nt, float f, float dt, float* s) {
+    if (!s || nt <= 0 || f <= 0 || dt <= 0) return fail(DS2D_EINVAL, "bad ricker arguments");
+    const float pi = 3.14159265358979f;
+    const float t0 = 1 / f;
+    const int kt = (int)(t0 / dt);
+    for (int i = 0; i < nt; ++i) {
+        const float tt = i * dt - kt * dt;
+        const float sp = pi * f * tt;
+        s[i] = (float)(1.0 * std::exp(-sp * sp) * (1. - 2. * sp * sp));
+    }
+    return 0;
+}
+
+}  // extern "C"

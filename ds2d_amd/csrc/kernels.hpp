@@ -1,0 +1,103 @@
+// Host-side launchers of the DS2D gfx950 kernels.
+#pragma once
+#include "common.hpp"
+
+namespace ds2d {
+
+// Geometry of one launch batch (device pointers to the slot table and the
+// receiver-column arena; scalar sizes for the grids).
+struct BatchShape {
+    const SlotDesc* slots;  // device
+    const int* rix;         // device, receiver columns (padded)
+    int nslots, max_nxp, nzp, nz, max_ntr, nt;
+    long long total_cells;  // sum of nxp*nzp
+    float dt, dx, dz;
+};
+
+// --- reference-compatible elastic kernels (kernels_ref.hip) ---------------
+void launch_ref_v(hipStream_t st, const BatchShape& b, EField f, Med m, float dir, int recon);
+void launch_ref_s(hipStream_t st, const BatchShape& b, EField f, Med m, Strain s, float dir, int recon);
+void launch_adj_v(hipStream_t st, const BatchShape& b, EField f, Med m, float dir);
+void launch_adj_s(hipStream_t st, const BatchShape& b, EField f, Med m, float dir);
+void launch_born_src(hipStream_t st, const BatchShape& b, const float* vx, const float* vz, EField tgt, Med m,
+                     float dir);
+void launch_born_adj(hipStream_t st, const BatchShape& b, EField src, EField tgt, Med m, float dir);
+void launch_inject_src(hipStream_t st, const BatchShape& b, float* txx, float* tzz, const float* wav, int it);
+void launch_record(hipStream_t st, const BatchShape& b, const float* vz, const float* vx, float* rz, float* rx,
+                   const float* vz2, const float* vx2, float* rz2, float* rx2, int it);
+void launch_inject_res(hipStream_t st, const BatchShape& b, float* vz, float* vx, const float* rz, const float* rx,
+                       int it);
+void launch_edge_io(hipStream_t st, const BatchShape& b, float* fa, float* fb, float* ea, float* eb, int it,
+                    int save);
+void launch_image(hipStream_t st, const BatchShape& b, Strain sg, Strain sb, EField bk, EField bb, EField bg, Med m,
+                  float* w7, long long wstride);
+
+// --- setup and reductions (kernels_setup.hip) ------------------------------
+struct SetupArgs {
+    const float *vp_all, *vs_all, *rho_all;  // global model (device); rho_all may be null
+    const float *img_pp, *img_ps;            // global image (device); may be null
+    int allnx, allnz;
+    float pml_rr, rho_const, rb_amp;
+    unsigned long long rb_seed;
+    int acoustic;                            // 1: also emit buoyancy/modulus for the acoustic kernels
+};
+// Writes vp, vs, rho, lam, mu, dampx, dampz, dmp, dms of every slot.
+void launch_setup(hipStream_t st, const BatchShape& b, const SetupArgs& a, Med m);
+// Same, plus buoyancy 1/rho, modulus rho*vp^2 and the reflectivity window with
+// a zero frame for the plain acoustic kernels.
+void launch_setup_acoustic(hipStream_t st, const BatchShape& b, const SetupArgs& a, Med m, float* bu, float* kk,
+                           float* refl);
+// Reference acoustic damping: one array, z profile over the x profile (FD2DGPU.cu:505-519).
+void launch_combine_damp(hipStream_t st, const BatchShape& b, const float* dampx, const float* dampz, float* damp);
+// Deterministic gather of per-slot window images into global images:
+// glob[k][gx][z] += sum over slots covering gx (slot order) of win[k].
+void launch_gather(hipStream_t st, const BatchShape& b, const float* win, long long wstride, float* glob,
+                   long long gstride, int nimg, int allnx, int allnz);
+// Per-trace FP64 sums for the NCC residual: out[slot_trace][6] =
+// {|d_x|^2, |D_x|^2, d_x.D_x, |d_z|^2, |D_z|^2, d_z.D_z}.
+void launch_ncc_sums(hipStream_t st, const BatchShape& b, const float* sz, const float* sx, const float* oz,
+                     const float* ox, double* sums, const int* trace_base);
+void launch_ncc_apply(hipStream_t st, const BatchShape& b, float* sz, float* sx, const float* oz, const float* ox,
+                      const double* sums, const int* trace_base, int iter);
+void launch_fill(hipStream_t st, float* p, long long n, float v);
+void launch_axpby(hipStream_t st, float* y, const float* x, long long n, float a, float b);  // y = a*x + b*y
+// Partial FP64 dot products, finished on the host in a fixed order.
+void launch_dot(hipStream_t st, const float* x, const float* y, long long n, double* partial, int nblocks);
+
+// --- acoustic kernels (kernels_acoustic.hip) -------------------------------
+struct AcouField {
+    float *vx, *vz, *p;
+};
+struct AcouMed {
+    const float *b, *k, *m;       // buoyancy, modulus, reflectivity window (zero frame)
+    const float *damp, *rho;      // reference acoustic layout only
+};
+void launch_acou_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, const float* wav,
+                   int it, float sign, int with_born);
+void launch_acou_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, float* rec_bg,
+                   float* rec_born, int it, int with_born);
+void launch_acou_adj_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m, float* grad,
+                       const float* res, int it);
+void launch_acou_adj_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m,
+                       const float* wav, int it);
+void launch_acou_ref_step(hipStream_t st, const BatchShape& b, float* px, float* pz, float* p, float* vx, float* vz,
+                          const float* k, const float* rho, const float* damp, const float* wav, int sxnum,
+                          float* rec, int it);
+
+// --- fast fused elastic forward (kernels_fast.hip) -------------------------
+struct FastArgs {
+    float *vx, *vz, *txx, *tzz, *txz;          // state
+    float *vx_x, *vx_z, *vz_x, *vz_z;          // split memory (used in damping strips only)
+    float *txx_x, *txx_z, *tzz_x, *tzz_z, *txz_x, *txz_z;
+    const float *bx, *lam, *mu;                // buoyancy*dt, lambda*dt, mu*dt
+    const float *dampx, *dampz;
+    const float* wav;
+    float *rec_z, *rec_x;
+};
+void launch_fast_v(hipStream_t st, const BatchShape& b, const FastArgs& a, const int* rmap, int it);
+void launch_fast_s(hipStream_t st, const BatchShape& b, const FastArgs& a);
+void launch_fast_params(hipStream_t st, const float* rho, const float* lam, const float* mu, float* bx, float* ldt,
+                        float* mdt, long long n, float dt);
+void launch_fast_rmap(hipStream_t st, const BatchShape& b, int* rmap);
+
+}  // namespace ds2d

@@ -1,0 +1,243 @@
+"""ctypes binding of libds2d.so (include/ds2d.h).
+
+The product path is the HIP library only: importing this module raises if the
+library has not been built, and every call raises on a non-zero status with
+the library's own error message.  There is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libds2d.so")
+
+_lib = None
+
+
+class DS2DError(RuntimeError):
+    pass
+
+
+class Geom(C.Structure):
+    _fields_ = [("dx", C.c_float), ("dz", C.c_float), ("dt", C.c_float),
+                ("nxpml", C.c_int), ("nzpml", C.c_int), ("allnx", C.c_int), ("allnz", C.c_int),
+                ("scale", C.c_int), ("pml", C.c_int), ("nt", C.c_int), ("nop", C.c_int), ("ntr", C.c_int),
+                ("physics", C.c_int), ("pml_rr", C.c_float), ("rho_const", C.c_float), ("rb_amp", C.c_float),
+                ("rb_seed", C.c_ulonglong)]
+
+
+class Shot(C.Structure):
+    _fields_ = [("ngx_min", C.c_int), ("nx", C.c_int), ("isx", C.c_int), ("isz", C.c_int), ("igz", C.c_int),
+                ("ntr", C.c_int), ("rec_ix", C.POINTER(C.c_int)), ("shot_id", C.c_int), ("boundary", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("elapsed_ms", C.c_double), ("stencil_ms", C.c_double), ("cell_updates", C.c_longlong),
+                ("stencil_launches", C.c_longlong), ("alg_bytes", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# name -> (restype, argtypes); the exported surface of include/ds2d.h
+_F = C.POINTER(C.c_float)
+_D = C.POINTER(C.c_double)
+_H = C.c_void_p
+SIGNATURES = {
+    "ds2d_last_error": (C.c_char_p, []),
+    "ds2d_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), _D]),
+    "ds2d_create": (C.c_int, [C.POINTER(Geom), _F, C.POINTER(_H)]),
+    "ds2d_destroy": (C.c_int, [_H]),
+    "ds2d_set_model": (C.c_int, [_H, _F, _F, _F]),
+    "ds2d_set_image": (C.c_int, [_H, _F, _F]),
+    "ds2d_set_shot": (C.c_int, [_H, C.POINTER(Shot), _F, _F]),
+    "ds2d_shot_gradient": (C.c_int, [_H, C.c_int, _D]),
+    "ds2d_batch_gradient": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.c_int, _D]),
+    "ds2d_take_gradients": (C.c_int, [_H, _F, _F, _F, _F, _F, _F, _F]),
+    "ds2d_forward": (C.c_int, [_H, _F, _F]),
+    "ds2d_batch_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.c_int, C.POINTER(Stats)]),
+    "ds2d_acoustic_ref_forward": (C.c_int, [_H, C.c_int, C.c_int, _F]),
+    "ds2d_born_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, _F, C.POINTER(Stats)]),
+    "ds2d_born_adjoint": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.POINTER(Stats)]),
+    "ds2d_lsrtm_cg": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, C.c_int, _F, _D, C.POINTER(Stats)]),
+    "ds2d_laplace_filter": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_float, _F]),
+    "ds2d_laplace_band": (C.c_int, [C.c_int, C.c_int, _F]),
+    "ds2d_ricker": (C.c_int, [C.c_int, C.c_float, C.c_float, _F]),
+}
+
+
+def load(path=LIB_PATH):
+    """Load libds2d.so; raise if it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise DS2DError(f"{path} not built: run `python -m ds2d_amd.build` (the HIP library is required)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise DS2DError(f"ds2d error {rc}: {_lib.ds2d_last_error().decode()}")
+
+
+def fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"], "float32 C-contiguous array required"
+    return a.ctypes.data_as(_F)
+
+
+def ricker(nt, f0, dt):
+    lib = load()
+    out = np.zeros(nt, np.float32)
+    _check(lib.ds2d_ricker(nt, f0, dt, fptr(out)))
+    return out
+
+
+def laplace_filter(img, dx, order=2):
+    """laplace_filter_2d(nz, nx, order, dx, dat) on an [nx][nz] image (copy)."""
+    lib = load()
+    a = np.ascontiguousarray(img, np.float32).copy()
+    nx, nz = a.shape
+    _check(lib.ds2d_laplace_filter(nz, nx, order, dx, fptr(a)))
+    return a
+
+
+def laplace_band(img):
+    lib = load()
+    a = np.ascontiguousarray(img, np.float32).copy()
+    nx, nz = a.shape
+    _check(lib.ds2d_laplace_band(nz, nx, fptr(a)))
+    return a
+
+
+def device_info():
+    lib = load()
+    name = C.create_string_buffer(64)
+    ncu = C.c_int(0)
+    mem = C.c_double(0)
+    _check(lib.ds2d_device_info(name, 64, C.byref(ncu), C.byref(mem)))
+    return name.value.decode(), ncu.value, mem.value
+
+
+class ShotSpec:
+    """One shot window in padded-window coordinates (see ds2d_shot)."""
+
+    def __init__(self, ngx_min, nx, isx, isz, igz, rec_ix, shot_id=0, boundary=0):
+        self.ngx_min, self.nx, self.isx, self.isz, self.igz = int(ngx_min), int(nx), int(isx), int(isz), int(igz)
+        self.rec_ix = np.ascontiguousarray(rec_ix, np.int32)
+        self.shot_id, self.boundary = int(shot_id), int(boundary)
+
+    @property
+    def ntr(self):
+        return int(self.rec_ix.size)
+
+
+def _shots_array(shots):
+    arr = (Shot * len(shots))()
+    for i, s in enumerate(shots):
+        arr[i] = Shot(s.ngx_min, s.nx, s.isx, s.isz, s.igz, s.ntr, s.rec_ix.ctypes.data_as(C.POINTER(C.c_int)),
+                      s.shot_id, s.boundary)
+    return arr
+
+
+class Solver:
+    """Owning wrapper of a ds2d handle (FD2DGPU_ELASTIC counterpart)."""
+
+    def __init__(self, *, dx, dz, dt, nt, allnx, allnz, pml, nx_max, ntr_max=0, wavelet, physics=0,
+                 pml_rr=1e-3, rho_const=0.0, rb_amp=0.3, rb_seed=1234):
+        self.lib = load()
+        g = Geom(dx, dz, dt, nx_max + 2 * pml, allnz + 2 * pml, allnx, allnz, 1, pml, nt, 4, ntr_max, physics,
+                 pml_rr, rho_const, rb_amp, rb_seed)
+        self.geom = g
+        self.nt, self.allnx, self.allnz, self.pml = nt, allnx, allnz, pml
+        self.wavelet = np.ascontiguousarray(wavelet, np.float32)
+        h = C.c_void_p()
+        _check(self.lib.ds2d_create(C.byref(g), fptr(self.wavelet), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.ds2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_model(self, vp, vs=None, rho=None):
+        self._vp = np.ascontiguousarray(vp, np.float32)
+        self._vs = None if vs is None else np.ascontiguousarray(vs, np.float32)
+        self._rho = None if rho is None else np.ascontiguousarray(rho, np.float32)
+        _check(self.lib.ds2d_set_model(self.h, fptr(self._vp), fptr(self._vs), fptr(self._rho)))
+
+    def set_image(self, pp, ps=None):
+        pp = np.ascontiguousarray(pp, np.float32)
+        ps = None if ps is None else np.ascontiguousarray(ps, np.float32)
+        _check(self.lib.ds2d_set_image(self.h, fptr(pp), fptr(ps)))
+
+    def _rec(self, shots):
+        return np.zeros(sum(s.ntr for s in shots) * self.nt, np.float32)
+
+    def batch_forward(self, shots, mode=1):
+        rz, rx = self._rec(shots), self._rec(shots)
+        st = Stats()
+        arr = _shots_array(shots)
+        _check(self.lib.ds2d_batch_forward(self.h, len(shots), arr, fptr(rz), fptr(rx), mode, C.byref(st)))
+        return rz, rx, st.as_dict()
+
+    def batch_gradient(self, shots, obs_z, obs_x, it):
+        mf = C.c_double(0.0)
+        arr = _shots_array(shots)
+        oz = np.ascontiguousarray(obs_z, np.float32)
+        ox = np.ascontiguousarray(obs_x, np.float32)
+        _check(self.lib.ds2d_batch_gradient(self.h, len(shots), arr, fptr(oz), fptr(ox), it, C.byref(mf)))
+        return mf.value
+
+    def take_gradients(self):
+        A = self.allnx * self.allnz
+        outs = [np.zeros(A, np.float32) for _ in range(7)]
+        _check(self.lib.ds2d_take_gradients(self.h, *[fptr(o) for o in outs]))
+        return [o.reshape(self.allnx, self.allnz) for o in outs]
+
+    def acoustic_ref_forward(self, sxnum, ngx_min):
+        nx = self.geom.nxpml - 2 * self.pml
+        rec = np.zeros(nx * self.nt, np.float32)
+        _check(self.lib.ds2d_acoustic_ref_forward(self.h, sxnum, ngx_min, fptr(rec)))
+        return rec.reshape(nx, self.nt)
+
+    def born_forward(self, shots, refl):
+        refl = np.ascontiguousarray(refl, np.float32)
+        rbg, rbn = self._rec(shots), self._rec(shots)
+        st = Stats()
+        _check(self.lib.ds2d_born_forward(self.h, len(shots), _shots_array(shots), fptr(refl), fptr(rbg),
+                                          fptr(rbn), C.byref(st)))
+        return rbg, rbn, st.as_dict()
+
+    def born_adjoint(self, shots, res):
+        res = np.ascontiguousarray(res, np.float32)
+        g = np.zeros(self.allnx * self.allnz, np.float32)
+        st = Stats()
+        _check(self.lib.ds2d_born_adjoint(self.h, len(shots), _shots_array(shots), fptr(res), fptr(g),
+                                          C.byref(st)))
+        return g.reshape(self.allnx, self.allnz), st.as_dict()
+
+    def lsrtm_cg(self, shots, d_obs, niter, image0=None):
+        img = np.zeros(self.allnx * self.allnz, np.float32) if image0 is None else \
+            np.ascontiguousarray(image0, np.float32).ravel().copy()
+        res = np.zeros(niter + 1, np.float64)
+        st = Stats()
+        d = np.ascontiguousarray(d_obs, np.float32)
+        _check(self.lib.ds2d_lsrtm_cg(self.h, len(shots), _shots_array(shots), fptr(d), niter, fptr(img),
+                                      res.ctypes.data_as(_D), C.byref(st)))
+        return img.reshape(self.allnx, self.allnz), res, st.as_dict()

@@ -1,0 +1,145 @@
+/*
+ * ds2d.h — C-ABI of the MI355X-native DS2D time-domain hot path (libds2d.so).
+ *
+ * Plain C types only (pointers, sizes, status codes).  Every entry point
+ * returns 0 on success and a negative DS2D_E* code on failure; the message is
+ * available from ds2d_last_error() (thread-local).  One handle may be used by
+ * one host thread at a time; all device work of a handle runs on its own HIP
+ * stream.  Host arrays are owned by the caller; device buffers are owned by
+ * the handle (allocated in ds2d_create / on first use, freed in ds2d_destroy).
+ *
+ * Layouts follow the reference: models and images are [ix][iz] with z
+ * contiguous (allnx*allnz floats, x-major); records are [trace][it].
+ *
+ * Reference interfaces replaced (upstream DS2D tree, file:line):
+ *   ds2d_create            FD2DGPU_ELASTIC::FD2DGPU_ELASTIC  FD2DGPU.cuh:130-131, FD2DGPU.cu:1350
+ *   ds2d_destroy           FD2DGPU_ELASTIC::~FD2DGPU_ELASTIC FD2DGPU.cu:1636
+ *   ds2d_set_model         GPUbufferVPVS                     FD2DGPU.cu:1311-1315 (main.cpp:940)
+ *   ds2d_set_image         GPUbufferM                        FD2DGPU.cu:1318-1322 (main.cpp:954)
+ *   ds2d_set_shot          record_copytoGPU + bufferVpVsHtoD + public fields
+ *                                                            FD2DGPU.cu:3357-3383, 1336-1348;
+ *                                                            main.cpp:994-1025
+ *   ds2d_shot_gradient     ELSRTM_ROT_EXP_IMAGE_SINGLESHOT   FD2DGPU.cuh:429-430, FD2DGPU.cu:4184-4879
+ *   ds2d_take_gradients    imagebuffer_resettozero           FD2DGPU.cu:3478-3505 (main.cpp:1035)
+ *   ds2d_forward           elastic_rotation_addS_for + born_modeling(1,0) + GPUseisrecord
+ *                                                            FD2DGPU.cu:2046-2049, 1834-1839, 355-365
+ *   ds2d_batch_gradient    the per-worker shot loop of lsrtm_all_iteration with the
+ *                          MPI shot dispatch replaced by one batched launch per step
+ *                                                            main.cpp:956-1036
+ *   ds2d_batch_forward     FD_ELASTIC forward modelling (FD2DGPU.cu:2241, dead) batched
+ *   ds2d_acoustic_ref_forward  FD() calculateP/calculateV/addS/recordD FD2DGPU.cu:523-583
+ *   ds2d_born_forward / ds2d_born_adjoint / ds2d_lsrtm_cg
+ *                          plain-L2 counterpart of the Born/adjoint pair inside
+ *                          ELSRTM_ROT_EXP_IMAGE_SINGLESHOT and of sjcgsolver
+ *                          (main.cpp:1224-1320); the reference has no exact adjoint
+ *   ds2d_laplace_filter    laplace_filter_2d(nz,nx,order,dx,dat) laplace.cpp:82-106
+ *   ds2d_laplace_band      laplace_filter_2d(nz,nx,dat)          laplace.cpp:3-57
+ *   ds2d_ricker            ricker1                               main.cpp:461-473
+ */
+#ifndef DS2D_H
+#define DS2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DS2D_OK 0
+#define DS2D_EINVAL (-1)   /* bad argument / inconsistent sizes            */
+#define DS2D_EDEVICE (-2)  /* HIP runtime error or no usable gfx950 device */
+#define DS2D_ENOMEM (-3)   /* device allocation failed                     */
+#define DS2D_ESTATE (-4)   /* call out of order (no model, no shot, ...)    */
+
+typedef struct ds2d_ctx ds2d_t;
+
+/* Constructor arguments of FD2DGPU_ELASTIC (main.cpp:939) plus the options
+ * the reference hard-codes.  nxpml/nzpml describe the largest window; a batch
+ * may hold narrower windows. */
+typedef struct {
+    float dx, dz, dt;
+    int nxpml, nzpml;       /* padded window (largest)                         */
+    int allnx, allnz;       /* global model                                    */
+    int scale;              /* trace decimation of the surface recorder (1)    */
+    int pml;                /* frame width (PML or random boundary)            */
+    int nt;                 /* time steps                                      */
+    int nop;                /* half stencil length; must be 4                  */
+    int ntr;                /* traces per shot (largest)                       */
+    int physics;            /* 0 elastic, 1 acoustic                           */
+    float pml_rr;           /* PML reflection coefficient (elastic 1e-3)       */
+    float rho_const;        /* > 0: constant density (reference: 2000)         */
+    float rb_amp;           /* random boundary relative amplitude (0.3)        */
+    unsigned long long rb_seed; /* random boundary seed                        */
+} ds2d_geom;
+
+/* One shot / aperture window (main.cpp:1011-1025 public fields). */
+typedef struct {
+    int ngx_min;            /* window start column in the global model          */
+    int nx;                 /* window width (physical cells)                    */
+    int isx, isz;           /* source cell, padded-window coordinates           */
+    int igz;                /* receiver row, padded coordinates (pml + gy)      */
+    int ntr;                /* number of traces                                 */
+    const int* rec_ix;      /* [ntr] receiver columns, padded coordinates       */
+    int shot_id;            /* seeds the random boundary of this shot           */
+    int boundary;           /* 0 PML, 1 random boundary                         */
+} ds2d_shot;
+
+/* Timing of the last run (HIP events on the handle's stream). */
+typedef struct {
+    double elapsed_ms;         /* whole run                                       */
+    double stencil_ms;         /* time-stepping loop only                         */
+    long long cell_updates;    /* sum over wavefields of nxpml*nzpml*nt            */
+    long long stencil_launches;/* launches of the dominant stencil kernel          */
+    double alg_bytes;          /* algorithmic HBM bytes of those launches          */
+} ds2d_stats;
+
+const char* ds2d_last_error(void);
+int ds2d_device_info(char* name, int len, int* n_cu, double* hbm_gb);
+
+int ds2d_create(const ds2d_geom* g, const float* wavelet, ds2d_t** out);
+int ds2d_destroy(ds2d_t* h);
+
+int ds2d_set_model(ds2d_t* h, const float* vp_all, const float* vs_all, const float* rho_all);
+int ds2d_set_image(ds2d_t* h, const float* pp, const float* ps);
+int ds2d_set_shot(ds2d_t* h, const ds2d_shot* s, const float* rec_z, const float* rec_x);
+
+/* Reference per-shot LSRTM gradient (compat mode: Born + 2nd-order Born,
+ * NCC residual, three-term imaging).  Accumulates into the handle's images. */
+int ds2d_shot_gradient(ds2d_t* h, int iter, double* misfit);
+/* Same for many shots in one batched launch per step (obs_* concatenated). */
+int ds2d_batch_gradient(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* obs_z,
+                        const float* obs_x, int iter, double* misfit);
+int ds2d_take_gradients(ds2d_t* h, float* pp, float* ps, float* illum, float* b_pp, float* b_ps,
+                        float* b2_pp, float* b2_ps);
+
+/* Elastic forward modelling of the current shot: vz, vx at (rec_ix, igz). */
+int ds2d_forward(ds2d_t* h, float* rec_z, float* rec_x);
+/* Batched forward modelling of many uneven windows.  Records concatenated in
+ * shot order, each [ntr_s][nt].  mode: 0 = plain per-field kernels,
+ * 1 = fused fast kernel (default). */
+int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec_z, float* rec_x,
+                       int mode, ds2d_stats* stats);
+
+/* Reference acoustic step (split p, mirrored stagger): pressure at depth pml. */
+int ds2d_acoustic_ref_forward(ds2d_t* h, int sxnum, int ngx_min, float* rec_p);
+
+/* Plain-L2 acoustic Born operator L (refl -> records) and its exact adjoint
+ * L^T (residual -> gradient, allnx*allnz, overwritten), batched over shots.
+ * Random boundary, no damping.  rec/res concatenated [shot][trace][it]. */
+int ds2d_born_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* refl, float* rec_bg,
+                      float* rec_born, ds2d_stats* stats);
+int ds2d_born_adjoint(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* res, float* grad,
+                      ds2d_stats* stats);
+
+/* Conjugate-gradient LSRTM (CGLS on the plain Born operator).  image in/out
+ * (allnx*allnz); resid[niter+1] receives |Lm-d|/|d| before each iteration and
+ * after the last. */
+int ds2d_lsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_obs, int niter,
+                  float* image, double* resid, ds2d_stats* stats);
+
+int ds2d_laplace_filter(int nz, int nx, int order, float dx, float* dat);
+int ds2d_laplace_band(int nz, int nx, float* dat);
+int ds2d_ricker(int nt, float f0, float dt, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DS2D_H */

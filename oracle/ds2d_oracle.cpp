@@ -1,0 +1,657 @@
+// DS2D CPU oracle — TEST INFRASTRUCTURE ONLY (see ds2d_oracle.hpp header).
+// extern "C" surface used by tests/ (ctypes) and bench.py's cpu_baseline leg.
+#include "ds2d_oracle.hpp"
+
+#include <cstdio>
+#include <omp.h>
+
+using namespace orc;
+
+extern "C" {
+
+// Shot description shared by the oracle drivers (all indices are padded-window
+// coordinates unless stated otherwise).
+typedef struct {
+    int allnx, allnz;    // global model size (x-major, z contiguous)
+    int ngx_min, nx;     // window start column in the global model, window width
+    int pml, nt;         // frame width, time steps
+    double dx, dz, dt;
+    int boundary;        // 0 = PML damping, 1 = random boundary (no damping)
+    double rr;           // PML reflection coefficient (elastic 1e-3, acoustic 1e-6)
+    unsigned long long rb_seed;
+    int rb_shot;
+    double rb_amp;       // random boundary relative amplitude (0.3)
+    double rho_const;    // > 0: constant density (reference compat), else rho array
+    int isx, isz;        // source cell
+    int igz, ntr;        // receiver row, trace count
+    const int* rec_ix;   // receiver columns [ntr]
+} orc_shot;
+
+}  // extern "C"
+
+namespace {
+
+template <typename R>
+struct Setup {
+    Grid<R> g;
+    Medium<R> m;
+    int nx, nz;
+};
+
+// Window, pad and build the elastic medium exactly as FD2DGPU_ELASTIC does per
+// shot: bufferVpVsHtoD (FD2DGPU.cu:1336-1348), then either set_random_boundary
+// (replaced by the seeded frame, FD2DGPU.cu:3156-3216) or
+// calculate_elastic_damp_C (FD2DGPU.cu:1292-1309).
+template <typename R>
+Setup<R> make_setup(const orc_shot* s, const R* vp, const R* vs, const R* rho) {
+    Setup<R> S;
+    S.nx = s->nx;
+    S.nz = s->allnz;
+    S.g = Grid<R>{s->nx + 2 * s->pml, s->allnz + 2 * s->pml, s->pml, (R)s->dx, (R)s->dz, (R)s->dt};
+    const size_t n = S.g.n();
+    Medium<R>& m = S.m;
+    m.vp.assign(n, 0); m.vs.assign(n, 0); m.rho.assign(n, 0); m.lam.assign(n, 0); m.mu.assign(n, 0);
+    m.dampx.assign(n, 0); m.dampz.assign(n, 0);
+    pad_window(vp, s->allnz, s->ngx_min, s->nx, S.nz, s->pml, m.vp.data());
+    if (vs) pad_window(vs, s->allnz, s->ngx_min, s->nx, S.nz, s->pml, m.vs.data());
+    std::vector<R> rhop(n, 0);
+    if (rho) pad_window(rho, s->allnz, s->ngx_min, s->nx, S.nz, s->pml, rhop.data());
+    if (s->boundary == 1) {
+        random_boundary(S.g, (uint64_t)s->rb_seed, s->rb_shot, (R)s->rb_amp, m.vp.data(), m.vs.data());
+    } else {
+        damp_profile(S.g, m.vp.data(), (R)s->rr, m.dampx.data(), m.dampz.data());
+    }
+    elastic_moduli(n, m.vp.data(), m.vs.data(), rhop.data(), (R)s->rho_const, m.rho.data(), m.lam.data(),
+                   m.mu.data());
+    return S;
+}
+
+template <typename R>
+void record_pair(const Grid<R>& g, const orc_shot* s, const std::vector<R>& vz, const std::vector<R>& vx,
+                 int it, R* rz, R* rxx) {
+    for (int r = 0; r < s->ntr; ++r) {
+        size_t c = g.at(s->rec_ix[r], s->igz);
+        if (rz) rz[(size_t)r * s->nt + it] = vz[c];
+        if (rxx) rxx[(size_t)r * s->nt + it] = vx[c];
+    }
+}
+
+// Elastic forward modelling: FD2DGPU.cu:2046 source before the step,
+// born_modeling(1,0) (FD2DGPU.cu:1834-1839), then the surface recorder
+// GPUseisrecord (FD2DGPU.cu:355-365) generalised to a receiver list.
+template <typename R>
+void elastic_forward(const orc_shot* s, const R* vp, const R* vs, const R* rho, const R* wav, R* rec_z,
+                     R* rec_x, R* final5) {
+    Setup<R> S = make_setup<R>(s, vp, vs, rho);
+    Elastic<R> f;
+    f.init(S.g.n());
+    const size_t src = S.g.at(s->isx, s->isz);
+    for (int it = 0; it < s->nt; ++it) {
+        f.txx[src] += wav[it];
+        f.tzz[src] += wav[it];
+        update_v(S.g, f, S.m, 1);
+        update_s(S.g, f, S.m, 1);
+        record_pair(S.g, s, f.vz, f.vx, it, rec_z, rec_x);
+    }
+    if (final5) {
+        const size_t n = S.g.n();
+        std::copy(f.vx.begin(), f.vx.end(), final5);
+        std::copy(f.vz.begin(), f.vz.end(), final5 + n);
+        std::copy(f.txx.begin(), f.txx.end(), final5 + 2 * n);
+        std::copy(f.tzz.begin(), f.tzz.end(), final5 + 3 * n);
+        std::copy(f.txz.begin(), f.txz.end(), final5 + 4 * n);
+    }
+}
+
+// FD2DGPU.cu:4532-4673 normalised cross-correlation residual (host, double
+// sums).  iter==0: r = -D/|D|^2; else r = ((dD/|d|^2) d - D)/(|d||D|) and
+// misfit -= (1/(2 ntr)) sum_tr (dD_x/(|d_x||D_x|) + dD_z/(|d_z||D_z|)).
+// A zero-norm trace yields a zero residual here (the reference divides by 0).
+template <typename R>
+void ncc_residual(int ntr, int nt, int iter, R* syn_z, R* syn_x, const R* obs_z, const R* obs_x,
+                  double* misfit) {
+    std::vector<double> dx(ntr, 0), Dx(ntr, 0), dDx(ntr, 0), dz(ntr, 0), Dz(ntr, 0), dDz(ntr, 0);
+    for (int i = 0; i < ntr; ++i)
+        for (int j = 0; j < nt; ++j) {
+            size_t k = (size_t)i * nt + j;
+            if (iter != 0) {
+                dx[i] += (double)syn_x[k] * syn_x[k];
+                dz[i] += (double)syn_z[k] * syn_z[k];
+                dDx[i] += (double)syn_x[k] * obs_x[k];
+                dDz[i] += (double)syn_z[k] * obs_z[k];
+            }
+            Dx[i] += (double)obs_x[k] * obs_x[k];
+            Dz[i] += (double)obs_z[k] * obs_z[k];
+        }
+    if (iter != 0) {
+        for (int i = 0; i < ntr; ++i) {
+            double tx = (dx[i] > 0 && Dx[i] > 0) ? dDx[i] / (std::sqrt(dx[i]) * std::sqrt(Dx[i])) : 0.0;
+            double tz = (dz[i] > 0 && Dz[i] > 0) ? dDz[i] / (std::sqrt(dz[i]) * std::sqrt(Dz[i])) : 0.0;
+            *misfit -= (double)(1.0f / (2 * ntr)) * (tx + tz);
+        }
+    }
+    for (int i = 0; i < ntr; ++i)
+        for (int j = 0; j < nt; ++j) {
+            size_t k = (size_t)i * nt + j;
+            if (iter == 0) {
+                double ax = Dx[i] > 0 ? 1.0 / (std::sqrt(Dx[i]) * std::sqrt(Dx[i])) : 0.0;
+                double az = Dz[i] > 0 ? 1.0 / (std::sqrt(Dz[i]) * std::sqrt(Dz[i])) : 0.0;
+                syn_x[k] = (R)(ax * (0.0 - (double)obs_x[k]));
+                syn_z[k] = (R)(az * (0.0 - (double)obs_z[k]));
+            } else {
+                bool okx = dx[i] > 0 && Dx[i] > 0, okz = dz[i] > 0 && Dz[i] > 0;
+                double ax = okx ? 1.0 / (std::sqrt(dx[i]) * std::sqrt(Dx[i])) : 0.0;
+                double az = okz ? 1.0 / (std::sqrt(dz[i]) * std::sqrt(Dz[i])) : 0.0;
+                double gx = okx ? dDx[i] / dx[i] : 0.0, gz = okz ? dDz[i] / dz[i] : 0.0;
+                syn_x[k] = (R)(ax * (gx * (double)syn_x[k] - (double)obs_x[k]));
+                syn_z[k] = (R)(az * (gz * (double)syn_z[k] - (double)obs_z[k]));
+            }
+        }
+}
+
+// ELSRTM_ROT_EXP_IMAGE_SINGLESHOT (FD2DGPU.cu:4184-4879), one shot, one
+// iteration: forward bg/b/b2 with scatter records (and PML edge bands), NCC
+// residual, then the reverse loop with source reconstruction, receiver-side
+// adjoint, receiver-side Born adjoint and the three-term imaging condition.
+// grads7 = [pp, ps, illum, b_pp, b_ps, b2_pp, b2_ps], each allnx*allnz, +=.
+template <typename R>
+void elsrtm_shot(const orc_shot* s, const R* vp, const R* vs, const R* rho, const R* wav, const R* img_pp,
+                 const R* img_ps, const R* obs_z, const R* obs_x, int iter, int col0, R* grads7,
+                 double* misfit) {
+    Setup<R> S = make_setup<R>(s, vp, vs, rho);
+    const Grid<R>& g = S.g;
+    const size_t n = g.n();
+    const int nt = s->nt, ntr = s->ntr, nz = S.nz;
+    const bool rbc = s->boundary == 1;
+    // buffer_image_extrapolation (FD2DGPU.cu:1324-1333): padded image window.
+    std::vector<R> dmp(n, 0), dms(n, 0);
+    pad_window(img_pp, s->allnz, col0, s->nx, nz, s->pml, dmp.data());
+    pad_window(img_ps, s->allnz, col0, s->nx, nz, s->pml, dms.data());
+
+    Elastic<R> bg, b, b2, bk, bbk;
+    bg.init(n); b.init(n); b2.init(n); bk.init(n); bbk.init(n);
+    Strain<R> Sg, Sb;
+    Sg.init(n); Sb.init(n);
+    std::vector<R> rs_z((size_t)ntr * nt, 0), rs_x(rs_z.size(), 0), rs2_z(rs_z.size(), 0), rs2_x(rs_z.size(), 0);
+    const int N = NOP;
+    EdgeBands<R> e_vx, e_vz, e_bvx, e_bvz;
+    if (!rbc) {
+        e_vx.init(nt, S.nx, nz, N); e_vz.init(nt, S.nx, nz, N);
+        e_bvx.init(nt, S.nx, nz, N); e_bvz.init(nt, S.nx, nz, N);
+    }
+    const size_t src = g.at(s->isx, s->isz);
+    for (int it = 0; it < nt; ++it) {
+        bg.txx[src] += wav[it];
+        bg.tzz[src] += wav[it];
+        update_v(g, bg, S.m, 1);
+        update_s(g, bg, S.m, 1);
+        add_born_source(g, bg.vx, bg.vz, b, dmp, dms, S.m, 1);
+        update_v(g, b, S.m, 1);
+        update_s(g, b, S.m, 1);
+        add_born_source(g, b.vx, b.vz, b2, dmp, dms, S.m, 1);
+        update_v(g, b2, S.m, 1);
+        update_s(g, b2, S.m, 1);
+        record_pair(g, s, b.vz, b.vx, it, rs_z.data(), rs_x.data());
+        record_pair(g, s, b2.vz, b2.vx, it, rs2_z.data(), rs2_x.data());
+        if (!rbc) {
+            edge_io(g, bg.vx, e_vx, it, true); edge_io(g, bg.vz, e_vz, it, true);
+            edge_io(g, b.vx, e_bvx, it, true); edge_io(g, b.vz, e_bvz, it, true);
+        }
+    }
+    // scatter_add (FD2DGPU.cu:3143-3149): s = 1*s + s2.
+    for (size_t k = 0; k < rs_z.size(); ++k) {
+        rs_z[k] = (R)1 * rs_z[k] + rs2_z[k];
+        rs_x[k] = (R)1 * rs_x[k] + rs2_x[k];
+    }
+    ncc_residual<R>(ntr, nt, iter, rs_z.data(), rs_x.data(), obs_z, obs_x, misfit);
+
+    const size_t A = (size_t)s->allnx * s->allnz;
+    R *gpp = grads7, *gps = grads7 + A, *gil = grads7 + 2 * A, *bpp = grads7 + 3 * A, *bps = grads7 + 4 * A,
+      *b2pp = grads7 + 5 * A, *b2ps = grads7 + 6 * A;
+    for (int it = nt - 1; it >= 0; --it) {
+        if (!rbc) {
+            edge_io(g, bg.vx, e_vx, it, false); edge_io(g, bg.vz, e_vz, it, false);
+            edge_io(g, b.vx, e_bvx, it, false); edge_io(g, b.vz, e_bvz, it, false);
+        }
+        recon_s(g, bg, S.m, -1, Sg);
+        recon_v(g, bg, S.m, -1);
+        recon_s(g, b, S.m, -1, Sb);
+        add_born_source(g, bg.vx, bg.vz, b, dmp, dms, S.m, 1);
+        recon_v(g, b, S.m, -1);
+        for (int r = 0; r < ntr; ++r) {
+            size_t c = g.at(s->rec_ix[r], s->igz);
+            bk.vz[c] += rs_z[(size_t)r * nt + it];
+            bk.vx[c] += rs_x[(size_t)r * nt + it];
+        }
+        adjoint_s(g, bk, S.m, -1);
+        adjoint_v(g, bk, S.m, -1);
+        adjoint_v(g, bbk, S.m, -1);
+        add_born_source_adjoint(g, bk, bbk.vx, bbk.vz, dmp, dms, S.m, -1);
+        adjoint_s(g, bbk, S.m, -1);
+        image_gradient(g, Sg, bk, S.m, col0, s->allnz, gpp, gps);
+        image_gradient(g, Sb, bk, S.m, col0, s->allnz, bpp, bps);
+        image_gradient(g, Sg, bbk, S.m, col0, s->allnz, b2pp, b2ps);
+        image_illum(g, bg, col0, s->allnz, gil);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Acoustic, reference layout (FD2DGPU.cu:253-340, 496-544): split p, single
+// damping array (dampX then dampZ into the same array, rr = 1e-6), rho = 1000,
+// source on vz at (sxnum+pml, pml), pressure recorded at depth pml.  The
+// velocity stagger is mirrored relative to the elastic kernels and vz uses
+// dt/dx (FD2DGPU.cu:318) — kept.
+// ---------------------------------------------------------------------------
+template <typename R>
+void acoustic_ref_forward(const orc_shot* s, const R* vel, const R* rho, const R* wav, int sxnum, R* rec_p) {
+    const int pml = s->pml, nx = s->nx, nz = s->allnz;
+    Grid<R> g{nx + 2 * pml, nz + 2 * pml, pml, (R)s->dx, (R)s->dz, (R)s->dt};
+    const size_t n = g.n();
+    std::vector<R> vt(n, 0), rr(n, 0), k(n, 0), damp(n, 0), px(n, 0), pz(n, 0), p(n, 0), vx(n, 0), vz(n, 0);
+    pad_window(vel, s->allnz, s->ngx_min, nx, nz, pml, vt.data());
+    std::vector<R> rhop(n, 0);
+    if (rho) pad_window(rho, s->allnz, s->ngx_min, nx, nz, pml, rhop.data());
+    // FD2DGPU.cu:505-519: dampX then dampZ write the same array.
+    {
+        std::vector<R> dxa(n, 0), dza(n, 0);
+        damp_profile(g, vt.data(), (R)s->rr, dxa.data(), dza.data());
+        for (int ix = 0; ix < g.nxp; ++ix)
+            for (int iz = 0; iz < g.nzp; ++iz) {
+                size_t c = g.at(ix, iz);
+                if (ix < pml || ix > g.nxp - pml) damp[c] = dxa[c];
+                if (iz < pml || iz > g.nzp - pml) damp[c] = dza[c];
+            }
+    }
+    for (size_t c = 0; c < n; ++c) {  // FD2DGPU.cu:496-502
+        rr[c] = s->rho_const > 0 ? (R)s->rho_const : rhop[c];
+        k[c] = vt[c] * vt[c] * rr[c];
+    }
+    const R dt = g.dt, dx = g.dx, dz = g.dz;
+    const int nxp = g.nxp, nzp = g.nzp;
+    for (int it = 0; it < s->nt; ++it) {
+#pragma omp parallel for
+        for (int ix = NOP - 1; ix <= nxp - NOP - 1; ++ix)
+            for (int iz = NOP - 1; iz <= nzp - NOP - 1; ++iz) {
+                size_t c = g.at(ix, iz);
+                R d1 = 1 - dt * damp[c] / 2, d2 = 1 + dt * damp[c] / 2;
+                R tvx = 0, tvz = 0;
+                for (int i = 1; i <= NOP; ++i) {
+                    tvx += (R)C8[i] * (vx[c + (size_t)i * nzp] - vx[c - (size_t)(i - 1) * nzp]);
+                    tvz += (R)C8[i] * (vz[c + i] - vz[c - i + 1]);
+                }
+                px[c] = (d1 * px[c] - k[c] * (dt / dx) * tvx) / d2;
+                pz[c] = (d1 * pz[c] - k[c] * (dt / dz) * tvz) / d2;
+                p[c] = px[c] + pz[c];
+            }
+#pragma omp parallel for
+        for (int ix = NOP; ix <= nxp - NOP; ++ix)
+            for (int iz = NOP; iz <= nzp - NOP; ++iz) {
+                size_t c = g.at(ix, iz);
+                R d1 = 1 - dt * damp[c] / 2, d2 = 1 + dt * damp[c] / 2;
+                if (iz <= nzp - NOP - 1) {  // GPUcalculate_vx bounds
+                    R tp = 0;
+                    for (int i = 1; i <= NOP; ++i)
+                        tp += (R)C8[i] * (p[c + (size_t)(i - 1) * nzp] - p[c - (size_t)i * nzp]);
+                    vx[c] = (d1 * vx[c] - ((R)1 / rr[c]) * (dt / dx) * tp) / d2;
+                }
+                if (ix <= nxp - NOP - 1) {  // GPUcalculate_vz bounds
+                    R tp = 0;
+                    for (int i = 1; i <= NOP; ++i) tp += (R)C8[i] * (p[c + i - 1] - p[c - i]);
+                    vz[c] = (d1 * vz[c] - ((R)1 / rr[c]) * (dt / dx) * tp) / d2;
+                }
+            }
+        vz[g.at(sxnum + pml, pml)] += wav[it];
+        for (int ix = 0; ix < nx; ++ix) rec_p[(size_t)ix * s->nt + it] = p[g.at(ix + pml, pml)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain-L2 acoustic Born modelling and its exact adjoint (the build's own
+// formulation; no reference counterpart).  It is the mu = 0 specialisation of
+// the elastic live stencils (born.cu:215-349: same stagger, same [nop,n-nop)
+// bounds, txx = tzz = p, txz = 0), random boundary, no damping.
+//   step n:  v  += b dt D+ (p + s_n delta_src)         (source transient)
+//            dp  = k dt D- v ;  p += dp
+//            dv += b dt D+ dpb ; dpb += k dt D- dv + 2 m dp   (Born, m = dv/v)
+//            record p, dpb at (rec_ix, igz)
+// The adjoint runs backward with bg reconstruction by exact reversal and is
+// the transpose of the Born map above (dot-product test).
+// ---------------------------------------------------------------------------
+template <typename R>
+struct Acou {
+    Grid<R> g;
+    std::vector<R> b, k, m;
+    int nx, nz;
+};
+
+template <typename R>
+Acou<R> acou_setup(const orc_shot* s, const R* vp, const R* rho, const R* refl) {
+    Acou<R> A;
+    A.nx = s->nx;
+    A.nz = s->allnz;
+    A.g = Grid<R>{s->nx + 2 * s->pml, s->allnz + 2 * s->pml, s->pml, (R)s->dx, (R)s->dz, (R)s->dt};
+    const size_t n = A.g.n();
+    std::vector<R> vpp(n, 0), vsp(n, 0), rp(n, 0);
+    pad_window(vp, s->allnz, s->ngx_min, s->nx, A.nz, s->pml, vpp.data());
+    if (rho) pad_window(rho, s->allnz, s->ngx_min, s->nx, A.nz, s->pml, rp.data());
+    if (s->boundary == 1) random_boundary(A.g, (uint64_t)s->rb_seed, s->rb_shot, (R)s->rb_amp, vpp.data(), vsp.data());
+    A.b.assign(n, 0); A.k.assign(n, 0); A.m.assign(n, 0);
+    for (size_t c = 0; c < n; ++c) {
+        R r = s->rho_const > 0 ? (R)s->rho_const : rp[c];
+        A.b[c] = (R)1 / r;
+        A.k[c] = r * vpp[c] * vpp[c];
+    }
+    if (refl)  // reflectivity window, zero outside the physical domain
+        for (int ix = 0; ix < s->nx; ++ix)
+            for (int iz = 0; iz < A.nz; ++iz)
+                A.m[A.g.at(ix + s->pml, iz + s->pml)] = refl[(size_t)(s->ngx_min + ix) * s->allnz + iz];
+    return A;
+}
+
+template <typename R>
+inline R dplus_x(const R* f, size_t c, int nzp) {
+    R a = 0;
+    for (int i = 1; i <= NOP; ++i) a += (R)C8[i] * (f[c + (size_t)i * nzp] - f[c - (size_t)(i - 1) * nzp]);
+    return a;
+}
+template <typename R>
+inline R dplus_z(const R* f, size_t c) {
+    R a = 0;
+    for (int i = 1; i <= NOP; ++i) a += (R)C8[i] * (f[c + i] - f[c - i + 1]);
+    return a;
+}
+template <typename R>
+inline R dminus_x(const R* f, size_t c, int nzp) {
+    R a = 0;
+    for (int i = 1; i <= NOP; ++i) a += (R)C8[i] * (f[c + (size_t)(i - 1) * nzp] - f[c - (size_t)i * nzp]);
+    return a;
+}
+template <typename R>
+inline R dminus_z(const R* f, size_t c) {
+    R a = 0;
+    for (int i = 1; i <= NOP; ++i) a += (R)C8[i] * (f[c + i - 1] - f[c - i]);
+    return a;
+}
+
+// v update with the transient source: v += (dt/d) b D+ (p + s delta_src).
+template <typename R>
+void acou_v(const Grid<R>& g, std::vector<R>& vx, std::vector<R>& vz, std::vector<R>& p, const std::vector<R>& b,
+            size_t src, R sval, R sign) {
+    R keep = p[src];
+    p[src] = keep + sval;
+    const int nzp = g.nzp;
+    const R ax = g.dt / g.dx, az = g.dt / g.dz;
+#pragma omp parallel for
+    for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+        for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+            size_t c = g.at(ix, iz);
+            vx[c] += sign * (ax * b[c] * dplus_x(p.data(), c, nzp));
+            vz[c] += sign * (az * b[c] * dplus_z(p.data(), c));
+        }
+    p[src] = keep;
+}
+
+template <typename R>
+void acou_forward(const orc_shot* s, const R* vp, const R* rho, const R* refl, const R* wav, R* rec_bg,
+                  R* rec_born, R* final3) {
+    Acou<R> A = acou_setup<R>(s, vp, rho, refl);
+    const Grid<R>& g = A.g;
+    const size_t n = g.n();
+    const int nzp = g.nzp;
+    const R ax = g.dt / g.dx, az = g.dt / g.dz;
+    std::vector<R> vx(n, 0), vz(n, 0), p(n, 0), bvx(n, 0), bvz(n, 0), bp(n, 0), dp(n, 0);
+    const size_t src = g.at(s->isx, s->isz);
+    for (int it = 0; it < s->nt; ++it) {
+        acou_v(g, vx, vz, p, A.b, src, wav[it], (R)1);
+#pragma omp parallel for
+        for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+            for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+                size_t c = g.at(ix, iz);
+                bvx[c] += ax * A.b[c] * dplus_x(bp.data(), c, nzp);
+                bvz[c] += az * A.b[c] * dplus_z(bp.data(), c);
+            }
+#pragma omp parallel for
+        for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+            for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+                size_t c = g.at(ix, iz);
+                R d = A.k[c] * (ax * dminus_x(vx.data(), c, nzp) + az * dminus_z(vz.data(), c));
+                dp[c] = d;
+            }
+#pragma omp parallel for
+        for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+            for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+                size_t c = g.at(ix, iz);
+                R bd = A.k[c] * (ax * dminus_x(bvx.data(), c, nzp) + az * dminus_z(bvz.data(), c));
+                p[c] += dp[c];
+                bp[c] += bd + 2 * A.m[c] * dp[c];
+            }
+        for (int r = 0; r < s->ntr; ++r) {
+            size_t c = g.at(s->rec_ix[r], s->igz);
+            if (rec_bg) rec_bg[(size_t)r * s->nt + it] = p[c];
+            if (rec_born) rec_born[(size_t)r * s->nt + it] = bp[c];
+        }
+    }
+    if (final3) {
+        std::copy(vx.begin(), vx.end(), final3);
+        std::copy(vz.begin(), vz.end(), final3 + n);
+        std::copy(p.begin(), p.end(), final3 + 2 * n);
+    }
+}
+
+// Adjoint of acou_forward's Born map: grad (global allnx*allnz, +=).
+template <typename R>
+void acou_adjoint(const orc_shot* s, const R* vp, const R* rho, const R* wav, const R* res, R* grad_glob,
+                  R* grad_pad_out) {
+    Acou<R> A = acou_setup<R>(s, vp, rho, nullptr);
+    const Grid<R>& g = A.g;
+    const size_t n = g.n();
+    const int nzp = g.nzp;
+    const R ax = g.dt / g.dx, az = g.dt / g.dz;
+    std::vector<R> vx(n, 0), vz(n, 0), p(n, 0), lvx(n, 0), lvz(n, 0), lp(n, 0), w(n, 0), grad(n, 0), dp(n, 0);
+    const size_t src = g.at(s->isx, s->isz);
+    // background forward to t = nt
+    for (int it = 0; it < s->nt; ++it) {
+        acou_v(g, vx, vz, p, A.b, src, wav[it], (R)1);
+#pragma omp parallel for
+        for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+            for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+                size_t c = g.at(ix, iz);
+                dp[c] = A.k[c] * (ax * dminus_x(vx.data(), c, nzp) + az * dminus_z(vz.data(), c));
+            }
+#pragma omp parallel for
+        for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+            for (int iz = NOP; iz < g.nzp - NOP; ++iz) p[g.at(ix, iz)] += dp[g.at(ix, iz)];
+    }
+    auto inU = [&](int ix, int iz) { return ix >= NOP && ix < g.nxp - NOP && iz >= NOP && iz < g.nzp - NOP; };
+    for (int it = s->nt - 1; it >= 0; --it) {
+        for (int r = 0; r < s->ntr; ++r) lp[g.at(s->rec_ix[r], s->igz)] += res[(size_t)r * s->nt + it];
+        // dp_n = k D- v^{n+1}; image; p^n = p^{n+1} - dp_n
+#pragma omp parallel for
+        for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+            for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+                size_t c = g.at(ix, iz);
+                R d = A.k[c] * (ax * dminus_x(vx.data(), c, nzp) + az * dminus_z(vz.data(), c));
+                grad[c] += 2 * d * lp[c];
+                dp[c] = d;
+            }
+        // A_p^T: lv += -(dt/d) D+ (k lp) with k lp masked to U
+        for (int ix = 0; ix < g.nxp; ++ix)
+            for (int iz = 0; iz < g.nzp; ++iz) {
+                size_t c = g.at(ix, iz);
+                w[c] = inU(ix, iz) ? A.k[c] * lp[c] : 0;
+            }
+#pragma omp parallel for
+        for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+            for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+                size_t c = g.at(ix, iz);
+                p[c] -= dp[c];
+                lvx[c] += -(ax * dplus_x(w.data(), c, nzp));
+                lvz[c] += -(az * dplus_z(w.data(), c));
+            }
+        // bg velocity reversal with the transient source
+        acou_v(g, vx, vz, p, A.b, src, wav[it], (R)-1);
+        // A_v^T: lp += -(dt/d) D- (b lv) with b lv masked to U
+        std::vector<R> ux(n, 0), uz(n, 0);
+        for (int ix = 0; ix < g.nxp; ++ix)
+            for (int iz = 0; iz < g.nzp; ++iz) {
+                size_t c = g.at(ix, iz);
+                if (inU(ix, iz)) { ux[c] = A.b[c] * lvx[c]; uz[c] = A.b[c] * lvz[c]; }
+            }
+#pragma omp parallel for
+        for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+            for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+                size_t c = g.at(ix, iz);
+                lp[c] += -(ax * dminus_x(ux.data(), c, nzp) + az * dminus_z(uz.data(), c));
+            }
+    }
+    if (grad_glob)
+        for (int ix = 0; ix < s->nx; ++ix)
+            for (int iz = 0; iz < A.nz; ++iz)
+                grad_glob[(size_t)(s->ngx_min + ix) * s->allnz + iz] += grad[g.at(ix + s->pml, iz + s->pml)];
+    if (grad_pad_out) std::copy(grad.begin(), grad.end(), grad_pad_out);
+}
+
+// laplace.cpp:82-106: 5-point Laplacian on [2, n-2) with a zero frame.
+template <typename R>
+void laplace_order(int nz, int nx, R dx, R* dat) {
+    std::vector<R> t1((size_t)nx * nz), t2((size_t)nx * nz, 0);
+    std::copy(dat, dat + (size_t)nx * nz, t1.begin());
+    for (int ix = 2; ix < nx - 2; ++ix)
+        for (int iz = 2; iz < nz - 2; ++iz) {
+            size_t c = (size_t)ix * nz + iz;
+            t2[c] = ((R)1.0 / dx / dx) * (t1[c + nz] + t1[c - nz] - 2 * t1[c] + t1[c + 1] + t1[c - 1] - 2 * t1[c]);
+        }
+    std::copy(t2.begin(), t2.end(), dat);
+}
+
+// laplace.cpp:3-57 band filter, stated intent (padded stride nz+4, zero frame;
+// the reference indexes its (nx+4)(nz+4) buffers with stride nz and reads
+// uninitialised frame cells):  out = Sx( orig - (Sx o Sz)^4 orig ), S = (1/4,1/2,1/4).
+// The n3 = 2 loop re-reads the same tmp2, so it equals one Sx pass.
+template <typename R>
+void laplace_band(int nz, int nx, R* dat) {
+    const int s = nz + 4;
+    const R a = (R)0.5, b = (R)0.25;
+    std::vector<R> t0((size_t)(nx + 4) * s, 0), t1(t0.size(), 0), t2(t0.size(), 0);
+    auto I = [&](int ix, int iz) { return (size_t)(ix + 2) * s + (iz + 2); };
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iz = 0; iz < nz; ++iz) t0[I(ix, iz)] = t1[I(ix, iz)] = dat[(size_t)ix * nz + iz];
+    for (int k = 0; k < 4; ++k) {
+        for (int ix = 0; ix < nx; ++ix)
+            for (int iz = 0; iz < nz; ++iz)
+                t2[I(ix, iz)] = t1[I(ix, iz)] * a + t1[I(ix, iz - 1)] * b + t1[I(ix, iz + 1)] * b;
+        for (int ix = 0; ix < nx; ++ix)
+            for (int iz = 0; iz < nz; ++iz)
+                t1[I(ix, iz)] = t2[I(ix, iz)] * a + t2[I(ix - 1, iz)] * b + t2[I(ix + 1, iz)] * b;
+    }
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iz = 0; iz < nz; ++iz) t2[I(ix, iz)] = t0[I(ix, iz)] - t1[I(ix, iz)];
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iz = 0; iz < nz; ++iz)
+            t1[I(ix, iz)] = t2[I(ix, iz)] * a + t2[I(ix - 1, iz)] * b + t2[I(ix + 1, iz)] * b;
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iz = 0; iz < nz; ++iz) dat[(size_t)ix * nz + iz] = t1[I(ix, iz)];
+}
+
+// Generic elastic stepper for invariant tests: nsteps of (v then s) for
+// dir = +1 or (s then v) for dir = -1 with the given damping, 15-array state.
+template <typename R>
+void elastic_run(int nxp, int nzp, R dx, R dz, R dt, const R* lam, const R* mu, const R* rho, const R* dampx,
+                 const R* dampz, int nsteps, int dir, R* fields15) {
+    Grid<R> g{nxp, nzp, 0, dx, dz, dt};
+    const size_t n = g.n();
+    Medium<R> m;
+    m.lam.assign(lam, lam + n); m.mu.assign(mu, mu + n); m.rho.assign(rho, rho + n);
+    m.dampx.assign(dampx, dampx + n); m.dampz.assign(dampz, dampz + n);
+    Elastic<R> f;
+    f.init(n);
+    auto all = f.all();
+    for (size_t a = 0; a < all.size(); ++a) std::copy(fields15 + a * n, fields15 + (a + 1) * n, all[a]->begin());
+    for (int k = 0; k < nsteps; ++k) {
+        if (dir > 0) { update_v(g, f, m, 1); update_s(g, f, m, 1); }
+        else { update_s(g, f, m, -1); update_v(g, f, m, -1); }
+    }
+    for (size_t a = 0; a < all.size(); ++a) std::copy(all[a]->begin(), all[a]->end(), fields15 + a * n);
+}
+
+// Adam update of lsrtm_all_iteration (main.cpp:1092-1130) on g + b_g + b_g2.
+template <typename R>
+void adam_update(int n, int iter, const R* g, const R* bg, const R* bg2, R* s0, R* r0, R* image) {
+    const float beta1 = 0.8f, beta2 = 0.999f, alpha = 3e-5f, eps = 5e8f, gama = 0.99f;
+    float alpha_k = iter == 0 ? alpha : (float)((alpha / 2) * std::pow((double)gama, (double)iter));
+    int k = iter + 1;
+    for (int i = 0; i < n; ++i) {
+        R sum = g[i] + bg[i] + bg2[i];
+        R sk = beta1 * s0[i] + (1 - beta1) * sum;
+        R rk = beta2 * r0[i] + (1 - beta2) * (sum * sum);
+        R sm = sk / (R)(1 - std::pow((double)beta1, (double)k));
+        R rm = rk / (R)(1 - std::pow((double)beta2, (double)k));
+        R d = alpha_k * (sm / (std::sqrt(rm) + eps));
+        image[i] = image[i] - d;
+        s0[i] = sk;
+        r0[i] = rk;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// extern "C" surface (d = FP64 truth, f = FP32 restatement)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int orc_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
+
+#define DEF_T(R, SUF)                                                                                        \
+    int orc_ricker_##SUF(int nt, double f, double dt, R* s) { ricker<R>(nt, (R)f, (R)dt, s); return 0; }     \
+    int orc_pad_window_##SUF(const R* glob, int allnz, int ngx_min, int nx, int nz, int pml, R* out) {        \
+        pad_window<R>(glob, allnz, ngx_min, nx, nz, pml, out); return 0; }                                     \
+    int orc_damp_profile_##SUF(int nxp, int nzp, int pml, const R* vp, double rr, R* dx_, R* dz_) {           \
+        Grid<R> g{nxp, nzp, pml, 1, 1, 1}; damp_profile<R>(g, vp, (R)rr, dx_, dz_); return 0; }               \
+    int orc_random_boundary_##SUF(int nxp, int nzp, int pml, unsigned long long seed, int shot, double amp,    \
+                                  R* vp, R* vs) {                                                              \
+        Grid<R> g{nxp, nzp, pml, 1, 1, 1}; random_boundary<R>(g, seed, shot, (R)amp, vp, vs); return 0; }     \
+    int orc_elastic_forward_##SUF(const orc_shot* s, const R* vp, const R* vs, const R* rho, const R* wav,    \
+                                  R* rz, R* rx, R* final5) {                                                   \
+        elastic_forward<R>(s, vp, vs, rho, wav, rz, rx, final5); return 0; }                                   \
+    int orc_elsrtm_shot_##SUF(const orc_shot* s, const R* vp, const R* vs, const R* rho, const R* wav,        \
+                              const R* ipp, const R* ips, const R* oz, const R* ox, int iter, int col0,        \
+                              R* grads7, double* misfit) {                                                     \
+        elsrtm_shot<R>(s, vp, vs, rho, wav, ipp, ips, oz, ox, iter, col0, grads7, misfit); return 0; }        \
+    int orc_acoustic_ref_forward_##SUF(const orc_shot* s, const R* vel, const R* rho, const R* wav, int sxnum, \
+                                       R* rec) {                                                               \
+        acoustic_ref_forward<R>(s, vel, rho, wav, sxnum, rec); return 0; }                                     \
+    int orc_acou_forward_##SUF(const orc_shot* s, const R* vp, const R* rho, const R* refl, const R* wav,     \
+                               R* rbg, R* rborn, R* final3) {                                                  \
+        acou_forward<R>(s, vp, rho, refl, wav, rbg, rborn, final3); return 0; }                               \
+    int orc_acou_adjoint_##SUF(const orc_shot* s, const R* vp, const R* rho, const R* wav, const R* res,      \
+                               R* grad, R* gpad) {                                                             \
+        acou_adjoint<R>(s, vp, rho, wav, res, grad, gpad); return 0; }                                         \
+    int orc_laplace_order_##SUF(int nz, int nx, double dx, R* dat) { laplace_order<R>(nz, nx, (R)dx, dat); return 0; } \
+    int orc_laplace_band_##SUF(int nz, int nx, R* dat) { laplace_band<R>(nz, nx, dat); return 0; }            \
+    int orc_elastic_run_##SUF(int nxp, int nzp, double dx, double dz, double dt, const R* lam, const R* mu,   \
+                              const R* rho, const R* dampx, const R* dampz, int nsteps, int dir, R* f15) {     \
+        elastic_run<R>(nxp, nzp, (R)dx, (R)dz, (R)dt, lam, mu, rho, dampx, dampz, nsteps, dir, f15); return 0; } \
+    int orc_ncc_residual_##SUF(int ntr, int nt, int iter, R* sz, R* sx, const R* oz, const R* ox, double* mf) { \
+        ncc_residual<R>(ntr, nt, iter, sz, sx, oz, ox, mf); return 0; }                                        \
+    int orc_adam_update_##SUF(int n, int iter, const R* g, const R* bg, const R* bg2, R* s0, R* r0, R* img) { \
+        adam_update<R>(n, iter, g, bg, bg2, s0, r0, img); return 0; }
+
+DEF_T(double, d)
+DEF_T(float, f)
+
+// laplace.cpp:59-78 cutM: zero the frame outside [xi,xe) x [zi,ze).
+int orc_cutM_f(float* x, int nz, int nx, int zi, int ze, int xi, int xe) {
+    for (int ix = 0; ix < xi; ++ix) for (int iz = 0; iz < nz; ++iz) x[(size_t)ix * nz + iz] = 0.f;
+    for (int ix = xe; ix < nx; ++ix) for (int iz = 0; iz < nz; ++iz) x[(size_t)ix * nz + iz] = 0.f;
+    for (int ix = 0; ix < nx; ++ix) for (int iz = 0; iz < zi; ++iz) x[(size_t)ix * nz + iz] = 0.f;
+    for (int ix = 0; ix < nx; ++ix) for (int iz = ze; iz < nz; ++iz) x[(size_t)ix * nz + iz] = 0.f;
+    return 0;
+}
+
+double orc_rb_uniform(unsigned long long seed, int shot, int ix, int iz) { return rb_uniform(seed, shot, ix, iz); }
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+"""ctypes binding of the CPU oracle (oracle/libds2d_oracle.so) for tests.
+
+Test infrastructure only: the oracle is the checker, never the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "libds2d_oracle.so")
+REF_LAPLACE = os.path.join(ROOT, "oracle", "_ref", "libref_laplace.so")
+
+_lib = None
+
+
+class OrcShot(C.Structure):
+    _fields_ = [("allnx", C.c_int), ("allnz", C.c_int), ("ngx_min", C.c_int), ("nx", C.c_int), ("pml", C.c_int),
+                ("nt", C.c_int), ("dx", C.c_double), ("dz", C.c_double), ("dt", C.c_double), ("boundary", C.c_int),
+                ("rr", C.c_double), ("rb_seed", C.c_ulonglong), ("rb_shot", C.c_int), ("rb_amp", C.c_double),
+                ("rho_const", C.c_double), ("isx", C.c_int), ("isz", C.c_int), ("igz", C.c_int), ("ntr", C.c_int),
+                ("rec_ix", C.POINTER(C.c_int))]
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE):
+            raise RuntimeError("oracle not built: make -C oracle")
+        _lib = C.CDLL(ORACLE)
+        _lib.orc_rb_uniform.restype = C.c_double
+        _lib.orc_rb_uniform.argtypes = [C.c_ulonglong, C.c_int, C.c_int, C.c_int]
+    return _lib
+
+
+def threads(n=0):
+    return load().orc_threads(n)
+
+
+def P(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def dtype_of(prec):
+    return np.float64 if prec == "d" else np.float32
+
+
+def make_shot(spec, allnx, allnz, pml, nt, dx, dz, dt, *, rr=1e-3, rb_seed=1234, rb_amp=0.3, rho_const=0.0):
+    """OrcShot from a ds2d_amd ShotSpec (keeps rec_ix alive on the struct)."""
+    rix = np.ascontiguousarray(spec.rec_ix, np.int32)
+    s = OrcShot(allnx, allnz, spec.ngx_min, spec.nx, pml, nt, dx, dz, dt, spec.boundary, rr, rb_seed,
+                spec.shot_id, rb_amp, rho_const, spec.isx, spec.isz, spec.igz, rix.size,
+                rix.ctypes.data_as(C.POINTER(C.c_int)))
+    s._keep = rix
+    return s
+
+
+def _c(a, prec):
+    return None if a is None else np.ascontiguousarray(a, dtype_of(prec))
+
+
+def elastic_forward(shot, vp, vs, rho, wav, prec="d", final=False):
+    lib = load()
+    t = dtype_of(prec)
+    rz = np.zeros(shot.ntr * shot.nt, t)
+    rx = np.zeros_like(rz)
+    n = (shot.nx + 2 * shot.pml) * (shot.allnz + 2 * shot.pml)
+    f5 = np.zeros(5 * n, t) if final else None
+    vp, vs, rho, wav = _c(vp, prec), _c(vs, prec), _c(rho, prec), _c(wav, prec)
+    getattr(lib, f"orc_elastic_forward_{prec}")(C.byref(shot), P(vp), P(vs), P(rho), P(wav), P(rz), P(rx), P(f5))
+    return (rz, rx, f5) if final else (rz, rx)
+
+
+def elsrtm_shot(shot, vp, vs, rho, wav, img_pp, img_ps, obs_z, obs_x, it, col0, grads7=None, prec="d"):
+    lib = load()
+    t = dtype_of(prec)
+    A = shot.allnx * shot.allnz
+    if grads7 is None:
+        grads7 = np.zeros(7 * A, t)
+    mf = C.c_double(0.0)
+    args = [_c(a, prec) for a in (vp, vs, rho, wav, img_pp, img_ps, obs_z, obs_x)]
+    getattr(lib, f"orc_elsrtm_shot_{prec}")(C.byref(shot), *[P(a) for a in args], it, col0, P(grads7), C.byref(mf))
+    return grads7, mf.value
+
+
+def acoustic_ref_forward(shot, vel, rho, wav, sxnum, prec="d"):
+    lib = load()
+    rec = np.zeros(shot.nx * shot.nt, dtype_of(prec))
+    vel, rho, wav = _c(vel, prec), _c(rho, prec), _c(wav, prec)
+    getattr(lib, f"orc_acoustic_ref_forward_{prec}")(C.byref(shot), P(vel), P(rho), P(wav), sxnum, P(rec))
+    return rec
+
+
+def acou_forward(shot, vp, rho, refl, wav, prec="d", final=False):
+    lib = load()
+    t = dtype_of(prec)
+    rbg = np.zeros(shot.ntr * shot.nt, t)
+    rbn = np.zeros_like(rbg)
+    n = (shot.nx + 2 * shot.pml) * (shot.allnz + 2 * shot.pml)
+    f3 = np.zeros(3 * n, t) if final else None
+    vp, rho, refl, wav = _c(vp, prec), _c(rho, prec), _c(refl, prec), _c(wav, prec)
+    getattr(lib, f"orc_acou_forward_{prec}")(C.byref(shot), P(vp), P(rho), P(refl), P(wav), P(rbg), P(rbn), P(f3))
+    return (rbg, rbn, f3) if final else (rbg, rbn)
+
+
+def acou_adjoint(shot, vp, rho, wav, res, prec="d"):
+    lib = load()
+    t = dtype_of(prec)
+    g = np.zeros(shot.allnx * shot.allnz, t)
+    vp, rho, wav, res = _c(vp, prec), _c(rho, prec), _c(wav, prec), _c(res, prec)
+    getattr(lib, f"orc_acou_adjoint_{prec}")(C.byref(shot), P(vp), P(rho), P(wav), P(res), P(g), None)
+    return g
+
+
+def ricker(nt, f, dt, prec="d"):
+    s = np.zeros(nt, dtype_of(prec))
+    getattr(load(), f"orc_ricker_{prec}")(nt, C.c_double(f), C.c_double(dt), P(s))
+    return s
+
+
+def pad_window(glob, ngx_min, nx, pml, prec="d"):
+    allnx, allnz = glob.shape
+    out = np.zeros((nx + 2 * pml, allnz + 2 * pml), dtype_of(prec))
+    g = _c(glob, prec)
+    getattr(load(), f"orc_pad_window_{prec}")(P(g), allnz, ngx_min, nx, allnz, pml, P(out))
+    return out
+
+
+def damp_profile(vp_pad, pml, rr, prec="d"):
+    nxp, nzp = vp_pad.shape
+    dx = np.zeros_like(vp_pad, dtype=dtype_of(prec))
+    dz = np.zeros_like(dx)
+    v = _c(vp_pad, prec)
+    getattr(load(), f"orc_damp_profile_{prec}")(nxp, nzp, pml, P(v), C.c_double(rr), P(dx), P(dz))
+    return dx, dz
+
+
+def random_boundary(vp_pad, vs_pad, pml, seed, shot, amp, prec="d"):
+    nxp, nzp = vp_pad.shape
+    vp = _c(vp_pad, prec).copy()
+    vs = _c(vs_pad, prec).copy()
+    getattr(load(), f"orc_random_boundary_{prec}")(nxp, nzp, pml, C.c_ulonglong(seed), shot, C.c_double(amp),
+                                                   P(vp), P(vs))
+    return vp, vs
+
+
+def laplace_order(img, dx, prec="d"):
+    a = _c(img, prec).copy()
+    nx, nz = a.shape
+    getattr(load(), f"orc_laplace_order_{prec}")(nz, nx, C.c_double(dx), P(a))
+    return a
+
+
+def laplace_band(img, prec="d"):
+    a = _c(img, prec).copy()
+    nx, nz = a.shape
+    getattr(load(), f"orc_laplace_band_{prec}")(nz, nx, P(a))
+    return a
+
+
+def elastic_run(lam, mu, rho, dampx, dampz, fields15, nsteps, direction, dx, dz, dt, prec="d"):
+    nxp, nzp = lam.shape
+    f = _c(fields15, prec).copy()
+    args = [_c(a, prec) for a in (lam, mu, rho, dampx, dampz)]
+    getattr(load(), f"orc_elastic_run_{prec}")(nxp, nzp, C.c_double(dx), C.c_double(dz), C.c_double(dt),
+                                               *[P(a) for a in args], nsteps, direction, P(f))
+    return f
+
+
+def ncc_residual(syn_z, syn_x, obs_z, obs_x, ntr, nt, it, prec="d"):
+    sz, sx = _c(syn_z, prec).copy(), _c(syn_x, prec).copy()
+    oz, ox = _c(obs_z, prec), _c(obs_x, prec)
+    mf = C.c_double(0.0)
+    getattr(load(), f"orc_ncc_residual_{prec}")(ntr, nt, it, P(sz), P(sx), P(oz), P(ox), C.byref(mf))
+    return sz, sx, mf.value
+
+
+def adam_update(g, bg, bg2, s0, r0, img, it, prec="d"):
+    s0, r0, img = _c(s0, prec).copy(), _c(r0, prec).copy(), _c(img, prec).copy()
+    g, bg, bg2 = _c(g, prec), _c(bg, prec), _c(bg2, prec)
+    getattr(load(), f"orc_adam_update_{prec}")(g.size, it, P(g), P(bg), P(bg2), P(s0), P(r0), P(img))
+    return s0, r0, img
+
+
+def rb_uniform(seed, shot, ix, iz):
+    return load().orc_rb_uniform(seed, shot, ix, iz)
+
+
+def ref_laplace_available():
+    return os.path.exists(REF_LAPLACE)
+
+
+def ref_laplace_order(img, dx):
+    lib = C.CDLL(REF_LAPLACE)
+    a = np.ascontiguousarray(img, np.float32).copy()
+    nx, nz = a.shape
+    lib.ref_laplace_order(nz, nx, 2, C.c_float(dx), a.ctypes.data_as(C.c_void_p))
+    return a
+
+
+def ref_cutM(img, zi, ze, xi, xe):
+    lib = C.CDLL(REF_LAPLACE)
+    a = np.ascontiguousarray(img, np.float32).copy()
+    nx, nz = a.shape
+    lib.ref_cutM(a.ctypes.data_as(C.c_void_p), nz, nx, zi, ze, xi, xe)
+    return a
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
