@@ -67,6 +67,7 @@ struct ds2d_ctx {
     std::map<std::string, DevBuf> ws;  // grow-only workspace
     hipEvent_t ev[4]{};
     ds2d_stats last{};
+    int chunk_w = 0;  // fused-kernel x-chunk width (0: default)
 
     ~ds2d_ctx() {
         for (auto& kv : ws)
@@ -109,7 +110,7 @@ struct Batch {
     std::vector<SlotDesc> slots;
     std::vector<int> rix, tbase;
     BatchShape shape{};
-    long long total_cells = 0, total_rec = 0, total_win = 0, total_edge = 0, total_rmap = 0;
+    long long total_cells = 0, total_rec = 0, total_win = 0, total_edge = 0, total_rmap = 0, total_hist = 0;
     int total_traces = 0;
     bool any_pml = false;
 };
@@ -164,6 +165,8 @@ int build_batch(ds2d_ctx* h, int nshots, const ds2d_shot* shots, Batch& B) {
             eoff += (long long)g.nt * (2LL * (2 * NOP) * q.nx + 2LL * (2 * NOP) * g.allnz);
         }
         moff += nxp;
+        S.hist_off = B.total_hist;
+        B.total_hist += (long long)g.nt * q.nx * g.allnz;
         B.tbase.push_back(B.total_traces);
         B.total_traces += q.ntr;
         for (int r = 0; r < q.ntr; ++r) B.rix.push_back(q.rec_ix[r]);
@@ -280,6 +283,38 @@ int check_h(ds2d_ctx* h) {
     HIP_TRY(hipSetDevice(0));
     return 0;
 }
+
+// Event triplets around sampled steps of a two-kernel time loop: [a] k0 [b] k1 [c].
+struct StepSampler {
+    std::vector<hipEvent_t> ev;
+    int every = 1, n = 0;
+    int init(int nt, int max_samples = 48) {
+        every = std::max(1, nt / max_samples);
+        n = 0;
+        ev.resize(3 * (nt / every + 1));
+        for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+        return 0;
+    }
+    bool take(int it) const { return it % every == 0; }
+    void mark(hipStream_t st, int k) { (void)hipEventRecord(ev[3 * n + k], st); if (k == 2) ++n; }
+    // average durations of kernel 0 and kernel 1 (ms); call after a stream sync
+    void averages(double* k0, double* k1) {
+        double a = 0, b = 0;
+        for (int i = 0; i < n; ++i) {
+            float x = 0, y = 0;
+            (void)hipEventElapsedTime(&x, ev[3 * i], ev[3 * i + 1]);
+            (void)hipEventElapsedTime(&y, ev[3 * i + 1], ev[3 * i + 2]);
+            a += x;
+            b += y;
+        }
+        *k0 = n ? a / n : 0;
+        *k1 = n ? b / n : 0;
+    }
+    ~StepSampler() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
 
 }  // namespace
 
@@ -525,12 +560,15 @@ int ds2d_take_gradients(ds2d_t* h, float* pp, float* ps, float* il, float* bpp, 
 // ---------------------------------------------------------------------------
 // Elastic forward modelling
 // ---------------------------------------------------------------------------
-int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec_z, float* rec_x, int mode,
-                       ds2d_stats* stats) {
+// Forward modelling for the first nt_run steps (nt_run <= nt; the autotuner
+// times short runs).
+static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec_z, float* rec_x, int mode,
+                           ds2d_stats* stats, int nt_run) {
     int rc;
     if ((rc = check_h(h))) return rc;
     if (!h->have_model) return fail(DS2D_ESTATE, "no model");
     if (h->g.physics != 0) return fail(DS2D_EINVAL, "elastic forward needs physics = 0");
+    if (mode < 0 || mode > 2) return fail(DS2D_EINVAL, "mode must be 0, 1 or 2");
     Batch B;
     if ((rc = build_batch(h, nshots, shots, B))) return rc;
     const BatchShape& sh = B.shape;
@@ -544,27 +582,82 @@ int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec
     float *rz = rs, *rx = rs + R;
     EField f;
     if ((rc = alloc_efield(h, "bg", n, f, false))) return rc;
-    const int nt = h->g.nt;
+    const int nt = std::max(1, std::min(nt_run, h->g.nt));
     long long cu = 0;
     for (auto& S : B.slots) cu += (long long)S.nxp * S.nzp;
     cu *= nt;
     stats_begin(h);
     launch_setup(st, sh, setup_args(h, false), m);
     HIP_TRY(hipMemsetAsync(f.vx, 0, (size_t)n * 15 * 4, st));
-    double bytes = 0;
+    double bytes_v = 0, bytes_s = 0;
     long long launches = 0;
+    StepSampler smp;
+    if ((rc = smp.init(nt))) return rc;
     if (mode == 0) {
         Strain none{nullptr, nullptr, nullptr, nullptr};
         stats_loop_begin(h);
         for (int it = 0; it < nt; ++it) {
+            const bool tk = smp.take(it);
             launch_inject_src(st, sh, f.txx, f.tzz, h->d_wav, it);
+            if (tk) smp.mark(st, 0);
             launch_ref_v(st, sh, f, m, 1.f, 0);
+            if (tk) smp.mark(st, 1);
             launch_ref_s(st, sh, f, m, none, 1.f, 0);
+            if (tk) smp.mark(st, 2);
             launch_record(st, sh, f.vz, f.vx, rz, rx, nullptr, nullptr, nullptr, nullptr, it);
         }
         stats_loop_end(h);
         launches = (long long)nt;
-        bytes = (double)n * 21 * 4;  // split-layout stress kernel: 12 read + 9 written per cell
+        // split layout, per padded cell: v reads 10 writes 6; s reads 12 writes 9
+        bytes_v = (double)B.total_cells * 16 * 4;
+        bytes_s = (double)B.total_cells * 21 * 4;
+    } else if (mode == 2 && sh.nzp <= FUSED_MAX_NZP) {
+        // single fused kernel per step, ping-pong state
+        float* par;
+        int* rmap;
+        if ((rc = h->fbuf("fpar", (size_t)n * 3, &par)) || (rc = h->fbuf("rmap", B.total_rmap, &rmap))) return rc;
+        launch_fast_params(st, m.rho, m.lam, m.mu, par, par + n, par + 2 * n, n, h->g.dt);
+        HIP_TRY(hipMemsetAsync(rmap, 0xff, B.total_rmap * 4, st));
+        launch_fast_rmap(st, sh, rmap);
+        EField g2;
+        if ((rc = alloc_efield(h, "b", n, g2, true))) return rc;
+        std::vector<ChunkDesc> ch;
+        const int W = h->chunk_w > 0 ? h->chunk_w : 64;
+        for (int s = 0; s < nshots; ++s)
+            for (int x0 = 0; x0 < B.slots[s].nxp; x0 += W)
+                ch.push_back(ChunkDesc{s, x0, std::min(x0 + W, B.slots[s].nxp), 0});
+        ChunkDesc* d_ch;
+        if ((rc = h->fbuf("chunks", ch.size(), &d_ch))) return rc;
+        HIP_TRY(hipMemcpyAsync(d_ch, ch.data(), ch.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice, st));
+        auto mk = [&](const EField& A, const EField& Bf) {
+            FusedArgs q{};
+            q.vx_a = A.vx; q.vz_a = A.vz; q.txx_a = A.txx; q.tzz_a = A.tzz; q.txz_a = A.txz;
+            q.vx_x_a = A.vx_x; q.vx_z_a = A.vx_z; q.vz_x_a = A.vz_x; q.vz_z_a = A.vz_z;
+            q.txx_x_a = A.txx_x; q.txx_z_a = A.txx_z; q.tzz_x_a = A.tzz_x; q.tzz_z_a = A.tzz_z;
+            q.txz_x_a = A.txz_x; q.txz_z_a = A.txz_z;
+            q.vx_b = Bf.vx; q.vz_b = Bf.vz; q.txx_b = Bf.txx; q.tzz_b = Bf.tzz; q.txz_b = Bf.txz;
+            q.vx_x_b = Bf.vx_x; q.vx_z_b = Bf.vx_z; q.vz_x_b = Bf.vz_x; q.vz_z_b = Bf.vz_z;
+            q.txx_x_b = Bf.txx_x; q.txx_z_b = Bf.txx_z; q.tzz_x_b = Bf.tzz_x; q.tzz_z_b = Bf.tzz_z;
+            q.txz_x_b = Bf.txz_x; q.txz_z_b = Bf.txz_z;
+            q.bx = par; q.lam = par + n; q.mu = par + 2 * n;
+            q.dampx = m.dampx; q.dampz = m.dampz;
+            q.wav = h->d_wav;
+            q.rec_z = rz; q.rec_x = rx;
+            return q;
+        };
+        const FusedArgs even = mk(f, g2), odd = mk(g2, f);
+        stats_loop_begin(h);
+        for (int it = 0; it < nt; ++it) {
+            const bool tk = smp.take(it);
+            if (tk) smp.mark(st, 0);
+            launch_fused_step(st, sh, (it & 1) ? odd : even, d_ch, (int)ch.size(), rmap, it);
+            if (tk) { smp.mark(st, 1); smp.mark(st, 2); }
+        }
+        stats_loop_end(h);
+        launches = (long long)nt;
+        // unsplit interior, per padded cell: reads stress 3 + velocity 2 + dt/rho, lambda, mu; writes 5
+        bytes_v = (double)B.total_cells * 13 * 4;
+        bytes_s = 0;
     } else {
         float* par;
         int* rmap;
@@ -583,14 +676,30 @@ int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec
         a.rec_z = rz; a.rec_x = rx;
         stats_loop_begin(h);
         for (int it = 0; it < nt; ++it) {
+            const bool tk = smp.take(it);
+            if (tk) smp.mark(st, 0);
             launch_fast_v(st, sh, a, rmap, it);
+            if (tk) smp.mark(st, 1);
             launch_fast_s(st, sh, a);
+            if (tk) smp.mark(st, 2);
         }
         stats_loop_end(h);
         launches = (long long)nt;
-        bytes = (double)n * 10 * 4;  // unsplit stress kernel: 7 read + 3 written per cell
+        // unsplit layout, per padded cell: v reads 6 writes 2; s reads 7 writes 3
+        bytes_v = (double)B.total_cells * 8 * 4;
+        bytes_s = (double)B.total_cells * 10 * 4;
     }
-    if ((rc = stats_end(h, stats, cu, launches, bytes))) return rc;
+    if ((rc = stats_end(h, stats, cu, launches, 0))) return rc;
+    {
+        double kv, ks;
+        smp.averages(&kv, &ks);
+        // report the kernel with the larger average duration as the dominant one
+        const bool s_dom = ks >= kv && bytes_s > 0;
+        h->last.kernel_avg_ms = s_dom ? ks : kv;
+        h->last.alg_bytes = s_dom ? bytes_s : bytes_v;
+        h->last.kernel_samples = smp.n;
+        if (stats) *stats = h->last;
+    }
     if (rec_z) HIP_TRY(hipMemcpyAsync(rec_z, rz, B.total_rec * 4, hipMemcpyDeviceToHost, st));
     if (rec_x) HIP_TRY(hipMemcpyAsync(rec_x, rx, B.total_rec * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -598,11 +707,47 @@ int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec
     return 0;
 }
 
+int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec_z, float* rec_x, int mode,
+                       ds2d_stats* stats) {
+    return batch_forward_n(h, nshots, shots, rec_z, rec_x, mode, stats, h ? h->g.nt : 0);
+}
+
+int ds2d_set_chunk(ds2d_t* h, int width) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (width < 0 || (width > 0 && width < 8)) return fail(DS2D_EINVAL, "chunk width must be 0 or >= 8");
+    h->chunk_w = width;
+    return 0;
+}
+
+// Time nsteps of the fused kernel for each candidate x-chunk width on this
+// batch (after one untimed warm run) and keep the fastest.
+int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nsteps, int* best_width,
+                          double* best_ms) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    static const int cand[] = {16, 24, 32, 48, 64, 96, 128, 192};
+    int bw = 64;
+    double bt = 1e300;
+    const int keep = h->chunk_w;
+    for (int w : cand) {
+        h->chunk_w = w;
+        ds2d_stats s{};
+        if ((rc = batch_forward_n(h, nshots, shots, nullptr, nullptr, 2, &s, nsteps))) { h->chunk_w = keep; return rc; }
+        if ((rc = batch_forward_n(h, nshots, shots, nullptr, nullptr, 2, &s, nsteps))) { h->chunk_w = keep; return rc; }
+        if (s.stencil_ms < bt) { bt = s.stencil_ms; bw = w; }
+    }
+    h->chunk_w = bw;
+    if (best_width) *best_width = bw;
+    if (best_ms) *best_ms = bt;
+    return 0;
+}
+
 int ds2d_forward(ds2d_t* h, float* rec_z, float* rec_x) {
     int rc;
     if ((rc = check_h(h))) return rc;
     if (!h->have_shot) return fail(DS2D_ESTATE, "no shot set");
-    return ds2d_batch_forward(h, 1, &h->cur, rec_z, rec_x, 1, nullptr);
+    return ds2d_batch_forward(h, 1, &h->cur, rec_z, rec_x, 2, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -664,6 +809,8 @@ struct AcouWS {
     float *win = nullptr;       // per-slot window images
     float *rec = nullptr;       // R floats: bg records
     float *rec2 = nullptr;      // R floats: born records / residual
+    float *hist = nullptr;      // resident background increments (store mode)
+    bool store = false;
     long long n = 0, R = 0, W = 0;
 };
 
@@ -689,6 +836,11 @@ int acou_prepare(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouWS
     if ((rc = h->fbuf("acrec", (size_t)w.R * 2, &w.rec))) return rc;
     w.rec2 = w.rec + w.R;
     if ((rc = h->fbuf("acwin", (size_t)w.W, &w.win))) return rc;
+    // background mode: keep the per-step increment resident when it fits
+    const double hist_gb = (double)B.total_hist * 4 / 1e9;
+    const double budget = h->g.hbm_budget_gb > 0 ? h->g.hbm_budget_gb : 64.0;
+    w.store = h->g.bg_mode == 1 || (h->g.bg_mode == 0 && hist_gb <= budget);
+    if (w.store && (rc = h->fbuf("hist", (size_t)B.total_hist, &w.hist))) return rc;
     return 0;
 }
 
@@ -703,32 +855,35 @@ int acou_set_refl(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouW
     return 0;
 }
 
-// Background + Born forward from rest; records in w.rec (bg) / w.rec2 (born).
-void acou_fwd(ds2d_ctx* h, const Batch& B, AcouWS& w, bool with_born, bool rec_bg) {
+// Forward pass from rest.  do_bg: propagate the background (storing its
+// increment when w.store); otherwise the Born field reads the stored
+// increment.  Records: w.rec (bg), w.rec2 (born).
+void acou_fwd(ds2d_ctx* h, const Batch& B, AcouWS& w, bool do_bg, bool do_born, bool rec_bg) {
     hipStream_t st = h->st;
     (void)hipMemsetAsync(w.bg.vx, 0, (size_t)w.n * 6 * 4, st);  // bg + bn
+    const int hmode = do_bg ? (w.store ? 1 : 0) : 2;
     for (int it = 0; it < h->g.nt; ++it) {
-        launch_acou_v(st, B.shape, w.bg, w.bn, w.m, h->d_wav, it, 1.f, with_born);
-        launch_acou_p(st, B.shape, w.bg, w.bn, w.m, rec_bg ? w.rec : nullptr, with_born ? w.rec2 : nullptr, it,
-                      with_born);
+        launch_acou_v(st, B.shape, w.bg, w.bn, w.m, h->d_wav, it, 1.f, do_bg, do_born);
+        launch_acou_p(st, B.shape, w.bg, w.bn, w.m, w.hist, rec_bg ? w.rec : nullptr, do_born ? w.rec2 : nullptr, it,
+                      do_bg, do_born, hmode);
     }
 }
 
-// Backward pass from the background state left by acou_fwd: residual in
-// w.rec2 (device), gradient accumulated in w.grad_pad (zeroed here).
+// Backward pass (residual in w.rec2): the gradient is accumulated in
+// w.grad_pad from the stored increment, or from the background rebuilt by
+// reversal from the state acou_fwd left at t = nt.
 void acou_bwd(ds2d_ctx* h, const Batch& B, AcouWS& w) {
     hipStream_t st = h->st;
     (void)hipMemsetAsync(w.lm.vx, 0, (size_t)w.n * 3 * 4, st);
     (void)hipMemsetAsync(w.grad_pad, 0, (size_t)w.n * 4, st);
     for (int it = h->g.nt - 1; it >= 0; --it) {
-        launch_acou_adj_p(st, B.shape, w.bg, w.lm, w.m, w.grad_pad, w.rec2, it);
-        launch_acou_adj_v(st, B.shape, w.bg, w.lm, w.m, h->d_wav, it);
+        launch_acou_adj_p(st, B.shape, w.bg, w.lm, w.m, w.grad_pad, w.store ? w.hist : nullptr, w.rec2, it);
+        launch_acou_adj_v(st, B.shape, w.bg, w.lm, w.m, h->d_wav, it, w.store ? 0 : 1);
     }
 }
 
 // Padded per-slot gradient -> global (deterministic slot order), overwrite.
 int acou_grad_to_global(ds2d_ctx* h, const Batch& B, AcouWS& w, float* d_glob) {
-    // copy interior of each slot into the window arena, then gather
     for (size_t s = 0; s < B.slots.size(); ++s) {
         const SlotDesc& S = B.slots[s];
         HIP_TRY(hipMemcpy2DAsync(w.win + S.win_off, (size_t)S.nz * 4,
@@ -745,10 +900,17 @@ int acou_begin(ds2d_ctx* h, int nshots, const ds2d_shot* shots, Batch& B) {
     int rc;
     if ((rc = check_h(h))) return rc;
     if (!h->have_model) return fail(DS2D_ESTATE, "no model");
+    if (!shots) return fail(DS2D_EINVAL, "null shots");
     for (int s = 0; s < nshots; ++s)
         if (shots[s].boundary != 1)
             return fail(DS2D_EINVAL, "plain Born/adjoint needs the random boundary (shot %d)", s);
     return build_batch(h, nshots, shots, B);
+}
+
+long long padded_cells(const Batch& B) {
+    long long cu = 0;
+    for (auto& S : B.slots) cu += (long long)S.nxp * S.nzp;
+    return cu;
 }
 
 }  // namespace
@@ -760,19 +922,19 @@ int ds2d_born_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, const float
     int rc;
     Batch B;
     if ((rc = acou_begin(h, nshots, shots, B))) return rc;
+    if (!refl) return fail(DS2D_EINVAL, "null reflectivity");
     const size_t A = (size_t)h->g.allnx * h->g.allnz;
     float* d_refl;
     if ((rc = h->fbuf("refl", A, &d_refl))) return rc;
     HIP_TRY(hipMemcpyAsync(d_refl, refl, A * 4, hipMemcpyHostToDevice, h->st));
     AcouWS w;
     if ((rc = acou_prepare(h, B, d_refl, w))) return rc;
+    w.store = false;  // a single pass needs no history
     stats_begin(h);
     stats_loop_begin(h);
-    acou_fwd(h, B, w, true, true);
+    acou_fwd(h, B, w, true, true, true);
     stats_loop_end(h);
-    long long cu = 0;
-    for (auto& S : B.slots) cu += (long long)S.nxp * S.nzp;
-    if ((rc = stats_end(h, stats, cu * h->g.nt * 2, h->g.nt, (double)B.total_cells * 10 * 4))) return rc;
+    if ((rc = stats_end(h, stats, padded_cells(B) * h->g.nt * 2, h->g.nt, 0))) return rc;
     if (rec_bg) HIP_TRY(hipMemcpyAsync(rec_bg, w.rec, B.total_rec * 4, hipMemcpyDeviceToHost, h->st));
     if (rec_born) HIP_TRY(hipMemcpyAsync(rec_born, w.rec2, B.total_rec * 4, hipMemcpyDeviceToHost, h->st));
     HIP_TRY(hipStreamSynchronize(h->st));
@@ -785,11 +947,12 @@ int ds2d_born_adjoint(ds2d_t* h, int nshots, const ds2d_shot* shots, const float
     int rc;
     Batch B;
     if ((rc = acou_begin(h, nshots, shots, B))) return rc;
+    if (!res || !grad) return fail(DS2D_EINVAL, "null residual or gradient");
     AcouWS w;
     if ((rc = acou_prepare(h, B, nullptr, w))) return rc;
     stats_begin(h);
     stats_loop_begin(h);
-    acou_fwd(h, B, w, false, false);
+    acou_fwd(h, B, w, true, false, false);
     HIP_TRY(hipMemcpyAsync(w.rec2, res, B.total_rec * 4, hipMemcpyHostToDevice, h->st));
     acou_bwd(h, B, w);
     stats_loop_end(h);
@@ -797,24 +960,23 @@ int ds2d_born_adjoint(ds2d_t* h, int nshots, const ds2d_shot* shots, const float
     float* d_g;
     if ((rc = h->fbuf("gglob", A, &d_g))) return rc;
     if ((rc = acou_grad_to_global(h, B, w, d_g))) return rc;
-    long long cu = 0;
-    for (auto& S : B.slots) cu += (long long)S.nxp * S.nzp;
-    if ((rc = stats_end(h, stats, cu * h->g.nt * 3, 2LL * h->g.nt, (double)B.total_cells * 10 * 4))) return rc;
+    if ((rc = stats_end(h, stats, padded_cells(B) * h->g.nt * (w.store ? 2 : 3), 2LL * h->g.nt, 0))) return rc;
     HIP_TRY(hipMemcpyAsync(grad, d_g, A * 4, hipMemcpyDeviceToHost, h->st));
     HIP_TRY(hipStreamSynchronize(h->st));
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-// CGLS on ||L m - d||^2 (Hestenes-Stiefel form).  Each iteration propagates the
-// background with the Born field (q = L p) and then runs the adjoint
-// backward from that background state (s = L^T r).
+// CGLS on ||L m - d||^2.  Store mode: one background pass fills the resident
+// increment history, then every iteration is one Born pass (q = L p) and one
+// adjoint pass (s = L^T r).  Reconstruct mode: every Born pass carries the
+// background and every adjoint pass reverses it.
 int ds2d_lsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_obs, int niter, float* image,
                   double* resid, ds2d_stats* stats) {
     int rc;
     Batch B;
     if ((rc = acou_begin(h, nshots, shots, B))) return rc;
-    if (niter < 0) return fail(DS2D_EINVAL, "niter < 0");
+    if (niter < 0 || !d_obs || !image) return fail(DS2D_EINVAL, "bad CG arguments");
     const long long A = (long long)h->g.allnx * h->g.allnz;
     hipStream_t st = h->st;
     float *x, *p, *s, *d, *r, *q;
@@ -826,10 +988,13 @@ int ds2d_lsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_
     HIP_TRY(hipMemcpyAsync(d, d_obs, B.total_rec * 4, hipMemcpyHostToDevice, st));
     AcouWS w;
     if ((rc = acou_prepare(h, B, x, w))) return rc;
+    const long long cells = padded_cells(B);
+    long long props = 0;
     stats_begin(h);
     stats_loop_begin(h);
-    // r = d - L x
-    acou_fwd(h, B, w, true, false);
+    // r = d - L x (this pass also fills the history in store mode)
+    acou_fwd(h, B, w, true, true, false);
+    props += 2;
     HIP_TRY(hipMemcpyAsync(r, d, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
     launch_axpby(st, r, w.rec2, B.total_rec, -1.f, 1.f);
     const double dd = host_dot(h, d, d, B.total_rec, &rc);
@@ -838,22 +1003,24 @@ int ds2d_lsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_
     double rr = host_dot(h, r, r, B.total_rec, &rc);
     if (rc) return rc;
     if (resid) resid[0] = std::sqrt(rr / dd);
-    // s = L^T r (background state is at t = nt after the forward above)
+    // s = L^T r
     HIP_TRY(hipMemcpyAsync(w.rec2, r, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
     acou_bwd(h, B, w);
+    props += w.store ? 1 : 2;
     if ((rc = acou_grad_to_global(h, B, w, s))) return rc;
     HIP_TRY(hipMemcpyAsync(p, s, A * 4, hipMemcpyDeviceToDevice, st));
     double gamma = host_dot(h, s, s, A, &rc);
     if (rc) return rc;
-    long long props = 3;
     for (int k = 0; k < niter; ++k) {
         if (gamma <= 0) {
-            if (resid) for (int j = k; j < niter; ++j) resid[j + 1] = std::sqrt(rr / dd);
+            if (resid)
+                for (int j = k; j < niter; ++j) resid[j + 1] = std::sqrt(rr / dd);
             break;
         }
         // q = L p
         if ((rc = acou_set_refl(h, B, p, w))) return rc;
-        acou_fwd(h, B, w, true, false);
+        acou_fwd(h, B, w, !w.store, true, false);
+        props += w.store ? 1 : 2;
         HIP_TRY(hipMemcpyAsync(q, w.rec2, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
         const double qq = host_dot(h, q, q, B.total_rec, &rc);
         if (rc) return rc;
@@ -866,18 +1033,16 @@ int ds2d_lsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_
         // s = L^T r
         HIP_TRY(hipMemcpyAsync(w.rec2, r, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
         acou_bwd(h, B, w);
+        props += w.store ? 1 : 2;
         if ((rc = acou_grad_to_global(h, B, w, s))) return rc;
         const double gnew = host_dot(h, s, s, A, &rc);
         if (rc) return rc;
         const double beta = gamma > 0 ? gnew / gamma : 0.0;
         launch_axpby(st, p, s, A, 1.f, (float)beta);  // p = s + beta p
         gamma = gnew;
-        props += 4;
     }
     stats_loop_end(h);
-    long long cu = 0;
-    for (auto& S : B.slots) cu += (long long)S.nxp * S.nzp;
-    if ((rc = stats_end(h, stats, cu * h->g.nt * props, 0, 0))) return rc;
+    if ((rc = stats_end(h, stats, cells * h->g.nt * props, 0, 0))) return rc;
     HIP_TRY(hipMemcpyAsync(image, x, A * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipGetLastError());

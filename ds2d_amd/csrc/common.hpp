@@ -30,6 +30,7 @@ struct SlotDesc {
     int shot_id;
     int boundary;        // 0 PML, 1 random boundary
     int rmap_off;        // first column of this slot in the receiver-map arena (nxp ints)
+    long long hist_off;  // first float of this slot's background-increment history (nt*nx*nz)
 };
 
 // The reference's split-PML elastic wavefield (15 arrays, FD2DGPU.cuh:280-294).

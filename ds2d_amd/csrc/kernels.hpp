@@ -73,13 +73,15 @@ struct AcouMed {
     const float *damp, *rho;      // reference acoustic layout only
 };
 void launch_acou_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, const float* wav,
-                   int it, float sign, int with_born);
-void launch_acou_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, float* rec_bg,
-                   float* rec_born, int it, int with_born);
+                   int it, float sign, int do_bg, int do_born);
+// hist_mode: 0 none, 1 store the background increment, 2 read it (do_bg = 0)
+void launch_acou_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, float* hist,
+                   float* rec_bg, float* rec_born, int it, int do_bg, int do_born, int hist_mode);
+// hist != null: gradient from the stored increment, background untouched
 void launch_acou_adj_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m, float* grad,
-                       const float* res, int it);
+                       const float* hist, const float* res, int it);
 void launch_acou_adj_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m,
-                       const float* wav, int it);
+                       const float* wav, int it, int do_bg);
 void launch_acou_ref_step(hipStream_t st, const BatchShape& b, float* px, float* pz, float* p, float* vx, float* vz,
                           const float* k, const float* rho, const float* damp, const float* wav, int sxnum,
                           float* rec, int it);
@@ -99,5 +101,25 @@ void launch_fast_s(hipStream_t st, const BatchShape& b, const FastArgs& a);
 void launch_fast_params(hipStream_t st, const float* rho, const float* lam, const float* mu, float* bx, float* ldt,
                         float* mdt, long long n, float dt);
 void launch_fast_rmap(hipStream_t st, const BatchShape& b, int* rmap);
+
+// --- fused single-kernel elastic step (kernels_fused.hip) -------------------
+struct ChunkDesc {
+    int slot, x0, x1, pad;  // columns [x0, x1) of slot
+};
+struct FusedArgs {
+    // state read (a) and written (b); split memory used in the damping strips only
+    const float *vx_a, *vz_a, *txx_a, *tzz_a, *txz_a;
+    const float *vx_x_a, *vx_z_a, *vz_x_a, *vz_z_a, *txx_x_a, *txx_z_a, *tzz_x_a, *tzz_z_a, *txz_x_a, *txz_z_a;
+    float *vx_b, *vz_b, *txx_b, *tzz_b, *txz_b;
+    float *vx_x_b, *vx_z_b, *vz_x_b, *vz_z_b, *txx_x_b, *txx_z_b, *tzz_x_b, *tzz_z_b, *txz_x_b, *txz_z_b;
+    const float *bx, *lam, *mu;  // dt/rho, lambda*dt, mu*dt
+    const float *dampx, *dampz;
+    const float* wav;
+    float *rec_z, *rec_x;
+};
+// Largest padded depth one workgroup column can hold (one z row per lane).
+constexpr int FUSED_MAX_NZP = 1024;
+void launch_fused_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, const ChunkDesc* chunks,
+                       int nchunks, const int* rmap, int it);
 
 }  // namespace ds2d

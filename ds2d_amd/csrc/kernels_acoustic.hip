@@ -39,61 +39,90 @@ __device__ __forceinline__ float dmx(const float* f, size_t c, int s) {
 
 }  // namespace
 
-// v += sign * (dt/d) b D+ (p + w delta_src);  born: dv += (dt/d) b D+ dp.
+// Physical-window cell test and its index in the per-step history of the
+// background increment dp (hist[hist_off + (it*nx + ix-pml)*nz + iz-pml]).
+__device__ __forceinline__ bool hist_cell(const SlotDesc& S, int ix, int iz, int it, size_t& h) {
+    const int lx = ix - S.pml, lz = iz - S.pml;
+    if (lx < 0 || lx >= S.nx || lz < 0 || lz >= S.nz) return false;
+    h = (size_t)S.hist_off + ((size_t)it * S.nx + lx) * S.nz + lz;
+    return true;
+}
+
+// bg: v += sign * (dt/d) b D+ (p + w delta_src);  born: dv += (dt/d) b D+ dp.
 __global__ __launch_bounds__(256) void k_acou_v(AcouField bg, AcouField bn, AcouMed m, const float* __restrict__ wav,
                                                 const SlotDesc* __restrict__ slots, float ax, float az, int it,
-                                                float sign, int with_born) {
+                                                float sign, int do_bg, int do_born) {
     const SlotDesc S = slots[blockIdx.z];
     Tile t;
     if (!tile_cell(S, t)) return;
     const size_t c = t.c, s = S.nzp;
-    const size_t src = (size_t)S.off + (size_t)S.isx * S.nzp + S.isz;
-    const float w = wav[it];
-    auto P = [&](size_t j) { return j == src ? bg.p[j] + w : bg.p[j]; };
-    const float a = DS2D_C1 * (P(c + s) - P(c)) + DS2D_C2 * (P(c + 2 * s) - P(c - s)) +
-                    DS2D_C3 * (P(c + 3 * s) - P(c - 2 * s)) + DS2D_C4 * (P(c + 4 * s) - P(c - 3 * s));
-    const float b = DS2D_C1 * (P(c + 1) - P(c)) + DS2D_C2 * (P(c + 2) - P(c - 1)) +
-                    DS2D_C3 * (P(c + 3) - P(c - 2)) + DS2D_C4 * (P(c + 4) - P(c - 3));
     const float bu = m.b[c];
-    bg.vx[c] += sign * (ax * bu * a);
-    bg.vz[c] += sign * (az * bu * b);
-    if (with_born) {
+    if (do_bg) {
+        const size_t src = (size_t)S.off + (size_t)S.isx * S.nzp + S.isz;
+        const float w = wav[it];
+        auto P = [&](size_t j) { return j == src ? bg.p[j] + w : bg.p[j]; };
+        const float a = DS2D_C1 * (P(c + s) - P(c)) + DS2D_C2 * (P(c + 2 * s) - P(c - s)) +
+                        DS2D_C3 * (P(c + 3 * s) - P(c - 2 * s)) + DS2D_C4 * (P(c + 4 * s) - P(c - 3 * s));
+        const float b = DS2D_C1 * (P(c + 1) - P(c)) + DS2D_C2 * (P(c + 2) - P(c - 1)) +
+                        DS2D_C3 * (P(c + 3) - P(c - 2)) + DS2D_C4 * (P(c + 4) - P(c - 3));
+        bg.vx[c] += sign * (ax * bu * a);
+        bg.vz[c] += sign * (az * bu * b);
+    }
+    if (do_born) {
         bn.vx[c] += ax * bu * dpx(bn.p, c, (int)s);
         bn.vz[c] += az * bu * dpx(bn.p, c, 1);
     }
 }
 
-// dp = k (dt/dx D- vx + dt/dz D- vz); p += dp;  born: dpb += k D- dv + 2 m dp.
-__global__ __launch_bounds__(256) void k_acou_p(AcouField bg, AcouField bn, AcouMed m,
-                                                const SlotDesc* __restrict__ slots, float ax, float az, int with_born) {
+// bg: dp = k (dt/dx D- vx + dt/dz D- vz); p += dp (hist_mode 1 also stores dp
+// on the physical window); without bg, hist_mode 2 reads dp back.
+// born: dpb += k D- dv + 2 m dp.
+__global__ __launch_bounds__(256) void k_acou_p(AcouField bg, AcouField bn, AcouMed m, float* hist,
+                                                const SlotDesc* __restrict__ slots, float ax, float az, int it,
+                                                int do_bg, int do_born, int hist_mode) {
     const SlotDesc S = slots[blockIdx.z];
     Tile t;
     if (!tile_cell(S, t)) return;
     const size_t c = t.c;
     const int s = S.nzp;
     const float kk = m.k[c];
-    const float dp = kk * (ax * dmx(bg.vx, c, s) + az * dmx(bg.vz, c, 1));
-    bg.p[c] += dp;
-    if (with_born) {
+    size_t h = 0;
+    const bool inside = hist_cell(S, t.ix, t.iz, it, h);
+    float dp = 0.f;
+    if (do_bg) {
+        dp = kk * (ax * dmx(bg.vx, c, s) + az * dmx(bg.vz, c, 1));
+        bg.p[c] += dp;
+        if (hist_mode == 1 && inside) hist[h] = dp;
+    } else if (hist_mode == 2 && inside) {
+        dp = hist[h];
+    }
+    if (do_born) {
         const float bd = kk * (ax * dmx(bn.vx, c, s) + az * dmx(bn.vz, c, 1));
         bn.p[c] += bd + 2 * m.m[c] * dp;
     }
 }
 
 // Backward kernel 1 (lam holds mu_n after the receiver injection):
-//   d = k D- v^{n+1};  grad += 2 d lp;  p^n = p^{n+1} - d;
+//   d = k D- v^{n+1} (reconstruct) or the stored increment (hist != null);
+//   grad += 2 d lp;  reconstruct: p^n = p^{n+1} - d;
 //   lv += -(dt/d) D+ (k lp) with k lp masked to the update region.
 __global__ __launch_bounds__(256) void k_acou_adj_p(AcouField bg, AcouField lm, AcouMed m, float* grad,
-                                                    const SlotDesc* __restrict__ slots, float ax, float az) {
+                                                    const float* __restrict__ hist,
+                                                    const SlotDesc* __restrict__ slots, float ax, float az, int it) {
     const SlotDesc S = slots[blockIdx.z];
     Tile t;
     if (!tile_cell(S, t)) return;
     const size_t c = t.c;
     const int s = S.nzp;
-    const float kk = m.k[c];
-    const float d = kk * (ax * dmx(bg.vx, c, s) + az * dmx(bg.vz, c, 1));
-    grad[c] += 2 * d * lm.p[c];
-    bg.p[c] -= d;
+    if (hist) {
+        size_t h = 0;
+        if (hist_cell(S, t.ix, t.iz, it, h)) grad[c] += 2 * hist[h] * lm.p[c];
+    } else {
+        const float kk = m.k[c];
+        const float d = kk * (ax * dmx(bg.vx, c, s) + az * dmx(bg.vz, c, 1));
+        grad[c] += 2 * d * lm.p[c];
+        bg.p[c] -= d;
+    }
     auto W = [&](int jx, int jz) {
         return in_u(S, jx, jz) ? m.k[(size_t)S.off + (size_t)jx * s + jz] * lm.p[(size_t)S.off + (size_t)jx * s + jz]
                                : 0.f;
@@ -107,26 +136,28 @@ __global__ __launch_bounds__(256) void k_acou_adj_p(AcouField bg, AcouField lm, 
     lm.vz[c] += -(az * gz);
 }
 
-// Backward kernel 2: background velocity reversal with the transient source,
-//   v^n = v^{n+1} - (dt/d) b D+ (p^n + w_n delta_src);
+// Backward kernel 2: (reconstruct only) background velocity reversal with the
+// transient source, v^n = v^{n+1} - (dt/d) b D+ (p^n + w_n delta_src);
 //   lp += -(dt/d) D- (b lv) with b lv masked to the update region.
 __global__ __launch_bounds__(256) void k_acou_adj_v(AcouField bg, AcouField lm, AcouMed m,
                                                     const float* __restrict__ wav, const SlotDesc* __restrict__ slots,
-                                                    float ax, float az, int it) {
+                                                    float ax, float az, int it, int do_bg) {
     const SlotDesc S = slots[blockIdx.z];
     Tile t;
     if (!tile_cell(S, t)) return;
     const size_t c = t.c, s = S.nzp;
-    const size_t src = (size_t)S.off + (size_t)S.isx * S.nzp + S.isz;
-    const float w = wav[it];
-    auto P = [&](size_t j) { return j == src ? bg.p[j] + w : bg.p[j]; };
-    const float a = DS2D_C1 * (P(c + s) - P(c)) + DS2D_C2 * (P(c + 2 * s) - P(c - s)) +
-                    DS2D_C3 * (P(c + 3 * s) - P(c - 2 * s)) + DS2D_C4 * (P(c + 4 * s) - P(c - 3 * s));
-    const float b = DS2D_C1 * (P(c + 1) - P(c)) + DS2D_C2 * (P(c + 2) - P(c - 1)) +
-                    DS2D_C3 * (P(c + 3) - P(c - 2)) + DS2D_C4 * (P(c + 4) - P(c - 3));
-    const float bu = m.b[c];
-    bg.vx[c] += -1.f * (ax * bu * a);
-    bg.vz[c] += -1.f * (az * bu * b);
+    if (do_bg) {
+        const size_t src = (size_t)S.off + (size_t)S.isx * S.nzp + S.isz;
+        const float w = wav[it];
+        auto P = [&](size_t j) { return j == src ? bg.p[j] + w : bg.p[j]; };
+        const float a = DS2D_C1 * (P(c + s) - P(c)) + DS2D_C2 * (P(c + 2 * s) - P(c - s)) +
+                        DS2D_C3 * (P(c + 3 * s) - P(c - 2 * s)) + DS2D_C4 * (P(c + 4 * s) - P(c - 3 * s));
+        const float b = DS2D_C1 * (P(c + 1) - P(c)) + DS2D_C2 * (P(c + 2) - P(c - 1)) +
+                        DS2D_C3 * (P(c + 3) - P(c - 2)) + DS2D_C4 * (P(c + 4) - P(c - 3));
+        const float bu = m.b[c];
+        bg.vx[c] += -1.f * (ax * bu * a);
+        bg.vz[c] += -1.f * (az * bu * b);
+    }
     const int ix = t.ix, iz = t.iz;
     auto UX = [&](int jx, int jz) {
         const size_t j = (size_t)S.off + (size_t)jx * s + jz;
@@ -166,29 +197,29 @@ __global__ void k_acou_inj(float* lp, const float* __restrict__ res, const int* 
 static dim3 cgrid(const BatchShape& b) { return dim3(ceil_div(b.nzp, BZ), ceil_div(b.max_nxp, BX), b.nslots); }
 
 void launch_acou_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, const float* wav,
-                   int it, float sign, int with_born) {
+                   int it, float sign, int do_bg, int do_born) {
     hipLaunchKernelGGL(k_acou_v, cgrid(b), dim3(BZ, BX), 0, st, bg, born, m, wav, b.slots, b.dt / b.dx, b.dt / b.dz,
-                       it, sign, with_born);
+                       it, sign, do_bg, do_born);
 }
-void launch_acou_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, float* rec_bg,
-                   float* rec_born, int it, int with_born) {
-    hipLaunchKernelGGL(k_acou_p, cgrid(b), dim3(BZ, BX), 0, st, bg, born, m, b.slots, b.dt / b.dx, b.dt / b.dz,
-                       with_born);
+void launch_acou_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, float* hist,
+                   float* rec_bg, float* rec_born, int it, int do_bg, int do_born, int hist_mode) {
+    hipLaunchKernelGGL(k_acou_p, cgrid(b), dim3(BZ, BX), 0, st, bg, born, m, hist, b.slots, b.dt / b.dx, b.dt / b.dz,
+                       it, do_bg, do_born, hist_mode);
     if (rec_bg || rec_born)
         hipLaunchKernelGGL(k_acou_rec, dim3(ceil_div(b.max_ntr, 64), b.nslots), dim3(64), 0, st, bg.p, born.p, rec_bg,
                            rec_born, b.rix, b.slots, b.nt, it);
 }
 void launch_acou_adj_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m, float* grad,
-                       const float* res, int it) {
+                       const float* hist, const float* res, int it) {
     hipLaunchKernelGGL(k_acou_inj, dim3(ceil_div(b.max_ntr, 64), b.nslots), dim3(64), 0, st, lam.p, res, b.rix,
                        b.slots, b.nt, it);
-    hipLaunchKernelGGL(k_acou_adj_p, cgrid(b), dim3(BZ, BX), 0, st, bg, lam, m, grad, b.slots, b.dt / b.dx,
-                       b.dt / b.dz);
+    hipLaunchKernelGGL(k_acou_adj_p, cgrid(b), dim3(BZ, BX), 0, st, bg, lam, m, grad, hist, b.slots, b.dt / b.dx,
+                       b.dt / b.dz, it);
 }
 void launch_acou_adj_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m,
-                       const float* wav, int it) {
+                       const float* wav, int it, int do_bg) {
     hipLaunchKernelGGL(k_acou_adj_v, cgrid(b), dim3(BZ, BX), 0, st, bg, lam, m, wav, b.slots, b.dt / b.dx,
-                       b.dt / b.dz, it);
+                       b.dt / b.dz, it, do_bg);
 }
 
 // ---------------------------------------------------------------------------

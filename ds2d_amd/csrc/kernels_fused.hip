@@ -1,0 +1,202 @@
+// Fused elastic time step for gfx950: velocity and stress of one time step in
+// a single launch, batched over uneven shot windows.
+//
+// Scheme: born.cu:215-349 (update_vx/vz then update_txx_tzz/txz) with the
+// transient explosive source of FD2DGPU.cu:2039-2049 and the surface
+// recorder of FD2DGPU.cu:355-365; split PML memory only in the damping
+// strips (elsewhere the split update equals the unsplit one).
+//
+// Layout of the work: one workgroup owns columns [x0, x1) of one shot window
+// and the full padded depth (one z row per lane, rows contiguous in memory
+// so every column load is a coalesced 64-lane row).  It marches along x:
+// at column j it loads the old stresses of column j+4, computes the new
+// velocity of column j and the new stress of column j-4.  The eight x taps
+// of every derivative live in per-lane register queues; the eight z taps
+// are read from a double-buffered LDS copy of the current column (one
+// barrier per column).  The state is ping-ponged (read A, write B), so the
+// four halo columns of velocity on each side are recomputed without races.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace ds2d {
+
+namespace {
+__device__ __forceinline__ bool damp_cell_f(const SlotDesc& S, int ix, int iz) {
+    return S.boundary == 0 && (ix < S.pml || ix > S.nxp - S.pml || iz < S.pml || iz > S.nzp - S.pml);
+}
+}  // namespace
+
+__global__ __launch_bounds__(1024) void k_fused_step(FusedArgs a, const ChunkDesc* __restrict__ chunks,
+                                                     int nchunks, int per_xcd, const int* __restrict__ rmap,
+                                                     const SlotDesc* __restrict__ slots, float idx, float idz,
+                                                     float dt, int nt, int it) {
+    // XCD-aware chunk order: consecutive workgroups land on different XCDs,
+    // so give each XCD a contiguous run of chunks (neighbours share halos in L2).
+    const int b = blockIdx.x;
+    const int ch = (b % 8) * per_xcd + b / 8;
+    if (ch >= nchunks) return;
+    const ChunkDesc C = chunks[ch];
+    const SlotDesc S = slots[C.slot];
+    const int z = threadIdx.x;
+    const int nzp = S.nzp, nxp = S.nxp;
+    const bool zin = z < nzp;
+    const bool zupd = z >= NOP && z < nzp - NOP;
+    const size_t base = (size_t)S.off + z;
+
+    __shared__ float lds[2][4][1024];
+
+    auto ld = [&](const float* p, int col) -> float {
+        return (zin && col >= 0 && col < nxp) ? p[base + (size_t)col * nzp] : 0.f;
+    };
+
+    const float c1 = DS2D_C1, c2 = DS2D_C2, c3 = DS2D_C3, c4 = DS2D_C4;
+    float qxx[9], qxz[9], qzz[5], qvx[9], qvz[8];
+    const int x0 = C.x0, x1 = C.x1;
+    const int j0 = x0 - 4;
+    // prefill: qxx/qxz[1..8] = old stress of columns j0-4 .. j0+3; qzz[1..4] = j0-4 .. j0-1
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        qxx[k + 1] = ld(a.txx_a, j0 - 4 + k);
+        qxz[k + 1] = ld(a.txz_a, j0 - 4 + k);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) qzz[k + 1] = ld(a.tzz_a, j0 - 4 + k);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) qvx[k] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) qvz[k] = 0.f;
+
+    const float w = a.wav[it];
+    // coefficient of the tap at offset d = (source - cell) of a forward
+    // half-point difference: +c_d for d = 1..4, -c_(1-d) for d = 0..-3
+    auto tap = [&](int d) -> float {
+        return d == 1 ? c1 : d == 2 ? c2 : d == 3 ? c3 : d == 4 ? c4 :
+               d == 0 ? -c1 : d == -1 ? -c2 : d == -2 ? -c3 : d == -3 ? -c4 : 0.f;
+    };
+
+    for (int j = j0; j < x1 + 4; ++j) {
+        // ---- shift queues and load the new column(s)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            qxx[k] = qxx[k + 1];
+            qxz[k] = qxz[k + 1];
+            qvx[k] = qvx[k + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) qzz[k] = qzz[k + 1];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) qvz[k] = qvz[k + 1];
+        qxx[8] = ld(a.txx_a, j + 4);
+        qxz[8] = ld(a.txz_a, j + 4);
+        qzz[4] = ld(a.tzz_a, j);
+        const float vx_old = ld(a.vx_a, j), vz_old = ld(a.vz_a, j);
+        // ---- z taps through LDS: txz, tzz of column j; new vx, vz of column j-4
+        const int p = j & 1;
+        lds[p][0][z] = qxz[4];
+        lds[p][1][z] = qzz[4];
+        lds[p][2][z] = qvx[4];
+        lds[p][3][z] = qvz[3];
+        __syncthreads();
+        // ---- velocity of column j (halo columns included)
+        float nvx = vx_old, nvz = vz_old;
+        if (zupd && j >= NOP && j < nxp - NOP) {
+            const float* L0 = lds[p][0];
+            const float* L1 = lds[p][1];
+            float ax = c1 * (qxx[5] - qxx[4]) + c2 * (qxx[6] - qxx[3]) + c3 * (qxx[7] - qxx[2]) + c4 * (qxx[8] - qxx[1]);
+            float ex = c1 * (qxz[4] - qxz[3]) + c2 * (qxz[5] - qxz[2]) + c3 * (qxz[6] - qxz[1]) + c4 * (qxz[7] - qxz[0]);
+            float bz = c1 * (L0[z] - L0[z - 1]) + c2 * (L0[z + 1] - L0[z - 2]) + c3 * (L0[z + 2] - L0[z - 3]) +
+                       c4 * (L0[z + 3] - L0[z - 4]);
+            float cz = c1 * (L1[z + 1] - L1[z]) + c2 * (L1[z + 2] - L1[z - 1]) + c3 * (L1[z + 3] - L1[z - 2]) +
+                       c4 * (L1[z + 4] - L1[z - 3]);
+            const int dxs = S.isx - j, dzs = S.isz - z;
+            if (z == S.isz) ax += w * tap(dxs);
+            if (j == S.isx) cz += w * tap(dzs);
+            const size_t c = base + (size_t)j * nzp;
+            const float bx = a.bx[c];
+            if (damp_cell_f(S, j, z)) {
+                const float dpx_ = a.dampx[c], dpz_ = a.dampz[c];
+                const float d1 = 1.f - dt * dpx_ / 2, d2 = 1.f + dt * dpx_ / 2;
+                const float d3 = 1.f - dt * dpz_ / 2, d4 = 1.f + dt * dpz_ / 2;
+                const float vxx = (d1 * a.vx_x_a[c] + bx * idx * ax) / d2;
+                const float vxz = (d3 * a.vx_z_a[c] + bx * idz * bz) / d4;
+                const float vzx = (d1 * a.vz_x_a[c] + bx * idx * ex) / d2;
+                const float vzz = (d3 * a.vz_z_a[c] + bx * idz * cz) / d4;
+                if (j >= x0 && j < x1) {
+                    a.vx_x_b[c] = vxx; a.vx_z_b[c] = vxz; a.vz_x_b[c] = vzx; a.vz_z_b[c] = vzz;
+                }
+                nvx = vxx + vxz;
+                nvz = vzx + vzz;
+            } else {
+                nvx = vx_old + bx * (ax * idx + bz * idz);
+                nvz = vz_old + bx * (ex * idx + cz * idz);
+            }
+        }
+        qvx[8] = nvx;
+        qvz[7] = nvz;
+        if (zin && j >= x0 && j < x1) {
+            const size_t c = base + (size_t)j * nzp;
+            a.vx_b[c] = nvx;
+            a.vz_b[c] = nvz;
+            if (z == S.igz) {
+                const int r = rmap[S.rmap_off + j];
+                if (r >= 0) {
+                    const size_t o = (size_t)S.rec_off + (size_t)r * nt + it;
+                    a.rec_z[o] = nvz;
+                    a.rec_x[o] = nvx;
+                }
+            }
+        }
+        // ---- stress of column cs = j - 4 (owned columns only)
+        const int cs = j - 4;
+        if (zin && cs >= x0 && cs < x1) {
+            const size_t c = base + (size_t)cs * nzp;
+            float txx = qxx[0], tzz = qzz[0], txz = qxz[0];
+            if (zupd && cs >= NOP && cs < nxp - NOP) {
+                const float* L2 = lds[p][2];
+                const float* L3 = lds[p][3];
+                const float dvx = (c1 * (qvx[4] - qvx[3]) + c2 * (qvx[5] - qvx[2]) + c3 * (qvx[6] - qvx[1]) +
+                                   c4 * (qvx[7] - qvx[0])) * idx;
+                const float dvzx = (c1 * (qvz[4] - qvz[3]) + c2 * (qvz[5] - qvz[2]) + c3 * (qvz[6] - qvz[1]) +
+                                    c4 * (qvz[7] - qvz[0])) * idx;
+                const float dvz = (c1 * (L3[z] - L3[z - 1]) + c2 * (L3[z + 1] - L3[z - 2]) +
+                                   c3 * (L3[z + 2] - L3[z - 3]) + c4 * (L3[z + 3] - L3[z - 4])) * idz;
+                const float dvxz = (c1 * (L2[z + 1] - L2[z]) + c2 * (L2[z + 2] - L2[z - 1]) +
+                                    c3 * (L2[z + 3] - L2[z - 2]) + c4 * (L2[z + 4] - L2[z - 3])) * idz;
+                const float l = a.lam[c], mu = a.mu[c], l2m = l + 2 * mu;
+                if (damp_cell_f(S, cs, z)) {
+                    const float dpx_ = a.dampx[c], dpz_ = a.dampz[c];
+                    const float d1 = 1.f - dt * dpx_ / 2, d2 = 1.f + dt * dpx_ / 2;
+                    const float d3 = 1.f - dt * dpz_ / 2, d4 = 1.f + dt * dpz_ / 2;
+                    const float a1 = (d1 * a.txx_x_a[c] + l2m * dvx) / d2;
+                    const float a2 = (d3 * a.txx_z_a[c] + l * dvz) / d4;
+                    const float b1 = (d1 * a.tzz_x_a[c] + l * dvx) / d2;
+                    const float b2 = (d3 * a.tzz_z_a[c] + l2m * dvz) / d4;
+                    const float e1 = (d1 * a.txz_x_a[c] + mu * dvzx) / d2;
+                    const float e2 = (d3 * a.txz_z_a[c] + mu * dvxz) / d4;
+                    a.txx_x_b[c] = a1; a.txx_z_b[c] = a2; a.tzz_x_b[c] = b1; a.tzz_z_b[c] = b2;
+                    a.txz_x_b[c] = e1; a.txz_z_b[c] = e2;
+                    txx = a1 + a2;
+                    tzz = b1 + b2;
+                    txz = e1 + e2;
+                } else {
+                    txx = txx + (l2m * dvx + l * dvz);
+                    tzz = tzz + (l * dvx + l2m * dvz);
+                    txz = txz + mu * (dvzx + dvxz);
+                }
+            }
+            a.txx_b[c] = txx;
+            a.tzz_b[c] = tzz;
+            a.txz_b[c] = txz;
+        }
+    }
+}
+
+void launch_fused_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, const ChunkDesc* chunks,
+                       int nchunks, const int* rmap, int it) {
+    const int per_xcd = (nchunks + 7) / 8;
+    const int threads = ((b.nzp + 63) / 64) * 64;
+    hipLaunchKernelGGL(k_fused_step, dim3(per_xcd * 8), dim3(threads), 0, st, a, chunks, nchunks, per_xcd, rmap,
+                       b.slots, 1.f / b.dx, 1.f / b.dz, b.dt, b.nt, it);
+}
+
+}  // namespace ds2d

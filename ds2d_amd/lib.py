@@ -24,7 +24,7 @@ class Geom(C.Structure):
                 ("nxpml", C.c_int), ("nzpml", C.c_int), ("allnx", C.c_int), ("allnz", C.c_int),
                 ("scale", C.c_int), ("pml", C.c_int), ("nt", C.c_int), ("nop", C.c_int), ("ntr", C.c_int),
                 ("physics", C.c_int), ("pml_rr", C.c_float), ("rho_const", C.c_float), ("rb_amp", C.c_float),
-                ("rb_seed", C.c_ulonglong)]
+                ("rb_seed", C.c_ulonglong), ("bg_mode", C.c_int), ("hbm_budget_gb", C.c_float)]
 
 
 class Shot(C.Structure):
@@ -34,7 +34,8 @@ class Shot(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("elapsed_ms", C.c_double), ("stencil_ms", C.c_double), ("cell_updates", C.c_longlong),
-                ("stencil_launches", C.c_longlong), ("alg_bytes", C.c_double)]
+                ("stencil_launches", C.c_longlong), ("alg_bytes", C.c_double), ("kernel_avg_ms", C.c_double),
+                ("kernel_samples", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -57,6 +58,8 @@ SIGNATURES = {
     "ds2d_take_gradients": (C.c_int, [_H, _F, _F, _F, _F, _F, _F, _F]),
     "ds2d_forward": (C.c_int, [_H, _F, _F]),
     "ds2d_batch_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.c_int, C.POINTER(Stats)]),
+    "ds2d_set_chunk": (C.c_int, [_H, C.c_int]),
+    "ds2d_autotune_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), C.c_int, C.POINTER(C.c_int), _D]),
     "ds2d_acoustic_ref_forward": (C.c_int, [_H, C.c_int, C.c_int, _F]),
     "ds2d_born_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, _F, C.POINTER(Stats)]),
     "ds2d_born_adjoint": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.POINTER(Stats)]),
@@ -153,10 +156,10 @@ class Solver:
     """Owning wrapper of a ds2d handle (FD2DGPU_ELASTIC counterpart)."""
 
     def __init__(self, *, dx, dz, dt, nt, allnx, allnz, pml, nx_max, ntr_max=0, wavelet, physics=0,
-                 pml_rr=1e-3, rho_const=0.0, rb_amp=0.3, rb_seed=1234):
+                 pml_rr=1e-3, rho_const=0.0, rb_amp=0.3, rb_seed=1234, bg_mode=0, hbm_budget_gb=64.0):
         self.lib = load()
         g = Geom(dx, dz, dt, nx_max + 2 * pml, allnz + 2 * pml, allnx, allnz, 1, pml, nt, 4, ntr_max, physics,
-                 pml_rr, rho_const, rb_amp, rb_seed)
+                 pml_rr, rho_const, rb_amp, rb_seed, bg_mode, hbm_budget_gb)
         self.geom = g
         self.nt, self.allnx, self.allnz, self.pml = nt, allnx, allnz, pml
         self.wavelet = np.ascontiguousarray(wavelet, np.float32)
@@ -189,8 +192,20 @@ class Solver:
     def _rec(self, shots):
         return np.zeros(sum(s.ntr for s in shots) * self.nt, np.float32)
 
-    def batch_forward(self, shots, mode=1):
-        rz, rx = self._rec(shots), self._rec(shots)
+    def set_chunk(self, width):
+        _check(self.lib.ds2d_set_chunk(self.h, int(width)))
+
+    def autotune(self, shots, nsteps=100):
+        """Pick the fused kernel's x-chunk width for this batch; returns (width, ms)."""
+        w = C.c_int(0)
+        ms = C.c_double(0)
+        _check(self.lib.ds2d_autotune_forward(self.h, len(shots), _shots_array(shots), nsteps, C.byref(w),
+                                              C.byref(ms)))
+        return w.value, ms.value
+
+    def batch_forward(self, shots, mode=2, fetch=True):
+        """Batched elastic forward; fetch=False leaves the records in HBM."""
+        rz, rx = (self._rec(shots), self._rec(shots)) if fetch else (None, None)
         st = Stats()
         arr = _shots_array(shots)
         _check(self.lib.ds2d_batch_forward(self.h, len(shots), arr, fptr(rz), fptr(rx), mode, C.byref(st)))

@@ -68,6 +68,10 @@ typedef struct {
     float rho_const;        /* > 0: constant density (reference: 2000)         */
     float rb_amp;           /* random boundary relative amplitude (0.3)        */
     unsigned long long rb_seed; /* random boundary seed                        */
+    int bg_mode;            /* plain Born/adjoint background: 0 auto, 1 keep the
+                               per-step increment resident in HBM, 2 rebuild it
+                               by reversal through the random boundary          */
+    float hbm_budget_gb;    /* auto mode: largest history kept resident (64)   */
 } ds2d_geom;
 
 /* One shot / aperture window (main.cpp:1011-1025 public fields). */
@@ -88,7 +92,9 @@ typedef struct {
     double stencil_ms;         /* time-stepping loop only                         */
     long long cell_updates;    /* sum over wavefields of nxpml*nzpml*nt            */
     long long stencil_launches;/* launches of the dominant stencil kernel          */
-    double alg_bytes;          /* algorithmic HBM bytes of those launches          */
+    double alg_bytes;          /* algorithmic HBM bytes of one such launch         */
+    double kernel_avg_ms;      /* its average launch duration (sampled events)     */
+    long long kernel_samples;  /* launches that were bracketed by events           */
 } ds2d_stats;
 
 const char* ds2d_last_error(void);
@@ -117,6 +123,12 @@ int ds2d_forward(ds2d_t* h, float* rec_z, float* rec_x);
  * 1 = fused fast kernel (default). */
 int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec_z, float* rec_x,
                        int mode, ds2d_stats* stats);
+/* x-chunk width of the fused kernel (0 = default 64), and an autotuner that
+ * times nsteps of the fused kernel per candidate width on this batch and
+ * keeps the fastest.  No reference counterpart (new capability). */
+int ds2d_set_chunk(ds2d_t* h, int width);
+int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nsteps, int* best_width,
+                          double* best_ms);
 
 /* Reference acoustic step (split p, mirrored stagger): pressure at depth pml. */
 int ds2d_acoustic_ref_forward(ds2d_t* h, int sxnum, int ngx_min, float* rec_p);

