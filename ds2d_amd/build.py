@@ -46,6 +46,17 @@ def build_lib(jobs=8):
     return LIB
 
 
+def build_driver():
+    """ds2d_amd/bin/ds2d_lsrtm: the parameter-file driver (plain C++ over the C-ABI)."""
+    out = os.path.join(HERE, "bin")
+    os.makedirs(out, exist_ok=True)
+    exe = os.path.join(out, "ds2d_lsrtm")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+          os.path.join(HERE, "driver", "ds2d_lsrtm.cpp"), "-o", exe, "-L", HERE, "-l:libds2d.so",
+          "-Wl,-rpath,$ORIGIN/.."])
+    return exe
+
+
 def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
@@ -56,6 +67,7 @@ def main(argv=None):
     ap.add_argument("--no-oracle", action="store_true")
     a = ap.parse_args(argv)
     print("built", build_lib(a.jobs))
+    print("built", build_driver())
     if not a.no_oracle:
         build_oracle()
         print("built oracle")

@@ -117,8 +117,9 @@ struct FusedArgs {
     const float* wav;
     float *rec_z, *rec_x;
 };
-// Largest padded depth one workgroup column can hold (one z row per lane).
-constexpr int FUSED_MAX_NZP = 1024;
+// Largest padded depth one workgroup column can hold (one z row per lane);
+// 768 lanes leave 170 VGPRs per lane for the tap queues and prefetch ring.
+constexpr int FUSED_MAX_NZP = 768;
 void launch_fused_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, const ChunkDesc* chunks,
                        int nchunks, const int* rmap, int it);
 

@@ -9,11 +9,12 @@
 // Layout of the work: one workgroup owns columns [x0, x1) of one shot window
 // and the full padded depth (one z row per lane, rows contiguous in memory
 // so every column load is a coalesced 64-lane row).  It marches along x:
-// at column j it loads the old stresses of column j+4, computes the new
-// velocity of column j and the new stress of column j-4.  The eight x taps
-// of every derivative live in per-lane register queues; the eight z taps
-// are read from a double-buffered LDS copy of the current column (one
-// barrier per column).  The state is ping-ponged (read A, write B), so the
+// at column j it computes the new velocity of column j and the new stress of
+// column j-4.  The eight x taps of every derivative live in per-lane
+// register queues; the eight z taps are read from a double-buffered LDS copy
+// of the current column (one barrier per column).  Every operand is loaded
+// PF columns ahead into a per-lane prefetch ring, so HBM latency overlaps
+// PF column iterations.  The state is ping-ponged (read A, write B), so the
 // four halo columns of velocity on each side are recomputed without races.
 #include "common.hpp"
 #include "kernels.hpp"
@@ -21,12 +22,25 @@
 namespace ds2d {
 
 namespace {
+constexpr int PF = 2;  // prefetch distance in columns
+
 __device__ __forceinline__ bool damp_cell_f(const SlotDesc& S, int ix, int iz) {
     return S.boundary == 0 && (ix < S.pml || ix > S.nxp - S.pml || iz < S.pml || iz > S.nzp - S.pml);
 }
+
+// Operands of one column for one lane.
+struct Col {
+    float txx4, txz4;            // stress of column j+4 (enters the x-tap queues)
+    float tzz, vx, vz, bx;       // stress tzz, old velocity, dt/rho of column j
+    float lam, mu;               // moduli of column j-4
+    float dxv, dzv;              // damping of column j (velocity, PML lanes only)
+    float vxx, vxz, vzx, vzz;    // old velocity splits of column j (PML lanes only)
+    float dxs, dzs;              // damping of column j-4 (stress, PML lanes only)
+    float sxx, sxz, szx, szz, sex, sez;  // old stress splits of column j-4 (PML lanes only)
+};
 }  // namespace
 
-__global__ __launch_bounds__(1024) void k_fused_step(FusedArgs a, const ChunkDesc* __restrict__ chunks,
+__global__ __launch_bounds__(FUSED_MAX_NZP) void k_fused_step(FusedArgs a, const ChunkDesc* __restrict__ chunks,
                                                      int nchunks, int per_xcd, const int* __restrict__ rmap,
                                                      const SlotDesc* __restrict__ slots, float idx, float idz,
                                                      float dt, int nt, int it) {
@@ -43,10 +57,42 @@ __global__ __launch_bounds__(1024) void k_fused_step(FusedArgs a, const ChunkDes
     const bool zupd = z >= NOP && z < nzp - NOP;
     const size_t base = (size_t)S.off + z;
 
-    __shared__ float lds[2][4][1024];
+    __shared__ float lds[2][4][FUSED_MAX_NZP];
 
     auto ld = [&](const float* p, int col) -> float {
         return (zin && col >= 0 && col < nxp) ? p[base + (size_t)col * nzp] : 0.f;
+    };
+    // operands of column j (velocity side) and j-4 (stress side)
+    auto fetch = [&](int j) {
+        Col o;
+        o.txx4 = ld(a.txx_a, j + 4);
+        o.txz4 = ld(a.txz_a, j + 4);
+        o.tzz = ld(a.tzz_a, j);
+        o.vx = ld(a.vx_a, j);
+        o.vz = ld(a.vz_a, j);
+        const bool vupd = zupd && j >= NOP && j < nxp - NOP;
+        o.bx = vupd ? ld(a.bx, j) : 0.f;
+        const int cs = j - 4;
+        const bool supd = zupd && cs >= NOP && cs < nxp - NOP;
+        o.lam = supd ? ld(a.lam, cs) : 0.f;
+        o.mu = supd ? ld(a.mu, cs) : 0.f;
+        const bool vpml = vupd && damp_cell_f(S, j, z);
+        o.dxv = vpml ? ld(a.dampx, j) : 0.f;
+        o.dzv = vpml ? ld(a.dampz, j) : 0.f;
+        o.vxx = vpml ? ld(a.vx_x_a, j) : 0.f;
+        o.vxz = vpml ? ld(a.vx_z_a, j) : 0.f;
+        o.vzx = vpml ? ld(a.vz_x_a, j) : 0.f;
+        o.vzz = vpml ? ld(a.vz_z_a, j) : 0.f;
+        const bool spml = supd && damp_cell_f(S, cs, z);
+        o.dxs = spml ? ld(a.dampx, cs) : 0.f;
+        o.dzs = spml ? ld(a.dampz, cs) : 0.f;
+        o.sxx = spml ? ld(a.txx_x_a, cs) : 0.f;
+        o.sxz = spml ? ld(a.txx_z_a, cs) : 0.f;
+        o.szx = spml ? ld(a.tzz_x_a, cs) : 0.f;
+        o.szz = spml ? ld(a.tzz_z_a, cs) : 0.f;
+        o.sex = spml ? ld(a.txz_x_a, cs) : 0.f;
+        o.sez = spml ? ld(a.txz_z_a, cs) : 0.f;
+        return o;
     };
 
     const float c1 = DS2D_C1, c2 = DS2D_C2, c3 = DS2D_C3, c4 = DS2D_C4;
@@ -65,6 +111,9 @@ __global__ __launch_bounds__(1024) void k_fused_step(FusedArgs a, const ChunkDes
     for (int k = 0; k < 9; ++k) qvx[k] = 0.f;
 #pragma unroll
     for (int k = 0; k < 8; ++k) qvz[k] = 0.f;
+    Col ring[PF];
+#pragma unroll
+    for (int k = 0; k < PF; ++k) ring[k] = fetch(j0 + k);
 
     const float w = a.wav[it];
     // coefficient of the tap at offset d = (source - cell) of a forward
@@ -75,7 +124,11 @@ __global__ __launch_bounds__(1024) void k_fused_step(FusedArgs a, const ChunkDes
     };
 
     for (int j = j0; j < x1 + 4; ++j) {
-        // ---- shift queues and load the new column(s)
+        // ---- consume the operands of column j, prefetch column j + PF
+        const Col o = ring[0];
+#pragma unroll
+        for (int k = 0; k < PF - 1; ++k) ring[k] = ring[k + 1];
+        ring[PF - 1] = fetch(j + PF);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             qxx[k] = qxx[k + 1];
@@ -86,10 +139,9 @@ __global__ __launch_bounds__(1024) void k_fused_step(FusedArgs a, const ChunkDes
         for (int k = 0; k < 4; ++k) qzz[k] = qzz[k + 1];
 #pragma unroll
         for (int k = 0; k < 7; ++k) qvz[k] = qvz[k + 1];
-        qxx[8] = ld(a.txx_a, j + 4);
-        qxz[8] = ld(a.txz_a, j + 4);
-        qzz[4] = ld(a.tzz_a, j);
-        const float vx_old = ld(a.vx_a, j), vz_old = ld(a.vz_a, j);
+        qxx[8] = o.txx4;
+        qxz[8] = o.txz4;
+        qzz[4] = o.tzz;
         // ---- z taps through LDS: txz, tzz of column j; new vx, vz of column j-4
         const int p = j & 1;
         lds[p][0][z] = qxz[4];
@@ -98,7 +150,7 @@ __global__ __launch_bounds__(1024) void k_fused_step(FusedArgs a, const ChunkDes
         lds[p][3][z] = qvz[3];
         __syncthreads();
         // ---- velocity of column j (halo columns included)
-        float nvx = vx_old, nvz = vz_old;
+        float nvx = o.vx, nvz = o.vz;
         if (zupd && j >= NOP && j < nxp - NOP) {
             const float* L0 = lds[p][0];
             const float* L1 = lds[p][1];
@@ -108,27 +160,25 @@ __global__ __launch_bounds__(1024) void k_fused_step(FusedArgs a, const ChunkDes
                        c4 * (L0[z + 3] - L0[z - 4]);
             float cz = c1 * (L1[z + 1] - L1[z]) + c2 * (L1[z + 2] - L1[z - 1]) + c3 * (L1[z + 3] - L1[z - 2]) +
                        c4 * (L1[z + 4] - L1[z - 3]);
-            const int dxs = S.isx - j, dzs = S.isz - z;
-            if (z == S.isz) ax += w * tap(dxs);
-            if (j == S.isx) cz += w * tap(dzs);
-            const size_t c = base + (size_t)j * nzp;
-            const float bx = a.bx[c];
+            if (z == S.isz) ax += w * tap(S.isx - j);
+            if (j == S.isx) cz += w * tap(S.isz - z);
+            const float bx = o.bx;
             if (damp_cell_f(S, j, z)) {
-                const float dpx_ = a.dampx[c], dpz_ = a.dampz[c];
-                const float d1 = 1.f - dt * dpx_ / 2, d2 = 1.f + dt * dpx_ / 2;
-                const float d3 = 1.f - dt * dpz_ / 2, d4 = 1.f + dt * dpz_ / 2;
-                const float vxx = (d1 * a.vx_x_a[c] + bx * idx * ax) / d2;
-                const float vxz = (d3 * a.vx_z_a[c] + bx * idz * bz) / d4;
-                const float vzx = (d1 * a.vz_x_a[c] + bx * idx * ex) / d2;
-                const float vzz = (d3 * a.vz_z_a[c] + bx * idz * cz) / d4;
+                const float d1 = 1.f - dt * o.dxv / 2, d2 = 1.f + dt * o.dxv / 2;
+                const float d3 = 1.f - dt * o.dzv / 2, d4 = 1.f + dt * o.dzv / 2;
+                const float vxx = (d1 * o.vxx + bx * idx * ax) / d2;
+                const float vxz = (d3 * o.vxz + bx * idz * bz) / d4;
+                const float vzx = (d1 * o.vzx + bx * idx * ex) / d2;
+                const float vzz = (d3 * o.vzz + bx * idz * cz) / d4;
                 if (j >= x0 && j < x1) {
+                    const size_t c = base + (size_t)j * nzp;
                     a.vx_x_b[c] = vxx; a.vx_z_b[c] = vxz; a.vz_x_b[c] = vzx; a.vz_z_b[c] = vzz;
                 }
                 nvx = vxx + vxz;
                 nvz = vzx + vzz;
             } else {
-                nvx = vx_old + bx * (ax * idx + bz * idz);
-                nvz = vz_old + bx * (ex * idx + cz * idz);
+                nvx = o.vx + bx * (ax * idx + bz * idz);
+                nvz = o.vz + bx * (ex * idx + cz * idz);
             }
         }
         qvx[8] = nvx;
@@ -140,9 +190,9 @@ __global__ __launch_bounds__(1024) void k_fused_step(FusedArgs a, const ChunkDes
             if (z == S.igz) {
                 const int r = rmap[S.rmap_off + j];
                 if (r >= 0) {
-                    const size_t o = (size_t)S.rec_off + (size_t)r * nt + it;
-                    a.rec_z[o] = nvz;
-                    a.rec_x[o] = nvx;
+                    const size_t q = (size_t)S.rec_off + (size_t)r * nt + it;
+                    a.rec_z[q] = nvz;
+                    a.rec_x[q] = nvx;
                 }
             }
         }
@@ -162,17 +212,16 @@ __global__ __launch_bounds__(1024) void k_fused_step(FusedArgs a, const ChunkDes
                                    c3 * (L3[z + 2] - L3[z - 3]) + c4 * (L3[z + 3] - L3[z - 4])) * idz;
                 const float dvxz = (c1 * (L2[z + 1] - L2[z]) + c2 * (L2[z + 2] - L2[z - 1]) +
                                     c3 * (L2[z + 3] - L2[z - 2]) + c4 * (L2[z + 4] - L2[z - 3])) * idz;
-                const float l = a.lam[c], mu = a.mu[c], l2m = l + 2 * mu;
+                const float l = o.lam, mu = o.mu, l2m = l + 2 * mu;
                 if (damp_cell_f(S, cs, z)) {
-                    const float dpx_ = a.dampx[c], dpz_ = a.dampz[c];
-                    const float d1 = 1.f - dt * dpx_ / 2, d2 = 1.f + dt * dpx_ / 2;
-                    const float d3 = 1.f - dt * dpz_ / 2, d4 = 1.f + dt * dpz_ / 2;
-                    const float a1 = (d1 * a.txx_x_a[c] + l2m * dvx) / d2;
-                    const float a2 = (d3 * a.txx_z_a[c] + l * dvz) / d4;
-                    const float b1 = (d1 * a.tzz_x_a[c] + l * dvx) / d2;
-                    const float b2 = (d3 * a.tzz_z_a[c] + l2m * dvz) / d4;
-                    const float e1 = (d1 * a.txz_x_a[c] + mu * dvzx) / d2;
-                    const float e2 = (d3 * a.txz_z_a[c] + mu * dvxz) / d4;
+                    const float d1 = 1.f - dt * o.dxs / 2, d2 = 1.f + dt * o.dxs / 2;
+                    const float d3 = 1.f - dt * o.dzs / 2, d4 = 1.f + dt * o.dzs / 2;
+                    const float a1 = (d1 * o.sxx + l2m * dvx) / d2;
+                    const float a2 = (d3 * o.sxz + l * dvz) / d4;
+                    const float b1 = (d1 * o.szx + l * dvx) / d2;
+                    const float b2 = (d3 * o.szz + l2m * dvz) / d4;
+                    const float e1 = (d1 * o.sex + mu * dvzx) / d2;
+                    const float e2 = (d3 * o.sez + mu * dvxz) / d4;
                     a.txx_x_b[c] = a1; a.txx_z_b[c] = a2; a.tzz_x_b[c] = b1; a.tzz_z_b[c] = b2;
                     a.txz_x_b[c] = e1; a.txz_z_b[c] = e2;
                     txx = a1 + a2;

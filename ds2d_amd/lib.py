@@ -59,6 +59,7 @@ SIGNATURES = {
     "ds2d_forward": (C.c_int, [_H, _F, _F]),
     "ds2d_batch_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.c_int, C.POINTER(Stats)]),
     "ds2d_set_chunk": (C.c_int, [_H, C.c_int]),
+    "ds2d_set_graph": (C.c_int, [_H, C.c_int]),
     "ds2d_autotune_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), C.c_int, C.POINTER(C.c_int), _D]),
     "ds2d_acoustic_ref_forward": (C.c_int, [_H, C.c_int, C.c_int, _F]),
     "ds2d_born_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, _F, C.POINTER(Stats)]),
@@ -194,6 +195,9 @@ class Solver:
 
     def set_chunk(self, width):
         _check(self.lib.ds2d_set_chunk(self.h, int(width)))
+
+    def set_graph(self, on):
+        _check(self.lib.ds2d_set_graph(self.h, 1 if on else 0))
 
     def autotune(self, shots, nsteps=100):
         """Pick the fused kernel's x-chunk width for this batch; returns (width, ms)."""

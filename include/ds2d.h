@@ -127,6 +127,8 @@ int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec
  * times nsteps of the fused kernel per candidate width on this batch and
  * keeps the fastest.  No reference counterpart (new capability). */
 int ds2d_set_chunk(ds2d_t* h, int width);
+/* Capture each time loop into one hipGraph (1) or launch step by step (0). */
+int ds2d_set_graph(ds2d_t* h, int on);
 int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nsteps, int* best_width,
                           double* best_ms);
 
