@@ -2,6 +2,8 @@
 
 The compute path is libds2d.so (hand-written HIP kernels for gfx950 behind the
 C-ABI in include/ds2d.h); this package only binds it (lib), generates the
-seeded synthetic workloads (models) and builds the library (build).
+seeded synthetic workloads (models), drives the learned-preconditioner
+workflow (learned) and builds the library (build).
 """
-from .lib import DS2DError, Solver, ShotSpec, load, ricker, laplace_filter, laplace_band, device_info  # noqa: F401
+from .lib import (DS2DError, Solver, ShotSpec, Unet, conv3x3_bf16, load, ricker, laplace_filter,  # noqa: F401
+                  laplace_band, device_info)

@@ -19,6 +19,7 @@
 #include "../../include/ds2d.h"
 #include "common.hpp"
 #include "kernels.hpp"
+#include "unet.hpp"
 
 using namespace ds2d;
 
@@ -51,6 +52,12 @@ struct DevBuf {
 
 }  // namespace
 
+// Error state shared with the other host files of the library.
+int ds2d_internal_fail(int code, const char* msg) {
+    g_err = msg;
+    return code;
+}
+
 struct ds2d_ctx {
     ds2d_geom g{};
     hipStream_t st = nullptr;
@@ -69,6 +76,7 @@ struct ds2d_ctx {
     ds2d_stats last{};
     int chunk_w = 0;    // fused-kernel x-chunk width (0: default)
     int use_graph = 0;  // capture time loops into one hipGraph per loop
+    int per_launch = 0; // plain Born/adjoint: shots per time loop (0: all)
 
     ~ds2d_ctx() {
         for (auto& kv : ws)
@@ -899,6 +907,29 @@ int acou_set_refl(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouW
     return 0;
 }
 
+// Shot groups launched as one time loop each: the whole batch, or runs of
+// per_launch consecutive slots (per_launch = 1 is the reference's one shot
+// after another).  Slots keep their arena offsets, so a group is a view.
+std::vector<BatchShape> launch_groups(const ds2d_ctx* h, const Batch& B) {
+    const int ns = (int)B.slots.size();
+    const int gs = h->per_launch > 0 ? std::min(h->per_launch, ns) : ns;
+    std::vector<BatchShape> out;
+    for (int s0 = 0; s0 < ns; s0 += gs) {
+        BatchShape sh = B.shape;
+        const int s1 = std::min(ns, s0 + gs);
+        sh.slots = B.shape.slots + s0;
+        sh.nslots = s1 - s0;
+        sh.max_nxp = 0;
+        sh.max_ntr = 1;
+        for (int k = s0; k < s1; ++k) {
+            sh.max_nxp = std::max(sh.max_nxp, B.slots[k].nxp);
+            sh.max_ntr = std::max(sh.max_ntr, B.slots[k].ntr);
+        }
+        out.push_back(sh);
+    }
+    return out;
+}
+
 // Forward pass from rest.  do_bg: propagate the background (storing its
 // increment when w.store); otherwise the Born field reads the stored
 // increment.  Records: w.rec (bg), w.rec2 (born).
@@ -906,11 +937,12 @@ void acou_fwd(ds2d_ctx* h, const Batch& B, AcouWS& w, bool do_bg, bool do_born, 
     hipStream_t st = h->st;
     (void)hipMemsetAsync(w.bg.vx, 0, (size_t)w.n * 6 * 4, st);  // bg + bn
     const int hmode = do_bg ? (w.store ? 1 : 0) : 2;
-    for (int it = 0; it < h->g.nt; ++it) {
-        launch_acou_v(st, B.shape, w.bg, w.bn, w.m, h->d_wav, it, 1.f, do_bg, do_born);
-        launch_acou_p(st, B.shape, w.bg, w.bn, w.m, w.hist, rec_bg ? w.rec : nullptr, do_born ? w.rec2 : nullptr, it,
-                      do_bg, do_born, hmode);
-    }
+    for (const BatchShape& sh : launch_groups(h, B))
+        for (int it = 0; it < h->g.nt; ++it) {
+            launch_acou_v(st, sh, w.bg, w.bn, w.m, h->d_wav, it, 1.f, do_bg, do_born);
+            launch_acou_p(st, sh, w.bg, w.bn, w.m, w.hist, rec_bg ? w.rec : nullptr, do_born ? w.rec2 : nullptr, it,
+                          do_bg, do_born, hmode);
+        }
 }
 
 // Backward pass (residual in w.rec2): the gradient is accumulated in
@@ -920,10 +952,11 @@ void acou_bwd(ds2d_ctx* h, const Batch& B, AcouWS& w) {
     hipStream_t st = h->st;
     (void)hipMemsetAsync(w.lm.vx, 0, (size_t)w.n * 3 * 4, st);
     (void)hipMemsetAsync(w.grad_pad, 0, (size_t)w.n * 4, st);
-    for (int it = h->g.nt - 1; it >= 0; --it) {
-        launch_acou_adj_p(st, B.shape, w.bg, w.lm, w.m, w.grad_pad, w.store ? w.hist : nullptr, w.rec2, it);
-        launch_acou_adj_v(st, B.shape, w.bg, w.lm, w.m, h->d_wav, it, w.store ? 0 : 1);
-    }
+    for (const BatchShape& sh : launch_groups(h, B))
+        for (int it = h->g.nt - 1; it >= 0; --it) {
+            launch_acou_adj_p(st, sh, w.bg, w.lm, w.m, w.grad_pad, w.store ? w.hist : nullptr, w.rec2, it);
+            launch_acou_adj_v(st, sh, w.bg, w.lm, w.m, h->d_wav, it, w.store ? 0 : 1);
+        }
 }
 
 // Padded per-slot gradient -> global (deterministic slot order), overwrite.
@@ -1087,6 +1120,179 @@ int ds2d_lsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_
     }
     stats_loop_end(h);
     if ((rc = stats_end(h, stats, cells * h->g.nt * props, 0, 0))) return rc;
+    HIP_TRY(hipMemcpyAsync(image, x, A * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int ds2d_set_batch(ds2d_t* h, int shots_per_launch) {
+    if (!h) return fail(DS2D_EINVAL, "null handle");
+    if (shots_per_launch < 0) return fail(DS2D_EINVAL, "shots per launch must be >= 0");
+    h->per_launch = shots_per_launch;
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Learned preconditioner z = rms(g) * U(g / rms(g)): positively homogeneous,
+// so the conjugacy recurrences see consistent scales across iterations.  The
+// image (allnx x allnz) is zero-padded to the U-Net's multiple-of-2^(L-1) grid.
+struct Precond {
+    ds2d_unet* u;
+    int H, W;
+    float *in, *out;
+    int apply(ds2d_ctx* h, const float* g, float* z) {
+        hipStream_t st = h->st;
+        const int nx = h->g.allnx, nz = h->g.allnz;
+        const long long A = (long long)nx * nz;
+        int rc;
+        const double gg = host_dot(h, g, g, A, &rc);
+        if (rc) return rc;
+        if (!(gg > 0)) {
+            HIP_TRY(hipMemsetAsync(z, 0, A * 4, st));
+            return 0;
+        }
+        const double rms = std::sqrt(gg / A);
+        HIP_TRY(hipMemsetAsync(in, 0, (size_t)H * W * 4, st));
+        HIP_TRY(hipMemcpy2DAsync(in, (size_t)W * 4, g, (size_t)nz * 4, (size_t)nz * 4, nx, hipMemcpyDeviceToDevice, st));
+        launch_axpby(st, in, in, (long long)H * W, 0.f, (float)(1.0 / rms));
+        if ((rc = unet_infer_dev(u, 1, H, W, in, out, st))) return rc;
+        HIP_TRY(hipMemcpy2DAsync(z, (size_t)nz * 4, out, (size_t)W * 4, (size_t)nz * 4, nx, hipMemcpyDeviceToDevice, st));
+        launch_axpby(st, z, z, A, 0.f, (float)rms);
+        return 0;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+// Flexible preconditioned CG (Polak-Ribiere+, exact line search on the
+// quadratic) with the learned direction, falling back to plain CGLS from the
+// current image when a learned step would lower |r| by less than 1%.
+int ds2d_lsrtm_learned(ds2d_t* h, ds2d_unet_t* u, int nshots, const ds2d_shot* shots, const float* d_obs,
+                       int niter, float* image, double* resid, int* learned_steps, ds2d_stats* stats) {
+    int rc;
+    Batch B;
+    if ((rc = acou_begin(h, nshots, shots, B))) return rc;
+    if (!u) return fail(DS2D_EINVAL, "null U-Net handle");
+    if (niter < 0 || !d_obs || !image) return fail(DS2D_EINVAL, "bad LSRTM arguments");
+    int L, mn, mh, mw;
+    unet_limits(u, &L, &mn, &mh, &mw);
+    const int qd = 1 << (L - 1);
+    Precond P{u, (h->g.allnx + qd - 1) / qd * qd, (h->g.allnz + qd - 1) / qd * qd, nullptr, nullptr};
+    if (P.H > mh || P.W > mw)
+        return fail(DS2D_EINVAL, "model %dx%d (padded %dx%d) exceeds the U-Net's %dx%d", h->g.allnx, h->g.allnz, P.H,
+                    P.W, mh, mw);
+    const long long A = (long long)h->g.allnx * h->g.allnz;
+    hipStream_t st = h->st;
+    float *x, *p, *g, *z, *gn, *zn;
+    if ((rc = h->fbuf("cg_x", A, &x)) || (rc = h->fbuf("cg_p", A, &p)) || (rc = h->fbuf("cg_s", A, &g)) ||
+        (rc = h->fbuf("lr_z", A, &z)) || (rc = h->fbuf("lr_gn", A, &gn)) || (rc = h->fbuf("lr_zn", A, &zn)) ||
+        (rc = h->fbuf("lr_in", (size_t)P.H * P.W, &P.in)) || (rc = h->fbuf("lr_out", (size_t)P.H * P.W, &P.out)))
+        return rc;
+    float *d, *r, *q;
+    const long long R = std::max<long long>(B.total_rec, 1);
+    if ((rc = h->fbuf("cg_d", R, &d)) || (rc = h->fbuf("cg_r", R, &r)) || (rc = h->fbuf("cg_q", R, &q))) return rc;
+    HIP_TRY(hipMemcpyAsync(x, image, A * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d, d_obs, B.total_rec * 4, hipMemcpyHostToDevice, st));
+    AcouWS w;
+    if ((rc = acou_prepare(h, B, x, w))) return rc;
+    const long long cells = padded_cells(B);
+    long long props = 0;
+    auto born = [&](const float* m, float* out) -> int {  // out = L m (records)
+        int e;
+        if ((e = acou_set_refl(h, B, m, w))) return e;
+        acou_fwd(h, B, w, !w.store, true, false);
+        props += w.store ? 1 : 2;
+        HIP_TRY(hipMemcpyAsync(out, w.rec2, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
+        return 0;
+    };
+    auto adjoint = [&](const float* res, float* out) -> int {  // out = L^T res (image)
+        HIP_TRY(hipMemcpyAsync(w.rec2, res, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
+        acou_bwd(h, B, w);
+        props += w.store ? 1 : 2;
+        return acou_grad_to_global(h, B, w, out);
+    };
+    auto dot = [&](const float* a, const float* b, long long n) { return host_dot(h, a, b, n, &rc); };
+    stats_begin(h);
+    stats_loop_begin(h);
+    // r = d - L x (fills the background history in store mode)
+    acou_fwd(h, B, w, true, true, false);
+    props += 2;
+    HIP_TRY(hipMemcpyAsync(r, d, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
+    launch_axpby(st, r, w.rec2, B.total_rec, -1.f, 1.f);
+    const double dd = dot(d, d, B.total_rec);
+    if (rc) return rc;
+    if (!(dd > 0)) return fail(DS2D_EINVAL, "observed data are all zero");
+    double rr = dot(r, r, B.total_rec);
+    if (rc) return rc;
+    if (resid) resid[0] = std::sqrt(rr / dd);
+    if ((rc = adjoint(r, g))) return rc;
+    bool learned = true;
+    int accepted = 0;
+    double gz = 0, gamma = 0;
+    if ((rc = P.apply(h, g, z))) return rc;
+    HIP_TRY(hipMemcpyAsync(p, z, A * 4, hipMemcpyDeviceToDevice, st));
+    gz = dot(g, z, A);
+    if (rc) return rc;
+    for (int k = 0; k < niter; ++k) {
+        if (learned) {
+            if ((rc = born(p, q))) return rc;
+            const double qr = dot(q, r, B.total_rec), qq = dot(q, q, B.total_rec);
+            if (rc) return rc;
+            const double rr_new = qq > 0 ? rr - qr * qr / qq : rr;
+            if (qq > 0 && qr > 0 && rr_new < 0.99 * 0.99 * rr) {
+                const double alpha = qr / qq;
+                launch_axpby(st, x, p, A, (float)alpha, 1.f);
+                launch_axpby(st, r, q, B.total_rec, (float)-alpha, 1.f);
+                rr = dot(r, r, B.total_rec);
+                if (rc) return rc;
+                if (resid) resid[k + 1] = std::sqrt(rr / dd);
+                ++accepted;
+                if ((rc = adjoint(r, gn))) return rc;
+                if ((rc = P.apply(h, gn, zn))) return rc;
+                const double zg_new = dot(zn, gn, A), zg_old = dot(zn, g, A);
+                if (rc) return rc;
+                const double beta = gz > 0 ? std::max(0.0, (zg_new - zg_old) / gz) : 0.0;
+                launch_axpby(st, p, zn, A, 1.f, (float)beta);  // p = zn + beta p
+                std::swap(g, gn);
+                std::swap(z, zn);
+                gz = zg_new;
+                continue;
+            }
+            // fall back: plain CGLS from the current image, s = g
+            learned = false;
+            HIP_TRY(hipMemcpyAsync(p, g, A * 4, hipMemcpyDeviceToDevice, st));
+            gamma = dot(g, g, A);
+            if (rc) return rc;
+        }
+        if (!(gamma > 0)) {
+            if (resid)
+                for (int j = k; j < niter; ++j) resid[j + 1] = std::sqrt(rr / dd);
+            break;
+        }
+        if ((rc = born(p, q))) return rc;
+        const double qq = dot(q, q, B.total_rec);
+        if (rc) return rc;
+        const double alpha = qq > 0 ? gamma / qq : 0.0;
+        launch_axpby(st, x, p, A, (float)alpha, 1.f);
+        launch_axpby(st, r, q, B.total_rec, (float)-alpha, 1.f);
+        rr = dot(r, r, B.total_rec);
+        if (rc) return rc;
+        if (resid) resid[k + 1] = std::sqrt(rr / dd);
+        if ((rc = adjoint(r, g))) return rc;
+        const double gnew = dot(g, g, A);
+        if (rc) return rc;
+        launch_axpby(st, p, g, A, 1.f, (float)(gnew / gamma));  // p = s + beta p
+        gamma = gnew;
+    }
+    stats_loop_end(h);
+    if ((rc = stats_end(h, stats, cells * h->g.nt * props, 0, 0))) return rc;
+    if (learned_steps) *learned_steps = accepted;
     HIP_TRY(hipMemcpyAsync(image, x, A * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipGetLastError());

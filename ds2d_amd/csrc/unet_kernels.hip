@@ -1,0 +1,615 @@
+// BF16 MFMA kernels of the learned LSRTM preconditioner (U-Net), gfx950.
+//
+// The 3x3 convolutions are implicit GEMMs on v_mfma_f32_32x32x16_bf16:
+//   forward / data gradient: rows = output pixels, K = (tap, input channel),
+//     columns = output channels.  NHWC makes both MFMA operands one 16-byte
+//     load per lane (8 consecutive channels of one pixel; 8 consecutive input
+//     channels of one weight row), so no LDS is needed.
+//   weight gradient: rows = output channels, columns = input channels,
+//     K = pixels.  Both operands are transposed through LDS (channel-major
+//     rows of a 64-pixel z segment); the three z shifts of a tap row come from
+//     one 16-byte + one 4-byte LDS read with v_alignbyte, and a 3-row ring
+//     keeps each input row in LDS for the three output rows that use it.
+// MFMA lane layouts (32x32x16, 64 lanes, lane = l, li = l % 32, kh = l / 32):
+//   A[i][k]: lane i + 32 * (k / 8), element k % 8
+//   B[k][j]: lane j + 32 * (k / 8), element k % 8
+//   D[i][j]: lane j + 32 * ((i / 4) % 2), element (i % 4) + 4 * (i / 8)
+#include <hip/hip_runtime.h>
+
+#include "unet.hpp"
+
+namespace ds2d {
+namespace unet {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16_t f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (bf16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(bf16_t b) { return __uint_as_float((unsigned)b << 16); }
+__device__ __forceinline__ u32x4 ld16(const bf16_t* p) { return *reinterpret_cast<const u32x4*>(p); }
+__device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution (forward and data gradient)
+// ---------------------------------------------------------------------------
+// Block = 4 waves stacked along pixels; a wave owns WM x 32 pixels times
+// WN x 32 output channels (WM * WN accumulator tiles of 16 floats per lane).
+template <int WM, int WN>
+__global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 31, kh = lane >> 5;
+    const long long npix = (long long)a.n * a.h * a.w;
+    const long long pbase = ((long long)blockIdx.x * 4 + wave) * (WM * 32);
+    if (pbase >= npix) return;  // no barriers in this kernel
+    const int co0 = blockIdx.y * (WN * 32);
+    const int ctot = a.c1 + a.c2;
+    const int h2 = a.h >> a.up, w2 = a.w >> a.up;
+    int pn[WM], px[WM], pz[WM];
+    bool pv[WM];
+#pragma unroll
+    for (int m = 0; m < WM; ++m) {
+        const long long p = pbase + m * 32 + li;
+        pv[m] = p < npix;
+        long long q = pv[m] ? p : 0;
+        pz[m] = (int)(q % a.w);
+        q /= a.w;
+        px[m] = (int)(q % a.h);
+        pn[m] = (int)(q / a.h);
+    }
+    f32x16 acc[WM][WN];
+#pragma unroll
+    for (int m = 0; m < WM; ++m)
+#pragma unroll
+        for (int j = 0; j < WN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[m][j][r] = 0.f;
+    const u32x4 zero = {0u, 0u, 0u, 0u};
+    for (int tap = 0; tap < 9; ++tap) {
+        const int dx = tap / 3 - 1, dz = tap % 3 - 1;
+        const bf16_t* s1[WM];
+        const bf16_t* s2[WM];
+#pragma unroll
+        for (int m = 0; m < WM; ++m) {
+            const int xs = px[m] + dx, zs = pz[m] + dz;
+            const bool ok = pv[m] && xs >= 0 && xs < a.h && zs >= 0 && zs < a.w;
+            s1[m] = ok ? a.x1 + (((long long)pn[m] * a.h + xs) * a.w + zs) * a.c1 + 8 * kh : nullptr;
+            s2[m] = (ok && a.c2) ? a.x2 + (((long long)pn[m] * h2 + (xs >> a.up)) * w2 + (zs >> a.up)) * a.c2 + 8 * kh
+                                 : nullptr;
+        }
+        const bf16_t* wt = a.wt + ((long long)tap * a.cout + co0 + li) * ctot + 8 * kh;
+#pragma unroll 2
+        for (int c = 0; c < ctot; c += 16) {
+            u32x4 A[WM], B[WN];
+#pragma unroll
+            for (int m = 0; m < WM; ++m)
+                A[m] = !s1[m] ? zero : (c < a.c1 ? ld16(s1[m] + c) : ld16(s2[m] + (c - a.c1)));
+#pragma unroll
+            for (int j = 0; j < WN; ++j) B[j] = ld16(wt + (long long)j * 32 * ctot + c);
+#pragma unroll
+            for (int m = 0; m < WM; ++m)
+#pragma unroll
+                for (int j = 0; j < WN; ++j) acc[m][j] = mfma(A[m], B[j], acc[m][j]);
+        }
+    }
+    const int c2o = a.cout - a.split;
+#pragma unroll
+    for (int m = 0; m < WM; ++m)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) {
+            const int co = co0 + j * 32 + li;
+            const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = (r & 3) + 4 * kh + 8 * (r >> 2);
+                const long long p = pbase + m * 32 + i;
+                if (p >= npix) continue;
+                float v = acc[m][j][r] + b;
+                if (a.relu) v = fmaxf(v, 0.f);
+                if (a.mask && !(bf2f(a.mask[p * a.cout + co]) > 0.f)) v = 0.f;
+                if (co < a.split) {
+                    if (a.out_bf16)
+                        ((bf16_t*)a.y1)[p * a.split + co] = f2bf(v);
+                    else
+                        ((float*)a.y1)[p * a.split + co] = v;
+                } else {
+                    if (a.out_bf16)
+                        ((bf16_t*)a.y2)[p * c2o + co - a.split] = f2bf(v);
+                    else
+                        ((float*)a.y2)[p * c2o + co - a.split] = v;
+                }
+            }
+        }
+}
+
+void launch_conv(hipStream_t st, const ConvArgs& a) {
+    const long long npix = (long long)a.n * a.h * a.w;
+    if (a.cout % 64 == 0) {
+        dim3 grid((unsigned)((npix + 255) / 256), a.cout / 64);
+        hipLaunchKernelGGL((k_conv<2, 2>), grid, dim3(256), 0, st, a);
+    } else {
+        dim3 grid((unsigned)((npix + 511) / 512), a.cout / 32);
+        hipLaunchKernelGGL((k_conv<4, 1>), grid, dim3(256), 0, st, a);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 weight gradient
+// ---------------------------------------------------------------------------
+constexpr int ZS = 64;  // z segment (pixels of K per row)
+constexpr int LP = 72;  // LDS row pitch in elements (144 B: 16-byte aligned rows)
+
+template <int TCO, int TCI>
+__global__ __launch_bounds__(64 * (TCO / 32) * (TCI / 32)) void k_wgrad(WgradArgs a) {
+    constexpr int NT = 64 * (TCO / 32) * (TCI / 32);
+    __shared__ __attribute__((aligned(16))) bf16_t As[TCO][LP];
+    __shared__ __attribute__((aligned(16))) bf16_t Bs[3][TCI][LP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 31, kh = lane >> 5;
+    const int cot = wave % (TCO / 32), cit = wave / (TCO / 32);
+    const int cin = a.c1 + a.c2;
+    const int ntci = cin / TCI;
+    const int co0 = (blockIdx.y / ntci) * TCO, ci0 = (blockIdx.y % ntci) * TCI;
+    const int chunk = blockIdx.x;
+    const int xb = chunk % a.nxblk;
+    const int seg = (chunk / a.nxblk) % a.nseg;
+    const int n = chunk / (a.nxblk * a.nseg);
+    const int zs = seg * ZS;
+    const int xa = xb * a.rows, xe = min(a.h, xa + a.rows);
+    const int h2 = a.h >> a.up, w2 = a.w >> a.up;
+    const int nks = (min(ZS, a.w - zs) + 15) / 16;
+
+    // Bs[slot][ci][zz] = in[n][x][zs + zz - 1][ci0 + ci] for zz in [0, ZS + 2), zero outside the image
+    auto load_in_row = [&](int x, int slot) {
+        constexpr int G = TCI / 8;
+        for (int q = tid; q < (ZS + 2) * G; q += NT) {
+            const int zz = q / G, g = q % G;
+            const int z = zs + zz - 1, c = ci0 + g * 8;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if (x >= 0 && x < a.h && z >= 0 && z < a.w)
+                v = c < a.c1 ? ld16(a.x1 + (((long long)n * a.h + x) * a.w + z) * a.c1 + c)
+                             : ld16(a.x2 + (((long long)n * h2 + (x >> a.up)) * w2 + (z >> a.up)) * a.c2 + (c - a.c1));
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                Bs[slot][g * 8 + 2 * k][zz] = (bf16_t)(v[k] & 0xFFFFu);
+                Bs[slot][g * 8 + 2 * k + 1][zz] = (bf16_t)(v[k] >> 16);
+            }
+        }
+    };
+    // As[co][zz] = dy[n][x][zs + zz][co0 + co], zero beyond the image
+    auto load_dy_row = [&](int x) {
+        constexpr int G = TCO / 8;
+        for (int q = tid; q < ZS * G; q += NT) {
+            const int zz = q / G, g = q % G;
+            const int z = zs + zz;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if (z < a.w) v = ld16(a.dy + (((long long)n * a.h + x) * a.w + z) * a.cout + co0 + g * 8);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                As[g * 8 + 2 * k][zz] = (bf16_t)(v[k] & 0xFFFFu);
+                As[g * 8 + 2 * k + 1][zz] = (bf16_t)(v[k] >> 16);
+            }
+        }
+    };
+
+    f32x16 acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
+
+    // ring slot of input row x (x >= -1): (x + 3) % 3
+    load_in_row(xa - 1, (xa + 2) % 3);
+    load_in_row(xa, xa % 3);
+    for (int x = xa; x < xe; ++x) {
+        load_in_row(x + 1, (x + 1) % 3);
+        load_dy_row(x);
+        __syncthreads();
+        for (int s = 0; s < nks; ++s) {
+            const int base = 16 * s + 8 * kh;
+            const u32x4 A = *reinterpret_cast<const u32x4*>(&As[cot * 32 + li][base]);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {  // dx = d - 1
+                const bf16_t* row = &Bs[(x + d + 2) % 3][cit * 32 + li][base];
+                const u32x4 q = *reinterpret_cast<const u32x4*>(row);
+                const unsigned e = *reinterpret_cast<const unsigned*>(row + 8);
+                const u32x4 bm = q;                                                   // dz = -1
+                const u32x4 b0 = {__builtin_amdgcn_alignbyte(q[1], q[0], 2), __builtin_amdgcn_alignbyte(q[2], q[1], 2),
+                                  __builtin_amdgcn_alignbyte(q[3], q[2], 2), __builtin_amdgcn_alignbyte(e, q[3], 2)};
+                const u32x4 bp = {q[1], q[2], q[3], e};                               // dz = +1
+                acc[d * 3 + 0] = mfma(A, bm, acc[d * 3 + 0]);
+                acc[d * 3 + 1] = mfma(A, b0, acc[d * 3 + 1]);
+                acc[d * 3 + 2] = mfma(A, bp, acc[d * 3 + 2]);
+            }
+        }
+        __syncthreads();
+    }
+    float* out = a.part + (long long)chunk * 9 * a.cout * cin;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = (r & 3) + 4 * kh + 8 * (r >> 2);
+            const int co = co0 + cot * 32 + i, ci = ci0 + cit * 32 + li;
+            out[((long long)t * a.cout + co) * cin + ci] = acc[t][r];
+        }
+}
+
+void wgrad_plan(int n, int h, int w, int cout, int cin, long long max_part, int* rows, int* nchunk) {
+    const int tco = cout % 64 == 0 ? 64 : 32, tci = cin % 64 == 0 ? 64 : 32;
+    const long long tiles = (long long)(cout / tco) * (cin / tci);
+    const int nseg = (w + ZS - 1) / ZS;
+    const long long want = std::max<long long>(1, (1024 + tiles - 1) / tiles);  // chunks for ~1024 blocks
+    long long per = std::max<long long>(1, (want + (long long)n * nseg - 1) / ((long long)n * nseg));
+    int r = (int)std::max<long long>(1, (h + per - 1) / per);
+    r = std::max(r, std::min(4, h));
+    const long long per_chunk = 9LL * cout * cin;
+    for (;;) {
+        const long long nc = (long long)n * nseg * ((h + r - 1) / r);
+        if (nc * per_chunk <= max_part || r >= h) {
+            *rows = r;
+            *nchunk = (int)nc;
+            return;
+        }
+        r = std::min(h, r * 2);
+    }
+}
+
+void launch_wgrad(hipStream_t st, const WgradArgs& a) {
+    const int cin = a.c1 + a.c2;
+    const int tco = a.cout % 64 == 0 ? 64 : 32, tci = cin % 64 == 0 ? 64 : 32;
+    dim3 grid((unsigned)(a.n * a.nseg * a.nxblk), (a.cout / tco) * (cin / tci));
+    if (tco == 64 && tci == 64)
+        hipLaunchKernelGGL((k_wgrad<64, 64>), grid, dim3(256), 0, st, a);
+    else if (tco == 64)
+        hipLaunchKernelGGL((k_wgrad<64, 32>), grid, dim3(128), 0, st, a);
+    else if (tci == 64)
+        hipLaunchKernelGGL((k_wgrad<32, 64>), grid, dim3(128), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_wgrad<32, 32>), grid, dim3(64), 0, st, a);
+}
+
+__global__ void k_reduce(const float* part, int nchunk, long long len, float* out, int accumulate) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= len) return;
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += part[(long long)c * len + e];
+    out[e] = accumulate ? out[e] + s : s;
+}
+
+void launch_reduce(hipStream_t st, const float* part, int nchunk, long long len, float* out, int accumulate) {
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, nchunk, len, out,
+                       accumulate);
+}
+
+// ---------------------------------------------------------------------------
+// First layer (one fp32 input channel), biases, head
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float tap_in(const float* x, int n, int h, int w, int ix, int iz, int t) {
+    const int xs = ix + t / 3 - 1, zs = iz + t % 3 - 1;
+    return (xs >= 0 && xs < h && zs >= 0 && zs < w) ? x[((long long)n * h + xs) * w + zs] : 0.f;
+}
+
+__global__ void k_conv1(const float* x, const float* wt, const float* b, bf16_t* y, int n, int h, int w, int cout) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long npix = (long long)n * h * w;
+    if (p >= npix) return;
+    const int iz = (int)(p % w), ix = (int)((p / w) % h), in = (int)(p / ((long long)w * h));
+    float v[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) v[t] = tap_in(x, in, h, w, ix, iz, t);
+    for (int c0 = 0; c0 < cout; c0 += 8) {
+        u32x4 o;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float s[2];
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int co = c0 + 2 * k + e;
+                float acc = 0.f;
+#pragma unroll
+                for (int t = 0; t < 9; ++t) acc += v[t] * wt[t * cout + co];
+                s[e] = fmaxf(acc + b[co], 0.f);
+            }
+            o[k] = (unsigned)f2bf(s[0]) | ((unsigned)f2bf(s[1]) << 16);
+        }
+        *reinterpret_cast<u32x4*>(y + p * cout + c0) = o;
+    }
+}
+
+void launch_conv1(hipStream_t st, const float* x, const float* w, const float* b, bf16_t* y, int n, int h, int wd,
+                  int cout) {
+    const long long npix = (long long)n * h * wd;
+    hipLaunchKernelGGL(k_conv1, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, x, w, b, y, n, h, wd, cout);
+}
+
+static int reduce_blocks(long long npix) { return (int)std::min<long long>(1024, (npix + 255) / 256); }
+int conv1_wgrad_blocks(long long npix) { return reduce_blocks(npix); }
+int bias_blocks(long long npix) { return reduce_blocks(npix); }
+int head_blocks(long long npix) { return reduce_blocks(npix); }
+
+// part[blk][t * cout + co] = sum_{p in blk} dy[p, co] * x[p + off(t)]; cout divides 256
+__global__ __launch_bounds__(256) void k_conv1_wgrad(const float* x, const bf16_t* dy, float* part, int n, int h,
+                                                     int w, int cout) {
+    __shared__ float red[9][256];
+    const long long npix = (long long)n * h * w;
+    const long long per = (npix + gridDim.x - 1) / gridDim.x;
+    const long long p0 = blockIdx.x * per, p1 = std::min(npix, p0 + per);
+    const int c = threadIdx.x % cout, sub = threadIdx.x / cout, nsub = 256 / cout;
+    float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (long long p = p0 + sub; p < p1; p += nsub) {
+        const int iz = (int)(p % w), ix = (int)((p / w) % h), in = (int)(p / ((long long)w * h));
+        const float g = bf2f(dy[p * cout + c]);
+#pragma unroll
+        for (int t = 0; t < 9; ++t) acc[t] += g * tap_in(x, in, h, w, ix, iz, t);
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) red[t][threadIdx.x] = acc[t];
+    __syncthreads();
+    if (threadIdx.x < cout)
+        for (int t = 0; t < 9; ++t) {
+            float s = 0.f;
+            for (int k = 0; k < nsub; ++k) s += red[t][k * cout + c];
+            part[(long long)blockIdx.x * 9 * cout + t * cout + c] = s;
+        }
+}
+
+void launch_conv1_wgrad(hipStream_t st, const float* x, const bf16_t* dy, float* part, int n, int h, int wd,
+                        int cout) {
+    const long long npix = (long long)n * h * wd;
+    hipLaunchKernelGGL(k_conv1_wgrad, dim3(conv1_wgrad_blocks(npix)), dim3(256), 0, st, x, dy, part, n, h, wd, cout);
+}
+
+// part[blk][co] = sum_{p in blk} dy[p, co]; cout divides 256
+__global__ __launch_bounds__(256) void k_bias_grad(const bf16_t* dy, long long npix, int cout, float* part) {
+    __shared__ float red[256];
+    const long long per = (npix + gridDim.x - 1) / gridDim.x;
+    const long long p0 = blockIdx.x * per, p1 = std::min(npix, p0 + per);
+    const int c = threadIdx.x % cout, sub = threadIdx.x / cout, nsub = 256 / cout;
+    float acc = 0.f;
+    for (long long p = p0 + sub; p < p1; p += nsub) acc += bf2f(dy[p * cout + c]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < cout) {
+        float s = 0.f;
+        for (int k = 0; k < nsub; ++k) s += red[k * cout + c];
+        part[(long long)blockIdx.x * cout + c] = s;
+    }
+}
+
+void launch_bias_grad(hipStream_t st, const bf16_t* dy, long long npix, int cout, float* part) {
+    hipLaunchKernelGGL(k_bias_grad, dim3(bias_blocks(npix)), dim3(256), 0, st, dy, npix, cout, part);
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// C0 channels (multiple of 8); 256 threads, one pixel per thread per pass
+template <int c0>
+__global__ __launch_bounds__(256) void k_head(const bf16_t* a, const float* wh, const float* bh, float* y,
+                                              const float* t, bf16_t* dz, float* part, double* part_loss,
+                                              long long npix) {
+    __shared__ float red[4][c0 + 1];
+    __shared__ double redl[4];
+    const long long per = (npix + gridDim.x - 1) / gridDim.x;
+    const long long p0 = blockIdx.x * per, p1 = std::min(npix, p0 + per);
+    float gw[c0 + 1];
+#pragma unroll
+    for (int c = 0; c <= c0; ++c) gw[c] = 0.f;
+    double ls = 0.0;
+    for (long long p = p0 + threadIdx.x; p < p1; p += 256) {
+        float av[c0];
+#pragma unroll
+        for (int c = 0; c < c0; c += 8) {
+            const u32x4 v = ld16(a + p * c0 + c);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                av[c + 2 * k] = bf2f((bf16_t)(v[k] & 0xFFFFu));
+                av[c + 2 * k + 1] = bf2f((bf16_t)(v[k] >> 16));
+            }
+        }
+        float s = 0.f;
+#pragma unroll
+        for (int c = 0; c < c0; ++c) s += av[c] * wh[c];
+        s += bh[0];
+        y[p] = s;
+        if (t) {
+            const float diff = s - t[p];
+            ls += (double)diff * diff;
+            const float g = 2.f * diff / (float)npix;
+#pragma unroll
+            for (int c = 0; c < c0; c += 8) {
+                u32x4 o;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int ca = c + 2 * k, cb = ca + 1;
+                    const float za = av[ca] > 0.f ? g * wh[ca] : 0.f;
+                    const float zb = av[cb] > 0.f ? g * wh[cb] : 0.f;
+                    o[k] = (unsigned)f2bf(za) | ((unsigned)f2bf(zb) << 16);
+                }
+                *reinterpret_cast<u32x4*>(dz + p * c0 + c) = o;
+            }
+#pragma unroll
+            for (int c = 0; c < c0; ++c) gw[c] += g * av[c];
+            gw[c0] += g;
+        }
+    }
+    if (!t) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c <= c0; ++c) {
+        const float v = wave_sum(gw[c]);
+        if (lane == 0) red[wave][c] = v;
+    }
+    const double lv = wave_sum_d(ls);
+    if (lane == 0) redl[wave] = lv;
+    __syncthreads();
+    if (threadIdx.x <= c0)
+        part[(long long)blockIdx.x * (c0 + 1) + threadIdx.x] =
+            red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    if (threadIdx.x == 0) part_loss[blockIdx.x] = redl[0] + redl[1] + redl[2] + redl[3];
+}
+
+void launch_head(hipStream_t st, const bf16_t* a, const float* wh, const float* bh, float* y, const float* t,
+                 bf16_t* dz, float* part, double* part_loss, long long npix, int c0) {
+    const dim3 g(head_blocks(npix));
+    switch (c0) {
+        case 8: hipLaunchKernelGGL(k_head<8>, g, dim3(256), 0, st, a, wh, bh, y, t, dz, part, part_loss, npix); break;
+        case 16: hipLaunchKernelGGL(k_head<16>, g, dim3(256), 0, st, a, wh, bh, y, t, dz, part, part_loss, npix); break;
+        case 32: hipLaunchKernelGGL(k_head<32>, g, dim3(256), 0, st, a, wh, bh, y, t, dz, part, part_loss, npix); break;
+        default: hipLaunchKernelGGL(k_head<64>, g, dim3(256), 0, st, a, wh, bh, y, t, dz, part, part_loss, npix); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pooling, upsampling gradients, weight conversion, Adam
+// ---------------------------------------------------------------------------
+// y[n][x][z][c] = bf16(mean of the 2x2 block), 8 channels per thread
+__global__ void k_pool(const bf16_t* x, bf16_t* y, int n, int h, int w, int c) {
+    const int ho = h / 2, wo = w / 2, G = c / 8;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n * ho * wo * G) return;
+    const int g = (int)(e % G);
+    const long long q = e / G;
+    const int oz = (int)(q % wo), ox = (int)((q / wo) % ho), in = (int)(q / ((long long)wo * ho));
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int da = 0; da < 2; ++da)
+        for (int db = 0; db < 2; ++db) {
+            const u32x4 v = ld16(x + (((long long)in * h + 2 * ox + da) * w + 2 * oz + db) * c + g * 8);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                s[2 * k] += bf2f((bf16_t)(v[k] & 0xFFFFu));
+                s[2 * k + 1] += bf2f((bf16_t)(v[k] >> 16));
+            }
+        }
+    u32x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = (unsigned)f2bf(0.25f * s[2 * k]) | ((unsigned)f2bf(0.25f * s[2 * k + 1]) << 16);
+    *reinterpret_cast<u32x4*>(y + q * c + g * 8) = o;
+}
+
+void launch_pool(hipStream_t st, const bf16_t* x, bf16_t* y, int n, int h, int w, int c) {
+    const long long m = (long long)n * (h / 2) * (w / 2) * (c / 8);
+    hipLaunchKernelGGL(k_pool, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, x, y, n, h, w, c);
+}
+
+__global__ void k_combine_pool(const float* dp, const float* dskip, const bf16_t* act, bf16_t* dz, int n, int h,
+                               int w, int c) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n * h * w * c) return;
+    const int ch = (int)(e % c);
+    const long long p = e / c;
+    const int iz = (int)(p % w), ix = (int)((p / w) % h), in = (int)(p / ((long long)w * h));
+    const long long q = ((long long)in * (h / 2) + ix / 2) * (w / 2) + iz / 2;
+    float v = dp[q * c + ch] * 0.25f + (dskip ? dskip[e] : 0.f);
+    dz[e] = f2bf(bf2f(act[e]) > 0.f ? v : 0.f);
+}
+
+void launch_combine_pool(hipStream_t st, const float* dp, const float* dskip, const bf16_t* act, bf16_t* dz, int n,
+                         int h, int w, int c) {
+    const long long m = (long long)n * h * w * c;
+    hipLaunchKernelGGL(k_combine_pool, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, dp, dskip, act, dz, n, h,
+                       w, c);
+}
+
+__global__ void k_combine_up(const float* dup, const bf16_t* act, bf16_t* dz, int n, int h, int w, int c) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n * h * w * c) return;
+    const int ch = (int)(e % c);
+    const long long p = e / c;
+    const int iz = (int)(p % w), ix = (int)((p / w) % h), in = (int)(p / ((long long)w * h));
+    const int H = 2 * h, W = 2 * w;
+    float s = 0.f;
+    for (int da = 0; da < 2; ++da)
+        for (int db = 0; db < 2; ++db) s += dup[(((long long)in * H + 2 * ix + da) * W + 2 * iz + db) * c + ch];
+    dz[e] = f2bf(bf2f(act[e]) > 0.f ? s : 0.f);
+}
+
+void launch_combine_up(hipStream_t st, const float* dup, const bf16_t* act, bf16_t* dz, int n, int h, int w, int c) {
+    const long long m = (long long)n * h * w * c;
+    hipLaunchKernelGGL(k_combine_up, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, dup, act, dz, n, h, w, c);
+}
+
+__global__ void k_prep_weights(const float* w, bf16_t* wf, bf16_t* wb, int cout, int cin) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per = (long long)cout * cin;
+    if (e >= 9 * per) return;
+    const int t = (int)(e / per);
+    const int co = (int)((e % per) / cin), ci = (int)(e % cin);
+    const bf16_t v = f2bf(w[e]);
+    wf[e] = v;
+    wb[((long long)(8 - t) * cin + ci) * cout + co] = v;
+}
+
+void launch_prep_weights(hipStream_t st, const float* w, bf16_t* wf, bf16_t* wb, int cout, int cin) {
+    const long long m = 9LL * cout * cin;
+    hipLaunchKernelGGL(k_prep_weights, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, w, wf, wb, cout, cin);
+}
+
+__global__ void k_to_bf16(const float* x, bf16_t* y, long long n) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) y[e] = f2bf(x[e]);
+}
+__global__ void k_from_bf16(const bf16_t* x, float* y, long long n) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) y[e] = bf2f(x[e]);
+}
+void launch_to_bf16(hipStream_t st, const float* x, bf16_t* y, long long n) {
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, y, n);
+}
+void launch_from_bf16(hipStream_t st, const bf16_t* x, float* y, long long n) {
+    hipLaunchKernelGGL(k_from_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, y, n);
+}
+
+__global__ void k_adam(float* th, const float* g, float* m, float* v, long long n, float lr, float b1, float b2,
+                       float eps, float bc1, float bc2) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const float gi = g[e];
+    const float mi = b1 * m[e] + (1.f - b1) * gi;
+    const float vi = b2 * v[e] + (1.f - b2) * gi * gi;
+    m[e] = mi;
+    v[e] = vi;
+    th[e] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+void launch_adam(hipStream_t st, float* theta, const float* g, float* m, float* v, long long n, float lr, float b1,
+                 float b2, float eps, float bc1, float bc2) {
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, theta, g, m, v, n, lr, b1, b2, eps,
+                       bc1, bc2);
+}
+
+__global__ void k_sum_double(const double* part, int n, double* out, double scale) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = scale * red[0];
+}
+
+void launch_sum_double(hipStream_t st, const double* part, int n, double* out, double scale) {
+    hipLaunchKernelGGL(k_sum_double, dim3(1), dim3(256), 0, st, part, n, out, scale);
+}
+
+}  // namespace unet
+}  // namespace ds2d

@@ -45,6 +45,19 @@ class Stats(C.Structure):
 _F = C.POINTER(C.c_float)
 _D = C.POINTER(C.c_double)
 _H = C.c_void_p
+class UnetCfg(C.Structure):
+    _fields_ = [("levels", C.c_int), ("base", C.c_int), ("max_batch", C.c_int), ("max_h", C.c_int),
+                ("max_w", C.c_int), ("seed", C.c_ulonglong)]
+
+
+class UnetStats(C.Structure):
+    _fields_ = [("elapsed_ms", C.c_double), ("ms_per_step", C.c_double), ("flops_per_step", C.c_double),
+                ("steps", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 SIGNATURES = {
     "ds2d_last_error": (C.c_char_p, []),
     "ds2d_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), _D]),
@@ -65,6 +78,19 @@ SIGNATURES = {
     "ds2d_born_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, _F, C.POINTER(Stats)]),
     "ds2d_born_adjoint": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.POINTER(Stats)]),
     "ds2d_lsrtm_cg": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, C.c_int, _F, _D, C.POINTER(Stats)]),
+    "ds2d_lsrtm_learned": (C.c_int, [_H, _H, C.c_int, C.POINTER(Shot), _F, C.c_int, _F, _D, C.POINTER(C.c_int),
+                                     C.POINTER(Stats)]),
+    "ds2d_set_batch": (C.c_int, [_H, C.c_int]),
+    "ds2d_unet_create": (C.c_int, [C.POINTER(UnetCfg), C.POINTER(_H)]),
+    "ds2d_unet_destroy": (C.c_int, [_H]),
+    "ds2d_unet_num_params": (C.c_int, [_H, C.POINTER(C.c_longlong)]),
+    "ds2d_unet_get_params": (C.c_int, [_H, _F]),
+    "ds2d_unet_set_params": (C.c_int, [_H, _F]),
+    "ds2d_unet_forward": (C.c_int, [_H, C.c_int, C.c_int, C.c_int, _F, _F, C.POINTER(UnetStats)]),
+    "ds2d_unet_loss_grad": (C.c_int, [_H, C.c_int, C.c_int, C.c_int, _F, _F, _D, _F]),
+    "ds2d_unet_set_data": (C.c_int, [_H, C.c_int, C.c_int, C.c_int, _F, _F]),
+    "ds2d_unet_train": (C.c_int, [_H, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _F, C.POINTER(UnetStats)]),
+    "ds2d_conv3x3_bf16": (C.c_int, [C.c_int] * 8 + [_F, _F, _F, _F, _F, C.c_int, C.c_int, _F, _D]),
     "ds2d_laplace_filter": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_float, _F]),
     "ds2d_laplace_band": (C.c_int, [C.c_int, C.c_int, _F]),
     "ds2d_ricker": (C.c_int, [C.c_int, C.c_float, C.c_float, _F]),
@@ -251,6 +277,24 @@ class Solver:
                                           C.byref(st)))
         return g.reshape(self.allnx, self.allnz), st.as_dict()
 
+    def set_batch(self, shots_per_launch):
+        """Shots per time loop for the plain Born/adjoint/CG entries (0 = all)."""
+        _check(self.lib.ds2d_set_batch(self.h, int(shots_per_launch)))
+
+    def lsrtm_learned(self, unet, shots, d_obs, niter, image0=None):
+        """LSRTM with the U-Net preconditioner (plain CGLS fallback)."""
+        img = np.zeros(self.allnx * self.allnz, np.float32) if image0 is None else \
+            np.ascontiguousarray(image0, np.float32).ravel().copy()
+        res = np.zeros(niter + 1, np.float64)
+        st = Stats()
+        acc = C.c_int(0)
+        d = np.ascontiguousarray(d_obs, np.float32)
+        _check(self.lib.ds2d_lsrtm_learned(self.h, unet.h, len(shots), _shots_array(shots), fptr(d), niter,
+                                           fptr(img), res.ctypes.data_as(_D), C.byref(acc), C.byref(st)))
+        out = st.as_dict()
+        out["learned_steps"] = acc.value
+        return img.reshape(self.allnx, self.allnz), res, out
+
     def lsrtm_cg(self, shots, d_obs, niter, image0=None):
         img = np.zeros(self.allnx * self.allnz, np.float32) if image0 is None else \
             np.ascontiguousarray(image0, np.float32).ravel().copy()
@@ -260,3 +304,96 @@ class Solver:
         _check(self.lib.ds2d_lsrtm_cg(self.h, len(shots), _shots_array(shots), fptr(d), niter, fptr(img),
                                       res.ctypes.data_as(_D), C.byref(st)))
         return img.reshape(self.allnx, self.allnz), res, st.as_dict()
+
+
+class Unet:
+    """The learned LSRTM preconditioner (BF16 MFMA U-Net) of libds2d.so."""
+
+    def __init__(self, *, levels=4, base=32, max_batch=8, max_h=512, max_w=256, seed=0):
+        self.lib = load()
+        self.levels, self.base = levels, base
+        cfg = UnetCfg(levels, base, max_batch, max_h, max_w, seed)
+        self.h = _H()
+        _check(self.lib.ds2d_unet_create(C.byref(cfg), C.byref(self.h)))
+        n = C.c_longlong(0)
+        _check(self.lib.ds2d_unet_num_params(self.h, C.byref(n)))
+        self.num_params = n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ds2d_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        th = np.zeros(self.num_params, np.float32)
+        _check(self.lib.ds2d_unet_get_params(self.h, fptr(th)))
+        return th
+
+    def set_params(self, theta):
+        th = np.ascontiguousarray(theta, np.float32)
+        if th.size != self.num_params:
+            raise ValueError(f"expected {self.num_params} parameters, got {th.size}")
+        _check(self.lib.ds2d_unet_set_params(self.h, fptr(th)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        n, hh, ww = x.shape
+        y = np.zeros_like(x)
+        st = UnetStats()
+        _check(self.lib.ds2d_unet_forward(self.h, n, hh, ww, fptr(x), fptr(y), C.byref(st)))
+        return y, st.as_dict()
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        n, hh, ww = x.shape
+        g = np.zeros(self.num_params, np.float32)
+        loss = C.c_double(0)
+        _check(self.lib.ds2d_unet_loss_grad(self.h, n, hh, ww, fptr(x), fptr(t), C.byref(loss), fptr(g)))
+        return loss.value, g
+
+    def set_data(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        n, hh, ww = x.shape
+        _check(self.lib.ds2d_unet_set_data(self.h, n, hh, ww, fptr(x), fptr(t)))
+
+    def train(self, nsteps, batch=8, lr=1e-3, seed=0):
+        lh = np.zeros(nsteps, np.float32)
+        st = UnetStats()
+        _check(self.lib.ds2d_unet_train(self.h, nsteps, batch, lr, seed, fptr(lh), C.byref(st)))
+        return lh, st.as_dict()
+
+
+def conv3x3_bf16(op, x1, wt, *, x2=None, up=0, dy=None, bias=None, relu=False, repeat=1):
+    """One BF16 MFMA conv kernel on NHWC float data (op 0 fwd, 1 data grad, 2 weight grad).
+    Returns (out, average kernel ms)."""
+    lib = load()
+    x1 = np.ascontiguousarray(x1, np.float32)
+    wt = np.ascontiguousarray(wt, np.float32)
+    n, hh, ww, c = x1.shape
+    cout = wt.shape[1]
+    cin = wt.shape[2]
+    c1 = c if op != 1 else cin
+    c2 = 0 if x2 is None else x2.shape[-1]
+    if op == 0:
+        out = np.zeros((n, hh, ww, cout), np.float32)
+    elif op == 1:
+        out = np.zeros((n, hh, ww, cin), np.float32)
+    else:
+        out = np.zeros((9, cout, cin), np.float32)
+    ms = C.c_double(0)
+    x2a = None if x2 is None else np.ascontiguousarray(x2, np.float32)
+    dya = None if dy is None else np.ascontiguousarray(dy, np.float32)
+    ba = None if bias is None else np.ascontiguousarray(bias, np.float32)
+    _check(lib.ds2d_conv3x3_bf16(op, n, hh, ww, c1 - c2 if op == 1 else c1, c2, up, cout, fptr(x1),
+                                 fptr(x2a) if x2a is not None else None, fptr(dya) if dya is not None else None,
+                                 fptr(wt), fptr(ba) if ba is not None else None, int(relu), repeat, fptr(out),
+                                 C.byref(ms)))
+    return out, ms.value

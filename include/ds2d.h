@@ -149,6 +149,69 @@ int ds2d_born_adjoint(ds2d_t* h, int nshots, const ds2d_shot* shots, const float
 int ds2d_lsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_obs, int niter,
                   float* image, double* resid, ds2d_stats* stats);
 
+/* ---------------------------------------------------------------------------
+ * Learned LSRTM preconditioner (no reference counterpart: new capability).
+ * A U-Net (levels, base channels; 3x3 convs + ReLU, 2x2 average pooling,
+ * nearest upsampling with skip concatenation, 1x1 head) mapping an RTM image
+ * to a multi-iteration LSRTM image.  Weights and activations are BF16 on
+ * the MFMA units with FP32 accumulation; master weights, gradients and Adam
+ * moments are FP32.  Images are [n][h][w] floats (an [ix][iz] image is
+ * h = nx, w = nz); h and w must be multiples of 2^(levels-1).
+ * Parameters are flattened layer by layer (enc0a, enc0b, ..., enc{L-1}b,
+ * dec{L-2}a, dec{L-2}b, ..., dec0b, head), each weight ([tap][cout][cin],
+ * tap = (dx+1)*3 + (dz+1)) followed by its bias.
+ * ------------------------------------------------------------------------- */
+typedef struct ds2d_unet ds2d_unet_t;
+typedef struct {
+    int levels;              /* 4 */
+    int base;                /* base channels, 32 (multiple of 32)            */
+    int max_batch, max_h, max_w;
+    unsigned long long seed; /* He-uniform initialisation, zero biases         */
+} ds2d_unet_cfg;
+typedef struct {
+    double elapsed_ms;       /* whole call                                     */
+    double ms_per_step;      /* training: per Adam step; inference: per call   */
+    double flops_per_step;   /* algorithmic conv FLOPs (2 per MAC)             */
+    int steps;
+} ds2d_unet_stats;
+
+int ds2d_unet_create(const ds2d_unet_cfg* cfg, ds2d_unet_t** out);
+int ds2d_unet_destroy(ds2d_unet_t* u);
+int ds2d_unet_num_params(const ds2d_unet_t* u, long long* n);
+int ds2d_unet_get_params(ds2d_unet_t* u, float* theta);
+int ds2d_unet_set_params(ds2d_unet_t* u, const float* theta); /* also resets Adam */
+int ds2d_unet_forward(ds2d_unet_t* u, int n, int h, int w, const float* x, float* y, ds2d_unet_stats* stats);
+/* Mean-squared-error loss of forward(x) against t and its gradient. */
+int ds2d_unet_loss_grad(ds2d_unet_t* u, int n, int h, int w, const float* x, const float* t, double* loss,
+                        float* grad);
+/* Keep a training set resident in HBM, then run Adam steps on random batches
+ * (seeded, without replacement within an epoch); loss_hist[nsteps]. */
+int ds2d_unet_set_data(ds2d_unet_t* u, int nsamples, int h, int w, const float* x, const float* t);
+int ds2d_unet_train(ds2d_unet_t* u, int nsteps, int batch, float lr, unsigned long long seed, float* loss_hist,
+                    ds2d_unet_stats* stats);
+/* One BF16 MFMA 3x3 convolution kernel on host FP32 data (inputs rounded to
+ * BF16 on the device, FP32 outputs).  op 0: y = conv(concat(x1, up^up(x2)))
+ * [+ bias] [ReLU] -> out [npix][cout]; op 1: data gradient of that conv,
+ * x1 = dy [npix][cout] -> out [npix][c1 + c2]; op 2: weight gradient, dy as
+ * extra input -> out [9][cout][c1 + c2].  kernel_ms: average duration of the
+ * conv kernel over `repeat` launches (HIP events). */
+int ds2d_conv3x3_bf16(int op, int n, int h, int w, int c1, int c2, int up, int cout, const float* x1,
+                      const float* x2, const float* dy, const float* wt, const float* bias, int relu, int repeat,
+                      float* out, double* kernel_ms);
+
+/* Learned LSRTM: flexible preconditioned CG on the plain Born operator in which
+ * the direction is the U-Net applied to the (RMS-normalised) gradient.  A step
+ * whose exact line search lowers the data residual by less than 1% (or a
+ * non-descent direction) is discarded and the solver continues as plain CGLS
+ * from the current image.  resid[niter+1] as ds2d_lsrtm_cg; *learned_steps
+ * receives the number of accepted learned steps. */
+int ds2d_lsrtm_learned(ds2d_t* h, ds2d_unet_t* u, int nshots, const ds2d_shot* shots, const float* d_obs,
+                       int niter, float* image, double* resid, int* learned_steps, ds2d_stats* stats);
+/* Launch shots in groups of at most n per time loop (0 = all shots in one
+ * batched launch, the default; 1 = one shot after another as the reference's
+ * per-rank shot loop does).  Applies to the plain Born/adjoint/CG entries. */
+int ds2d_set_batch(ds2d_t* h, int shots_per_launch);
+
 int ds2d_laplace_filter(int nz, int nx, int order, float dx, float* dat);
 int ds2d_laplace_band(int nz, int nx, float* dat);
 int ds2d_ricker(int nt, float f0, float dt, float* out);
