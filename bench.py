@@ -11,7 +11,9 @@ stay in HBM (fetched once, untimed).
 
 Other workloads (--workload): c1 = single-shot elastic forward on 251x101
 (2500 steps); c2 = acoustic CG LSRTM of 26 random-boundary shots on 251x101
-(time to a relative data residual, residual history).
+(time to a relative data residual, residual history); c4 = the learned
+preconditioner: training pairs from the plain method, U-Net training, and
+plain vs learned LSRTM time to the residual target on held-out models.
 
 Prints one JSON line: value = Gcell-updates/s (one cell-update = one padded
 cell advanced one step for one wavefield); roofline of the dominant kernel
@@ -38,7 +40,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--train-models", type=int, default=64, help="c4: training models (seeds 1000..)")
+    ap.add_argument("--eval-models", type=int, default=4, help="c4: held-out models (seeds 2000..)")
+    ap.add_argument("--train-steps", type=int, default=1500, help="c4: Adam steps")
+    ap.add_argument("--max-iter", type=int, default=60, help="c4: iteration cap of each solver")
+    ap.add_argument("--target", type=float, default=1e-2, help="c4: relative data residual target")
     ap.add_argument("--nshots", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None)
     ap.add_argument("--niter", type=int, default=10, help="c2: CG iterations")
@@ -127,7 +134,7 @@ def run_forward(a, cfg):
     s.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
     s.set_graph(a.graph)
     chunk = None
-    if a.mode == 2:  # untimed: pick the fused kernel's x-chunk width for this batch
+    if a.mode == 2:  # untimed: pick the fused kernel's x-chunk width (or the two-kernel step) for this batch
         chunk, _ = s.autotune(cfg["shots"], nsteps=min(60, cfg["nt"]))
     for _ in range(a.warmup):
         s.batch_forward(cfg["shots"], mode=a.mode, fetch=False)
@@ -147,7 +154,8 @@ def run_forward(a, cfg):
         "data": f"synthetic: seeded layered+faulted elastic model (seed {cfg['seed']}), Gardner density",
         "config": {
             "workload": cfg["desc"], "shots": len(cfg["shots"]), "nt": cfg["nt"], "cell_updates_per_step": cu,
-            "kernels": KERNELS[a.mode], "parallelism": "shots batched on one GPU",
+            "kernels": KERNELS[1 if (a.mode == 2 and chunk == 0) else a.mode] +
+                       (" (autotuned)" if a.mode == 2 else ""), "parallelism": "shots batched on one GPU",
             "autotuned_chunk_width": chunk, "hipgraph": a.graph},
         "roofline": roofline(stats),
     }
@@ -197,6 +205,80 @@ def run_lsrtm(a, cfg):
     return out
 
 
+def run_learned(a):
+    """C4: learned LSRTM preconditioner.  Training pairs (RTM image -> 20-iteration
+    CG image) from the plain method on seeded 512x256 models, U-Net training
+    with the BF16 MFMA kernels, then plain CG vs learned LSRTM on held-out
+    models: wall time and iterations to the residual target."""
+    from ds2d_amd import learned as LR
+    kw = dict(nx=512, nz=256, nshots=a.nshots or 16, nt=a.nt or 1500)
+    t0 = time.perf_counter()
+    X, T, info, s = LR.generate_pairs(range(1000, 1000 + a.train_models), niter=20, **kw)
+    t_gen = time.perf_counter() - t0
+    print(f"pairs: {len(X)} in {t_gen:.1f} s", file=sys.stderr, flush=True)
+    u, loss, tst = LR.train_unet(X, T, steps=a.train_steps, batch=8, lr=1e-3, seed=0)
+    print(f"training: loss {loss[:20].mean():.4f} -> {loss[-20:].mean():.4f}, {tst['ms_per_step']:.2f} ms/step",
+          file=sys.stderr, flush=True)
+    evals = []
+    for seed in range(2000, 2000 + a.eval_models):
+        prob = LR.acoustic_problem(seed, **kw)
+        r = LR.compare(s, u, prob, a.max_iter, a.target)
+        evals.append(r)
+        print(f"seed {seed}: plain {r['plain']['resid'][-1]:.4f} ({r['plain']['iters_to_target']}) learned "
+              f"{r['learned']['resid'][-1]:.4f} ({r['learned']['iters_to_target']})", file=sys.stderr, flush=True)
+    # reference-style plain GPU LSRTM: one shot per time loop; same iterates, timed per iteration
+    prob = LR.acoustic_problem(2000, **kw)
+    s.set_model(prob["vp"], None, prob["rho"])
+    _, d, _ = s.born_forward(prob["shots"], prob["refl"])
+    s.set_batch(1)
+    t1 = time.perf_counter()
+    s.lsrtm_cg(prob["shots"], d, 2)
+    per_iter_single = (time.perf_counter() - t1) / 2
+    s.set_batch(0)
+    t1 = time.perf_counter()
+    s.lsrtm_cg(prob["shots"], d, 2)
+    per_iter_batched = (time.perf_counter() - t1) / 2
+
+    def mean_or_none(v):
+        v = [x for x in v if x is not None]
+        return float(np.mean(v)) if v else None
+
+    it_plain = [e["plain"]["iters_to_target"] for e in evals]
+    it_learn = [e["learned"]["iters_to_target"] for e in evals]
+    t_learn = [e["learned"]["time_to_target_s"] for e in evals]
+    t_plain = [e["plain"]["time_to_target_s"] for e in evals]
+    # common reachable target when 1e-2 is not reached within the cap
+    fin_p = [e["plain"]["resid"][-1] for e in evals]
+    speed = None
+    if all(k is not None for k in it_learn):
+        plain_single = [None if k is None else k * per_iter_single for k in it_plain]
+        ps = mean_or_none(plain_single)
+        lb = mean_or_none(t_learn)
+        speed = None if (ps is None or not lb) else round(ps / lb, 2)
+    out = {
+        "metric": "LSRTM time to 1e-2 data residual, learned+batched vs plain", "value": speed,
+        "unit": "x (plain one-shot-per-launch CG time / learned batched time to the target)",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 stencils, bf16 U-Net (fp32 accumulate)",
+        "data": "synthetic: seeded 512x256 layered+faulted models with lenses; Born data of the true reflectivity",
+        "config": {"workload": f"c4_learned_lsrtm: {kw['nx']}x{kw['nz']}, {kw['nshots']} shots, nt {kw['nt']}, "
+                               f"{a.train_models} training models, {a.eval_models} held-out, U-Net 4 levels x 32",
+                   "target": a.target, "max_iter": a.max_iter, "pair_generation_s": round(t_gen, 1),
+                   "train_steps": a.train_steps, "train_ms_per_step": round(tst["ms_per_step"], 3),
+                   "train_tflops": round(tst["flops_per_step"] / tst["ms_per_step"] / 1e9, 2),
+                   "train_loss_first_last": [float(loss[:20].mean()), float(loss[-20:].mean())],
+                   "iters_to_target_plain": it_plain, "iters_to_target_learned": it_learn,
+                   "time_to_target_plain_batched_s": t_plain, "time_to_target_learned_batched_s": t_learn,
+                   "final_resid_plain": fin_p, "final_resid_learned": [e["learned"]["resid"][-1] for e in evals],
+                   "image_err_plain": [e["plain"]["image_rel_err"] for e in evals],
+                   "image_err_learned": [e["learned"]["image_rel_err"] for e in evals],
+                   "per_iter_s_one_shot_per_launch": round(per_iter_single, 3),
+                   "per_iter_s_batched": round(per_iter_batched, 3),
+                   "learned_steps_accepted": [e["learned"]["learned_steps"] for e in evals]},
+    }
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -218,6 +300,8 @@ def main():
         cfg["desc"] = (f"c1_elastic_forward: 251x101, dx=dz=10 m, pml 20, dt 0.8 ms, {cfg['nt']} steps, one shot, "
                        "vx/vz at every surface node")
         out = run_forward(a, cfg)
+    elif a.workload == "c4":
+        out = run_learned(a)
     else:
         cfg = models.c2_acoustic_lsrtm(nt=a.nt or 2500, nshots=a.nshots or 26)
         cfg["seed"] = 2

@@ -12,17 +12,21 @@
 // at column j it computes the new velocity of column j and the new stress of
 // column j-4.  The eight x taps of every derivative live in per-lane
 // register queues; the eight z taps are read from a double-buffered LDS copy
-// of the current column (one barrier per column).  Every operand is loaded
-// PF columns ahead into a per-lane prefetch ring, so HBM latency overlaps
-// PF column iterations.  The state is ping-ponged (read A, write B), so the
-// four halo columns of velocity on each side are recomputed without races.
+// of the current column (one barrier per column).  The march is unrolled by
+// two with one operand set per phase: the operands of column j + 2 are
+// loaded right after column j is consumed, into the registers column j just
+// freed, so no loaded value is copied before use and HBM latency overlaps
+// two column iterations.  The eight operands every cell needs are loaded
+// unconditionally from clamped addresses (values of clamped cells are never
+// used), which keeps those loads free of branches.  The state is ping-ponged
+// (read A, write B), so the four halo columns of velocity on each side are
+// recomputed without races.
 #include "common.hpp"
 #include "kernels.hpp"
 
 namespace ds2d {
 
 namespace {
-constexpr int PF = 2;  // prefetch distance in columns
 
 __device__ __forceinline__ bool damp_cell_f(const SlotDesc& S, int ix, int iz) {
     return S.boundary == 0 && (ix < S.pml || ix > S.nxp - S.pml || iz < S.pml || iz > S.nzp - S.pml);
@@ -62,20 +66,25 @@ __global__ __launch_bounds__(FUSED_MAX_NZP) void k_fused_step(FusedArgs a, const
     auto ld = [&](const float* p, int col) -> float {
         return (zin && col >= 0 && col < nxp) ? p[base + (size_t)col * nzp] : 0.f;
     };
+    // clamped, unconditional load (the value of a clamped cell is never used)
+    const size_t basec = (size_t)S.off + (size_t)min(z, nzp - 1);
+    auto ldc = [&](const float* p, int col) -> float {
+        return p[basec + (size_t)min(max(col, 0), nxp - 1) * nzp];
+    };
     // operands of column j (velocity side) and j-4 (stress side)
     auto fetch = [&](int j) {
         Col o;
-        o.txx4 = ld(a.txx_a, j + 4);
-        o.txz4 = ld(a.txz_a, j + 4);
-        o.tzz = ld(a.tzz_a, j);
-        o.vx = ld(a.vx_a, j);
-        o.vz = ld(a.vz_a, j);
+        o.txx4 = ldc(a.txx_a, j + 4);
+        o.txz4 = ldc(a.txz_a, j + 4);
+        o.tzz = ldc(a.tzz_a, j);
+        o.vx = ldc(a.vx_a, j);
+        o.vz = ldc(a.vz_a, j);
         const bool vupd = zupd && j >= NOP && j < nxp - NOP;
-        o.bx = vupd ? ld(a.bx, j) : 0.f;
+        o.bx = ldc(a.bx, j);
         const int cs = j - 4;
         const bool supd = zupd && cs >= NOP && cs < nxp - NOP;
-        o.lam = supd ? ld(a.lam, cs) : 0.f;
-        o.mu = supd ? ld(a.mu, cs) : 0.f;
+        o.lam = ldc(a.lam, cs);
+        o.mu = ldc(a.mu, cs);
         const bool vpml = vupd && damp_cell_f(S, j, z);
         o.dxv = vpml ? ld(a.dampx, j) : 0.f;
         o.dzv = vpml ? ld(a.dampz, j) : 0.f;
@@ -111,9 +120,6 @@ __global__ __launch_bounds__(FUSED_MAX_NZP) void k_fused_step(FusedArgs a, const
     for (int k = 0; k < 9; ++k) qvx[k] = 0.f;
 #pragma unroll
     for (int k = 0; k < 8; ++k) qvz[k] = 0.f;
-    Col ring[PF];
-#pragma unroll
-    for (int k = 0; k < PF; ++k) ring[k] = fetch(j0 + k);
 
     const float w = a.wav[it];
     // coefficient of the tap at offset d = (source - cell) of a forward
@@ -123,12 +129,9 @@ __global__ __launch_bounds__(FUSED_MAX_NZP) void k_fused_step(FusedArgs a, const
                d == 0 ? -c1 : d == -1 ? -c2 : d == -2 ? -c3 : d == -3 ? -c4 : 0.f;
     };
 
-    for (int j = j0; j < x1 + 4; ++j) {
-        // ---- consume the operands of column j, prefetch column j + PF
-        const Col o = ring[0];
-#pragma unroll
-        for (int k = 0; k < PF - 1; ++k) ring[k] = ring[k + 1];
-        ring[PF - 1] = fetch(j + PF);
+    // One column: velocity of column j and stress of column j - 4, from the
+    // operands o of column j; p selects the LDS buffer of this phase.
+    auto column = [&](const int j, const Col& o, const int p) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             qxx[k] = qxx[k + 1];
@@ -143,7 +146,6 @@ __global__ __launch_bounds__(FUSED_MAX_NZP) void k_fused_step(FusedArgs a, const
         qxz[8] = o.txz4;
         qzz[4] = o.tzz;
         // ---- z taps through LDS: txz, tzz of column j; new vx, vz of column j-4
-        const int p = j & 1;
         lds[p][0][z] = qxz[4];
         lds[p][1][z] = qzz[4];
         lds[p][2][z] = qvx[4];
@@ -237,6 +239,16 @@ __global__ __launch_bounds__(FUSED_MAX_NZP) void k_fused_step(FusedArgs a, const
             a.tzz_b[c] = tzz;
             a.txz_b[c] = txz;
         }
+    };
+
+    // even number of columns: an extra trailing column stores nothing
+    const int jend = j0 + ((x1 + 4 - j0 + 1) & ~1);
+    Col cA = fetch(j0), cB = fetch(j0 + 1);
+    for (int j = j0; j < jend; j += 2) {
+        column(j, cA, 0);
+        cA = fetch(j + 2);
+        column(j + 1, cB, 1);
+        cB = fetch(j + 3);
     }
 }
 

@@ -119,13 +119,15 @@ int ds2d_take_gradients(ds2d_t* h, float* pp, float* ps, float* illum, float* b_
 /* Elastic forward modelling of the current shot: vz, vx at (rec_ix, igz). */
 int ds2d_forward(ds2d_t* h, float* rec_z, float* rec_x);
 /* Batched forward modelling of many uneven windows.  Records concatenated in
- * shot order, each [ntr_s][nt].  mode: 0 = plain per-field kernels,
- * 1 = fused fast kernel (default). */
+ * shot order, each [ntr_s][nt].  mode: 0 = reference split-field kernels,
+ * 1 = two fast kernels per step (unsplit interior), 2 = one fused kernel per
+ * step (or what ds2d_autotune_forward picked). */
 int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec_z, float* rec_x,
                        int mode, ds2d_stats* stats);
 /* x-chunk width of the fused kernel (0 = default 64), and an autotuner that
- * times nsteps of the fused kernel per candidate width on this batch and
- * keeps the fastest.  No reference counterpart (new capability). */
+ * times nsteps per candidate on this batch -- the fused kernel at each chunk
+ * width and the two-kernel step (reported as width 0) -- and keeps the
+ * fastest for mode 2.  No reference counterpart (new capability). */
 int ds2d_set_chunk(ds2d_t* h, int width);
 /* Capture each time loop into one hipGraph (1) or launch step by step (0). */
 int ds2d_set_graph(ds2d_t* h, int on);
