@@ -75,6 +75,7 @@ struct ds2d_ctx {
     hipEvent_t ev[4]{};
     ds2d_stats last{};
     int chunk_w = 0;    // fused-kernel x-chunk width (0: default)
+    int chunk_t = 0;    // fused-kernel workgroup size (256/512/768; 0: default)
     int use_graph = 0;  // capture time loops into one hipGraph per loop
     int per_launch = 0; // plain Born/adjoint: shots per time loop (0: all)
     int fwd_two = 0;    // autotuned: mode 2 runs the two-kernel step instead of the fused one
@@ -645,7 +646,7 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
         // split layout, per padded cell: v reads 10 writes 6; s reads 12 writes 9
         bytes_v = (double)B.total_cells * 16 * 4;
         bytes_s = (double)B.total_cells * 21 * 4;
-    } else if (mode == 2 && sh.nzp <= FUSED_MAX_NZP && !h->fwd_two) {
+    } else if (mode == 2 && !h->fwd_two) {
         // single fused kernel per step, ping-pong state
         float* par;
         int* rmap;
@@ -657,9 +658,12 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
         if ((rc = alloc_efield(h, "b", n, g2, true))) return rc;
         std::vector<ChunkDesc> ch;
         const int W = h->chunk_w > 0 ? h->chunk_w : 64;
+        const int T = h->chunk_t > 0 ? h->chunk_t : 256;
+        const int rows = T - 2 * FUSED_HALO;
         for (int s = 0; s < nshots; ++s)
             for (int x0 = 0; x0 < B.slots[s].nxp; x0 += W)
-                ch.push_back(ChunkDesc{s, x0, std::min(x0 + W, B.slots[s].nxp), 0});
+                for (int z0 = 0; z0 < B.slots[s].nzp; z0 += rows)
+                    ch.push_back(ChunkDesc{s, x0, std::min(x0 + W, B.slots[s].nxp), z0});
         ChunkDesc* d_ch;
         if ((rc = h->fbuf("chunks", ch.size(), &d_ch))) return rc;
         HIP_TRY(hipMemcpyAsync(d_ch, ch.data(), ch.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice, st));
@@ -684,14 +688,14 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
         if (h->use_graph) {
             // whole time loop as one graph; per-launch time = loop time / nt
             if ((rc = run_steps(h, 0, nt, 1, [&](int it) {
-                     launch_fused_step(st, sh, (it & 1) ? odd : even, d_ch, (int)ch.size(), rmap, it);
+                     launch_fused_step(st, sh, (it & 1) ? odd : even, d_ch, (int)ch.size(), rmap, it, T);
                  })))
                 return rc;
         } else {
             for (int it = 0; it < nt; ++it) {
                 const bool tk = smp.take(it);
                 if (tk) smp.mark(st, 0);
-                launch_fused_step(st, sh, (it & 1) ? odd : even, d_ch, (int)ch.size(), rmap, it);
+                launch_fused_step(st, sh, (it & 1) ? odd : even, d_ch, (int)ch.size(), rmap, it, T);
                 if (tk) { smp.mark(st, 1); smp.mark(st, 2); }
             }
         }
@@ -765,11 +769,14 @@ int ds2d_set_graph(ds2d_t* h, int on) {
     return 0;
 }
 
-int ds2d_set_chunk(ds2d_t* h, int width) {
+int ds2d_set_chunk(ds2d_t* h, int width, int threads) {
     int rc;
     if ((rc = check_h(h))) return rc;
     if (width < 0 || (width > 0 && width < 8)) return fail(DS2D_EINVAL, "chunk width must be 0 or >= 8");
+    if (threads != 0 && !fused_threads_ok(threads)) return fail(DS2D_EINVAL, "threads must be 0, 256, 512 or 768");
     h->chunk_w = width;
+    h->chunk_t = threads;
+    h->fwd_two = 0;
     return 0;
 }
 
@@ -777,28 +784,39 @@ int ds2d_set_chunk(ds2d_t* h, int width) {
 // batch, and of the two-kernel step (width 0), each after one untimed warm
 // run, and keep the fastest for mode 2.
 int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nsteps, int* best_width,
-                          double* best_ms) {
+                          int* best_threads, double* best_ms) {
     int rc;
     if ((rc = check_h(h))) return rc;
-    static const int cand[] = {0, 16, 24, 32, 48, 64, 96, 128, 192};
-    int bw = 64;
+    struct Cand {
+        int w, t;
+    };
+    static const Cand cand[] = {{0, 0},    {32, 256}, {64, 256}, {128, 256}, {32, 512},
+                                {64, 512}, {128, 512}, {64, 768}, {128, 768}};
+    Cand best{64, 256};
     double bt = 1e300;
-    const int keep = h->chunk_w, keep_two = h->fwd_two;
-    for (int w : cand) {
-        h->fwd_two = w == 0;
-        h->chunk_w = w;
+    const int keep_w = h->chunk_w, keep_t = h->chunk_t, keep_two = h->fwd_two;
+    for (const Cand& c : cand) {
+        h->fwd_two = c.w == 0;
+        h->chunk_w = c.w;
+        h->chunk_t = c.t;
         ds2d_stats s{};
         for (int rep = 0; rep < 2; ++rep)
             if ((rc = batch_forward_n(h, nshots, shots, nullptr, nullptr, 2, &s, nsteps))) {
-                h->chunk_w = keep;
+                h->chunk_w = keep_w;
+                h->chunk_t = keep_t;
                 h->fwd_two = keep_two;
                 return rc;
             }
-        if (s.stencil_ms < bt) { bt = s.stencil_ms; bw = w; }
+        if (s.stencil_ms < bt) {
+            bt = s.stencil_ms;
+            best = c;
+        }
     }
-    h->fwd_two = bw == 0;
-    h->chunk_w = bw;
-    if (best_width) *best_width = bw;
+    h->fwd_two = best.w == 0;
+    h->chunk_w = best.w;
+    h->chunk_t = best.t;
+    if (best_width) *best_width = best.w;
+    if (best_threads) *best_threads = best.t;
     if (best_ms) *best_ms = bt;
     return 0;
 }

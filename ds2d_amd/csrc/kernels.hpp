@@ -104,7 +104,7 @@ void launch_fast_rmap(hipStream_t st, const BatchShape& b, int* rmap);
 
 // --- fused single-kernel elastic step (kernels_fused.hip) -------------------
 struct ChunkDesc {
-    int slot, x0, x1, pad;  // columns [x0, x1) of slot
+    int slot, x0, x1, z0;  // columns [x0, x1) of slot, rows from z0 (block of threads - 2*FUSED_HALO rows)
 };
 struct FusedArgs {
     // state read (a) and written (b); split memory used in the damping strips only
@@ -117,10 +117,12 @@ struct FusedArgs {
     const float* wav;
     float *rec_z, *rec_x;
 };
-// Largest padded depth one workgroup column can hold (one z row per lane);
-// 768 lanes leave 170 VGPRs per lane for the tap queues and prefetch ring.
-constexpr int FUSED_MAX_NZP = 768;
+// A fused workgroup is `threads` lanes (256, 512 or 768), one z row per lane:
+// FUSED_HALO rows above and below its block are loaded to feed the z taps,
+// so it owns threads - 2*FUSED_HALO rows.
+constexpr int FUSED_HALO = 8;
+bool fused_threads_ok(int threads);
 void launch_fused_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, const ChunkDesc* chunks,
-                       int nchunks, const int* rmap, int it);
+                       int nchunks, const int* rmap, int it, int threads);
 
 }  // namespace ds2d

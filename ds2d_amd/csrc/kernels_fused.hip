@@ -6,9 +6,13 @@
 // recorder of FD2DGPU.cu:355-365; split PML memory only in the damping
 // strips (elsewhere the split update equals the unsplit one).
 //
-// Layout of the work: one workgroup owns columns [x0, x1) of one shot window
-// and the full padded depth (one z row per lane, rows contiguous in memory
-// so every column load is a coalesced 64-lane row).  It marches along x:
+// Layout of the work: one workgroup owns columns [x0, x1) and a block of
+// rows of one shot window, one z row per lane (rows contiguous in memory, so
+// every column load is a coalesced 64-lane row).  Eight halo rows above and
+// below the block are loaded too: velocity is computed on the block plus
+// four rows each side, which is what the stress z taps of the block read.
+// Several small workgroups per CU keep loads in flight while others wait at
+// their barrier.  The workgroup marches along x:
 // at column j it computes the new velocity of column j and the new stress of
 // column j-4.  The eight x taps of every derivative live in per-lane
 // register queues; the eight z taps are read from a double-buffered LDS copy
@@ -44,10 +48,11 @@ struct Col {
 };
 }  // namespace
 
-__global__ __launch_bounds__(FUSED_MAX_NZP) void k_fused_step(FusedArgs a, const ChunkDesc* __restrict__ chunks,
-                                                     int nchunks, int per_xcd, const int* __restrict__ rmap,
-                                                     const SlotDesc* __restrict__ slots, float idx, float idz,
-                                                     float dt, int nt, int it) {
+template <int T>
+__global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* __restrict__ chunks, int nchunks,
+                                                  int per_xcd, const int* __restrict__ rmap,
+                                                  const SlotDesc* __restrict__ slots, float idx, float idz, float dt,
+                                                  int nt, int it) {
     // XCD-aware chunk order: consecutive workgroups land on different XCDs,
     // so give each XCD a contiguous run of chunks (neighbours share halos in L2).
     const int b = blockIdx.x;
@@ -55,13 +60,16 @@ __global__ __launch_bounds__(FUSED_MAX_NZP) void k_fused_step(FusedArgs a, const
     if (ch >= nchunks) return;
     const ChunkDesc C = chunks[ch];
     const SlotDesc S = slots[C.slot];
-    const int z = threadIdx.x;
+    const int L = threadIdx.x;                 // lane = LDS index
+    const int z = C.z0 - FUSED_HALO + L;       // padded row of this lane
     const int nzp = S.nzp, nxp = S.nxp;
-    const bool zin = z < nzp;
+    const bool zin = z >= 0 && z < nzp;
+    const bool zown = zin && L >= FUSED_HALO && L < T - FUSED_HALO;  // rows this workgroup stores
+    const bool vlane = L >= 4 && L < T - 4;    // velocity needed (z taps of owned stress rows)
     const bool zupd = z >= NOP && z < nzp - NOP;
     const size_t base = (size_t)S.off + z;
 
-    __shared__ float lds[2][4][FUSED_MAX_NZP];
+    __shared__ float lds[2][4][T];
 
     auto ld = [&](const float* p, int col) -> float {
         return (zin && col >= 0 && col < nxp) ? p[base + (size_t)col * nzp] : 0.f;
@@ -146,22 +154,22 @@ __global__ __launch_bounds__(FUSED_MAX_NZP) void k_fused_step(FusedArgs a, const
         qxz[8] = o.txz4;
         qzz[4] = o.tzz;
         // ---- z taps through LDS: txz, tzz of column j; new vx, vz of column j-4
-        lds[p][0][z] = qxz[4];
-        lds[p][1][z] = qzz[4];
-        lds[p][2][z] = qvx[4];
-        lds[p][3][z] = qvz[3];
+        lds[p][0][L] = qxz[4];
+        lds[p][1][L] = qzz[4];
+        lds[p][2][L] = qvx[4];
+        lds[p][3][L] = qvz[3];
         __syncthreads();
-        // ---- velocity of column j (halo columns included)
+        // ---- velocity of column j (halo columns and rows included)
         float nvx = o.vx, nvz = o.vz;
-        if (zupd && j >= NOP && j < nxp - NOP) {
+        if (vlane && zupd && j >= NOP && j < nxp - NOP) {
             const float* L0 = lds[p][0];
             const float* L1 = lds[p][1];
             float ax = c1 * (qxx[5] - qxx[4]) + c2 * (qxx[6] - qxx[3]) + c3 * (qxx[7] - qxx[2]) + c4 * (qxx[8] - qxx[1]);
             float ex = c1 * (qxz[4] - qxz[3]) + c2 * (qxz[5] - qxz[2]) + c3 * (qxz[6] - qxz[1]) + c4 * (qxz[7] - qxz[0]);
-            float bz = c1 * (L0[z] - L0[z - 1]) + c2 * (L0[z + 1] - L0[z - 2]) + c3 * (L0[z + 2] - L0[z - 3]) +
-                       c4 * (L0[z + 3] - L0[z - 4]);
-            float cz = c1 * (L1[z + 1] - L1[z]) + c2 * (L1[z + 2] - L1[z - 1]) + c3 * (L1[z + 3] - L1[z - 2]) +
-                       c4 * (L1[z + 4] - L1[z - 3]);
+            float bz = c1 * (L0[L] - L0[L - 1]) + c2 * (L0[L + 1] - L0[L - 2]) + c3 * (L0[L + 2] - L0[L - 3]) +
+                       c4 * (L0[L + 3] - L0[L - 4]);
+            float cz = c1 * (L1[L + 1] - L1[L]) + c2 * (L1[L + 2] - L1[L - 1]) + c3 * (L1[L + 3] - L1[L - 2]) +
+                       c4 * (L1[L + 4] - L1[L - 3]);
             if (z == S.isz) ax += w * tap(S.isx - j);
             if (j == S.isx) cz += w * tap(S.isz - z);
             const float bx = o.bx;
@@ -172,7 +180,7 @@ __global__ __launch_bounds__(FUSED_MAX_NZP) void k_fused_step(FusedArgs a, const
                 const float vxz = (d3 * o.vxz + bx * idz * bz) / d4;
                 const float vzx = (d1 * o.vzx + bx * idx * ex) / d2;
                 const float vzz = (d3 * o.vzz + bx * idz * cz) / d4;
-                if (j >= x0 && j < x1) {
+                if (zown && j >= x0 && j < x1) {
                     const size_t c = base + (size_t)j * nzp;
                     a.vx_x_b[c] = vxx; a.vx_z_b[c] = vxz; a.vz_x_b[c] = vzx; a.vz_z_b[c] = vzz;
                 }
@@ -185,7 +193,7 @@ __global__ __launch_bounds__(FUSED_MAX_NZP) void k_fused_step(FusedArgs a, const
         }
         qvx[8] = nvx;
         qvz[7] = nvz;
-        if (zin && j >= x0 && j < x1) {
+        if (zown && j >= x0 && j < x1) {
             const size_t c = base + (size_t)j * nzp;
             a.vx_b[c] = nvx;
             a.vz_b[c] = nvz;
@@ -200,7 +208,7 @@ __global__ __launch_bounds__(FUSED_MAX_NZP) void k_fused_step(FusedArgs a, const
         }
         // ---- stress of column cs = j - 4 (owned columns only)
         const int cs = j - 4;
-        if (zin && cs >= x0 && cs < x1) {
+        if (zown && cs >= x0 && cs < x1) {
             const size_t c = base + (size_t)cs * nzp;
             float txx = qxx[0], tzz = qzz[0], txz = qxz[0];
             if (zupd && cs >= NOP && cs < nxp - NOP) {
@@ -210,10 +218,10 @@ __global__ __launch_bounds__(FUSED_MAX_NZP) void k_fused_step(FusedArgs a, const
                                    c4 * (qvx[7] - qvx[0])) * idx;
                 const float dvzx = (c1 * (qvz[4] - qvz[3]) + c2 * (qvz[5] - qvz[2]) + c3 * (qvz[6] - qvz[1]) +
                                     c4 * (qvz[7] - qvz[0])) * idx;
-                const float dvz = (c1 * (L3[z] - L3[z - 1]) + c2 * (L3[z + 1] - L3[z - 2]) +
-                                   c3 * (L3[z + 2] - L3[z - 3]) + c4 * (L3[z + 3] - L3[z - 4])) * idz;
-                const float dvxz = (c1 * (L2[z + 1] - L2[z]) + c2 * (L2[z + 2] - L2[z - 1]) +
-                                    c3 * (L2[z + 3] - L2[z - 2]) + c4 * (L2[z + 4] - L2[z - 3])) * idz;
+                const float dvz = (c1 * (L3[L] - L3[L - 1]) + c2 * (L3[L + 1] - L3[L - 2]) +
+                                   c3 * (L3[L + 2] - L3[L - 3]) + c4 * (L3[L + 3] - L3[L - 4])) * idz;
+                const float dvxz = (c1 * (L2[L + 1] - L2[L]) + c2 * (L2[L + 2] - L2[L - 1]) +
+                                    c3 * (L2[L + 3] - L2[L - 2]) + c4 * (L2[L + 4] - L2[L - 3])) * idz;
                 const float l = o.lam, mu = o.mu, l2m = l + 2 * mu;
                 if (damp_cell_f(S, cs, z)) {
                     const float d1 = 1.f - dt * o.dxs / 2, d2 = 1.f + dt * o.dxs / 2;
@@ -252,12 +260,22 @@ __global__ __launch_bounds__(FUSED_MAX_NZP) void k_fused_step(FusedArgs a, const
     }
 }
 
+bool fused_threads_ok(int threads) { return threads == 256 || threads == 512 || threads == 768; }
+
 void launch_fused_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, const ChunkDesc* chunks,
-                       int nchunks, const int* rmap, int it) {
+                       int nchunks, const int* rmap, int it, int threads) {
     const int per_xcd = (nchunks + 7) / 8;
-    const int threads = ((b.nzp + 63) / 64) * 64;
-    hipLaunchKernelGGL(k_fused_step, dim3(per_xcd * 8), dim3(threads), 0, st, a, chunks, nchunks, per_xcd, rmap,
-                       b.slots, 1.f / b.dx, 1.f / b.dz, b.dt, b.nt, it);
+    const dim3 grid(per_xcd * 8);
+    const float idx = 1.f / b.dx, idz = 1.f / b.dz;
+    if (threads == 256)
+        hipLaunchKernelGGL(k_fused_step<256>, grid, dim3(256), 0, st, a, chunks, nchunks, per_xcd, rmap, b.slots, idx,
+                           idz, b.dt, b.nt, it);
+    else if (threads == 512)
+        hipLaunchKernelGGL(k_fused_step<512>, grid, dim3(512), 0, st, a, chunks, nchunks, per_xcd, rmap, b.slots, idx,
+                           idz, b.dt, b.nt, it);
+    else
+        hipLaunchKernelGGL(k_fused_step<768>, grid, dim3(768), 0, st, a, chunks, nchunks, per_xcd, rmap, b.slots, idx,
+                           idz, b.dt, b.nt, it);
 }
 
 }  // namespace ds2d
