@@ -75,7 +75,7 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
         return (zin && col >= 0 && col < nxp) ? p[base + (size_t)col * nzp] : 0.f;
     };
     // clamped, unconditional load (the value of a clamped cell is never used)
-    const size_t basec = (size_t)S.off + (size_t)min(z, nzp - 1);
+    const size_t basec = (size_t)S.off + (size_t)min(max(z, 0), nzp - 1);
     auto ldc = [&](const float* p, int col) -> float {
         return p[basec + (size_t)min(max(col, 0), nxp - 1) * nzp];
     };
