@@ -646,7 +646,7 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
         // split layout, per padded cell: v reads 10 writes 6; s reads 12 writes 9
         bytes_v = (double)B.total_cells * 16 * 4;
         bytes_s = (double)B.total_cells * 21 * 4;
-    } else if (mode == 2 && !h->fwd_two) {
+    } else if (mode == 2 && !h->fwd_two && n <= FUSED_MAX_CELLS) {
         // single fused kernel per step, ping-pong state
         float* par;
         int* rmap;
@@ -669,18 +669,15 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
         HIP_TRY(hipMemcpyAsync(d_ch, ch.data(), ch.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice, st));
         auto mk = [&](const EField& A, const EField& Bf) {
             FusedArgs q{};
-            q.vx_a = A.vx; q.vz_a = A.vz; q.txx_a = A.txx; q.tzz_a = A.tzz; q.txz_a = A.txz;
-            q.vx_x_a = A.vx_x; q.vx_z_a = A.vx_z; q.vz_x_a = A.vz_x; q.vz_z_a = A.vz_z;
-            q.txx_x_a = A.txx_x; q.txx_z_a = A.txx_z; q.tzz_x_a = A.tzz_x; q.tzz_z_a = A.tzz_z;
-            q.txz_x_a = A.txz_x; q.txz_z_a = A.txz_z;
-            q.vx_b = Bf.vx; q.vz_b = Bf.vz; q.txx_b = Bf.txx; q.tzz_b = Bf.tzz; q.txz_b = Bf.txz;
-            q.vx_x_b = Bf.vx_x; q.vx_z_b = Bf.vx_z; q.vz_x_b = Bf.vz_x; q.vz_z_b = Bf.vz_z;
-            q.txx_x_b = Bf.txx_x; q.txx_z_b = Bf.txx_z; q.tzz_x_b = Bf.tzz_x; q.tzz_z_b = Bf.tzz_z;
-            q.txz_x_b = Bf.txz_x; q.txz_z_b = Bf.txz_z;
-            q.bx = par; q.lam = par + n; q.mu = par + 2 * n;
-            q.dampx = m.dampx; q.dampz = m.dampz;
+            q.A = A.vx;  // alloc_efield: 15 fields of n floats from vx on
+            q.B = Bf.vx;
+            q.n = n;
+            q.par = par;
+            q.dampx = m.dampx;
+            q.dampz = m.dampz;
             q.wav = h->d_wav;
-            q.rec_z = rz; q.rec_x = rx;
+            q.rec_z = rz;
+            q.rec_x = rx;
             return q;
         };
         const FusedArgs even = mk(f, g2), odd = mk(g2, f);

@@ -107,16 +107,19 @@ struct ChunkDesc {
     int slot, x0, x1, z0;  // columns [x0, x1) of slot, rows from z0 (block of threads - 2*FUSED_HALO rows)
 };
 struct FusedArgs {
-    // state read (a) and written (b); split memory used in the damping strips only
-    const float *vx_a, *vz_a, *txx_a, *tzz_a, *txz_a;
-    const float *vx_x_a, *vx_z_a, *vz_x_a, *vz_z_a, *txx_x_a, *txx_z_a, *tzz_x_a, *tzz_z_a, *txz_x_a, *txz_z_a;
-    float *vx_b, *vz_b, *txx_b, *tzz_b, *txz_b;
-    float *vx_x_b, *vx_z_b, *vz_x_b, *vz_z_b, *txx_x_b, *txx_z_b, *tzz_x_b, *tzz_z_b, *txz_x_b, *txz_z_b;
-    const float *bx, *lam, *mu;  // dt/rho, lambda*dt, mu*dt
+    // state read (A) and written (B): 15 fields of n floats each, in EField
+    // order vx vz txx tzz txz | vx_x vx_z vz_x vz_z txx_x txx_z tzz_x tzz_z
+    // txz_x txz_z (the split fields are used in the damping strips only)
+    const float* A;
+    float* B;
+    long long n;
+    const float* par;  // dt/rho | lambda*dt | mu*dt, n floats each
     const float *dampx, *dampz;
     const float* wav;
     float *rec_z, *rec_x;
 };
+// Fused addressing uses 32-bit byte offsets within each field.
+constexpr long long FUSED_MAX_CELLS = (1LL << 30) - (1LL << 20);
 // A fused workgroup is `threads` lanes (256, 512 or 768), one z row per lane:
 // FUSED_HALO rows above and below its block are loaded to feed the z taps,
 // so it owns threads - 2*FUSED_HALO rows.
