@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--train-steps", type=int, default=1500, help="c4: Adam steps")
     ap.add_argument("--max-iter", type=int, default=60, help="c4: iteration cap of each solver")
     ap.add_argument("--target", type=float, default=1e-2, help="c4: relative data residual target")
+    ap.add_argument("--save-unet", default=None, help="c4: write the trained U-Net parameters (.npy)")
+    ap.add_argument("--load-unet", default=None, help="c4: skip pair generation and training, load parameters")
     ap.add_argument("--nshots", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None)
     ap.add_argument("--niter", type=int, default=10, help="c2: CG iterations")
@@ -214,16 +216,27 @@ def run_learned(a):
     from ds2d_amd import learned as LR
     kw = dict(nx=512, nz=256, nshots=a.nshots or 16, nt=a.nt or 1500)
     t0 = time.perf_counter()
-    X, T, info, s = LR.generate_pairs(range(1000, 1000 + a.train_models), niter=20, **kw)
-    t_gen = time.perf_counter() - t0
-    print(f"pairs: {len(X)} in {t_gen:.1f} s", file=sys.stderr, flush=True)
-    u, loss, tst = LR.train_unet(X, T, steps=a.train_steps, batch=8, lr=1e-3, seed=0)
-    print(f"training: loss {loss[:20].mean():.4f} -> {loss[-20:].mean():.4f}, {tst['ms_per_step']:.2f} ms/step",
-          file=sys.stderr, flush=True)
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    if a.load_unet:
+        from ds2d_amd.lib import Unet
+        s = LR.make_solver(LR.acoustic_problem(1000, **kw))
+        u = Unet(levels=4, base=32, max_batch=8, max_h=kw["nx"], max_w=kw["nz"], seed=0)
+        u.set_params(np.load(a.load_unet))
+        t_gen, loss, tst = 0.0, np.zeros(20, np.float32), {"ms_per_step": 0.0, "flops_per_step": 0.0}
+    else:
+        X, T, info, s = LR.generate_pairs(range(1000, 1000 + a.train_models), niter=20, log=log, **kw)
+        t_gen = time.perf_counter() - t0
+        log(f"pairs: {len(X)} in {t_gen:.1f} s")
+        u, loss, tst = LR.train_unet(X, T, steps=a.train_steps, batch=8, lr=1e-3, seed=0)
+        log(f"training: loss {loss[:20].mean():.4f} -> {loss[-20:].mean():.4f}, {tst['ms_per_step']:.2f} ms/step")
+        if a.save_unet:
+            np.save(a.save_unet, u.get_params())
     evals = []
     for seed in range(2000, 2000 + a.eval_models):
         prob = LR.acoustic_problem(seed, **kw)
-        r = LR.compare(s, u, prob, a.max_iter, a.target)
+        r = LR.compare(s, u, prob, a.max_iter, a.target, log=log)
         evals.append(r)
         print(f"seed {seed}: plain {r['plain']['resid'][-1]:.4f} ({r['plain']['iters_to_target']}) learned "
               f"{r['learned']['resid'][-1]:.4f} ({r['learned']['iters_to_target']})", file=sys.stderr, flush=True)
@@ -266,7 +279,8 @@ def run_learned(a):
                                f"{a.train_models} training models, {a.eval_models} held-out, U-Net 4 levels x 32",
                    "target": a.target, "max_iter": a.max_iter, "pair_generation_s": round(t_gen, 1),
                    "train_steps": a.train_steps, "train_ms_per_step": round(tst["ms_per_step"], 3),
-                   "train_tflops": round(tst["flops_per_step"] / tst["ms_per_step"] / 1e9, 2),
+                   "train_tflops": round(tst["flops_per_step"] / tst["ms_per_step"] / 1e9, 2)
+                   if tst["ms_per_step"] else None,
                    "train_loss_first_last": [float(loss[:20].mean()), float(loss[-20:].mean())],
                    "iters_to_target_plain": it_plain, "iters_to_target_learned": it_learn,
                    "time_to_target_plain_batched_s": t_plain, "time_to_target_learned_batched_s": t_learn,

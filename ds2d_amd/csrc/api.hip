@@ -625,6 +625,12 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
     launch_setup(st, sh, setup_args(h, false), m);
     HIP_TRY(hipMemsetAsync(f.vx, 0, (size_t)n * 15 * 4, st));
     double bytes_v = 0, bytes_s = 0;
+    // cells of the update region that carry split PML memory (PML windows only)
+    double pml_cells = 0;
+    for (auto& S : B.slots)
+        if (S.boundary == 0 && S.pml >= NOP)
+            pml_cells += (double)(S.nxp - 2 * NOP) * (S.nzp - 2 * NOP) -
+                         (double)(S.nxp - 2 * S.pml + 1) * (S.nzp - 2 * S.pml + 1);
     long long launches = 0;
     StepSampler smp;
     if ((rc = smp.init(nt))) return rc;
@@ -698,8 +704,9 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
         }
         stats_loop_end(h);
         launches = (long long)nt;
-        // unsplit interior, per padded cell: reads stress 3 + velocity 2 + dt/rho, lambda, mu; writes 5
-        bytes_v = (double)B.total_cells * 13 * 4;
+        // unsplit interior, per padded cell: reads stress 3 + velocity 2 + dt/rho, lambda, mu; writes 5.
+        // Each PML cell also reads dampx, dampz and reads + writes 4 velocity and 6 stress splits.
+        bytes_v = (double)B.total_cells * 13 * 4 + pml_cells * 22 * 4;
         bytes_s = 0;
     } else {
         float* par;
@@ -728,9 +735,10 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
         }
         stats_loop_end(h);
         launches = (long long)nt;
-        // unsplit layout, per padded cell: v reads 6 writes 2; s reads 7 writes 3
-        bytes_v = (double)B.total_cells * 8 * 4;
-        bytes_s = (double)B.total_cells * 10 * 4;
+        // unsplit layout, per padded cell: v reads 6 writes 2; s reads 7 writes 3.
+        // PML cells: v also reads dampx, dampz, 4 splits and writes 4; s reads dampx, dampz, 6 and writes 6.
+        bytes_v = (double)B.total_cells * 8 * 4 + pml_cells * 10 * 4;
+        bytes_s = (double)B.total_cells * 10 * 4 + pml_cells * 14 * 4;
     }
     if ((rc = stats_end(h, stats, cu, launches, 0))) return rc;
     {

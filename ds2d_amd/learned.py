@@ -40,12 +40,13 @@ def _rms(a):
     return r if r > 0 else 1.0
 
 
-def generate_pairs(seeds, niter=20, solver=None, **kw):
+def generate_pairs(seeds, niter=20, solver=None, log=None, **kw):
     """(X, T, info): RTM images and `niter`-iteration CG images, RMS-normalised,
-    shape [len(seeds)][nx][nz]."""
+    shape [len(seeds)][nx][nz].  log: optional callable for progress lines."""
     X, T, info = [], [], []
     s = solver
-    for seed in seeds:
+    seeds = list(seeds)
+    for i, seed in enumerate(seeds):
         prob = acoustic_problem(seed, **kw)
         if s is None:
             s = make_solver(prob)
@@ -56,6 +57,8 @@ def generate_pairs(seeds, niter=20, solver=None, **kw):
         X.append(g0 / _rms(g0))
         T.append(img / _rms(img))
         info.append(dict(seed=seed, resid=res.tolist()))
+        if log:
+            log(f"pair {i + 1}/{len(seeds)} (seed {seed}): {niter}-iteration residual {res[-1]:.4f}")
     return np.asarray(X, np.float32), np.asarray(T, np.float32), info, s
 
 
@@ -74,7 +77,7 @@ def first_hit(res, target=1e-2):
     return k[0] if k else None
 
 
-def compare(solver, unet, prob, niter, target=1e-2):
+def compare(solver, unet, prob, niter, target=1e-2, log=None):
     """Plain CG vs learned LSRTM on one problem: residual histories, wall
     times, iterations to `target`, image error against the true reflectivity."""
     solver.set_model(prob["vp"], None, prob["rho"])
@@ -91,6 +94,8 @@ def compare(solver, unet, prob, niter, target=1e-2):
         wall = time.perf_counter() - t0
         err = float(np.linalg.norm(img - prob["refl"]) / np.linalg.norm(prob["refl"]))
         k = first_hit(res, target)
+        if log:
+            log(f"seed {prob['seed']} {name}: residual {res[-1]:.4f} after {niter} iterations, {wall:.1f} s")
         out[name] = dict(resid=[float(r) for r in res], wall_s=wall, iters_to_target=k,
                          time_to_target_s=None if k is None else wall * k / max(niter, 1), image_rel_err=err,
                          learned_steps=acc)
