@@ -887,6 +887,7 @@ namespace {
 
 struct AcouWS {
     AcouField bg, bn, lm;
+    AcouField bn2, lm2;         // second state of the fused (ping-pong) Born / adjoint steps
     AcouMed m;
     float *grad_pad = nullptr;  // per-slot padded gradient (n)
     float *win = nullptr;       // per-slot window images
@@ -894,8 +895,14 @@ struct AcouWS {
     float *rec2 = nullptr;      // R floats: born records / residual
     float *hist = nullptr;      // resident background increments (store mode)
     bool store = false;
+    bool fused = false;         // store mode with the fused one-launch-per-step kernels
+    int* rmap = nullptr;        // column -> trace of every slot (-1: none)
+    ChunkDesc* chunks = nullptr;
+    std::vector<std::pair<int, int>> gchunks;  // per launch group: first chunk, count
     long long n = 0, R = 0, W = 0;
 };
+
+constexpr int ACOU_CHUNK_W = 64, ACOU_THREADS = 256;
 
 int acou_prepare(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouWS& w) {
     int rc;
@@ -904,12 +911,14 @@ int acou_prepare(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouWS
     w.R = std::max<long long>(B.total_rec, 1);
     w.W = std::max<long long>(B.total_win, 1);
     float* base;
-    if ((rc = h->fbuf("acou", (size_t)n * 13, &base))) return rc;
+    if ((rc = h->fbuf("acou", (size_t)n * 19, &base))) return rc;
     w.bg = AcouField{base, base + n, base + 2 * n};
     w.bn = AcouField{base + 3 * n, base + 4 * n, base + 5 * n};
     w.lm = AcouField{base + 6 * n, base + 7 * n, base + 8 * n};
     w.grad_pad = base + 9 * n;
     float *bu = base + 10 * n, *kk = base + 11 * n, *refl = base + 12 * n;
+    w.bn2 = AcouField{base + 13 * n, base + 14 * n, base + 15 * n};
+    w.lm2 = AcouField{base + 16 * n, base + 17 * n, base + 18 * n};
     Med m;
     if ((rc = alloc_med(h, n, m))) return rc;
     SetupArgs a = setup_args(h, false);
@@ -924,7 +933,43 @@ int acou_prepare(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouWS
     const double budget = h->g.hbm_budget_gb > 0 ? h->g.hbm_budget_gb : 64.0;
     w.store = h->g.bg_mode == 1 || (h->g.bg_mode == 0 && hist_gb <= budget);
     if (w.store && (rc = h->fbuf("hist", (size_t)B.total_hist, &w.hist))) return rc;
+    // fused Born / adjoint steps (store mode; 32-bit byte offsets within a field)
+    w.fused = w.store && n <= FUSED_MAX_CELLS;
+    if (w.fused) {
+        if ((rc = h->fbuf("armap", std::max<long long>(B.total_rmap, 1), &w.rmap))) return rc;
+        HIP_TRY(hipMemsetAsync(w.rmap, 0xff, B.total_rmap * 4, h->st));
+        launch_fast_rmap(h->st, B.shape, w.rmap);
+        std::vector<ChunkDesc> ch;
+        const int ns = (int)B.slots.size();
+        const int gs = h->per_launch > 0 ? std::min(h->per_launch, ns) : ns;
+        const int rows = ACOU_THREADS - 16;
+        w.gchunks.clear();
+        for (int s0 = 0; s0 < ns; s0 += gs) {
+            const int first = (int)ch.size();
+            for (int s = s0; s < std::min(ns, s0 + gs); ++s)
+                for (int x0 = 0; x0 < B.slots[s].nxp; x0 += ACOU_CHUNK_W)
+                    for (int z0 = 0; z0 < B.slots[s].nzp; z0 += rows)
+                        ch.push_back(ChunkDesc{s, x0, std::min(x0 + ACOU_CHUNK_W, B.slots[s].nxp), z0});
+            w.gchunks.push_back({first, (int)ch.size() - first});
+        }
+        if ((rc = h->fbuf("achunks", ch.size(), &w.chunks))) return rc;
+        HIP_TRY(hipMemcpyAsync(w.chunks, ch.data(), ch.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice, h->st));
+    }
     return 0;
+}
+
+AcouFusedArgs acou_fused_args(ds2d_ctx* h, AcouWS& w, const AcouField& A, const AcouField& Bf) {
+    AcouFusedArgs q{};
+    q.A = A.vx;
+    q.B = Bf.vx;
+    q.n = w.n;
+    q.b = w.m.b;
+    q.k = w.m.k;
+    q.m = w.m.m;
+    q.hist = w.hist;
+    q.ax = h->g.dt / h->g.dx;
+    q.az = h->g.dt / h->g.dz;
+    return q;
 }
 
 // Re-window a new reflectivity into the existing workspace (zero frame).
@@ -967,6 +1012,17 @@ std::vector<BatchShape> launch_groups(const ds2d_ctx* h, const Batch& B) {
 void acou_fwd(ds2d_ctx* h, const Batch& B, AcouWS& w, bool do_bg, bool do_born, bool rec_bg) {
     hipStream_t st = h->st;
     (void)hipMemsetAsync(w.bg.vx, 0, (size_t)w.n * 6 * 4, st);  // bg + bn
+    if (w.fused && !do_bg && do_born && !rec_bg) {
+        // Born field only, from the resident history: one fused launch per step
+        AcouFusedArgs even = acou_fused_args(h, w, w.bn, w.bn2), odd = acou_fused_args(h, w, w.bn2, w.bn);
+        even.rec = odd.rec = w.rec2;
+        const auto groups = launch_groups(h, B);
+        for (size_t gi = 0; gi < groups.size(); ++gi)
+            for (int it = 0; it < h->g.nt; ++it)
+                launch_acou_born_fused(st, groups[gi], (it & 1) ? odd : even, w.chunks + w.gchunks[gi].first,
+                                       w.gchunks[gi].second, w.rmap, it, ACOU_THREADS);
+        return;
+    }
     const int hmode = do_bg ? (w.store ? 1 : 0) : 2;
     for (const BatchShape& sh : launch_groups(h, B))
         for (int it = 0; it < h->g.nt; ++it) {
@@ -983,6 +1039,17 @@ void acou_bwd(ds2d_ctx* h, const Batch& B, AcouWS& w) {
     hipStream_t st = h->st;
     (void)hipMemsetAsync(w.lm.vx, 0, (size_t)w.n * 3 * 4, st);
     (void)hipMemsetAsync(w.grad_pad, 0, (size_t)w.n * 4, st);
+    if (w.fused) {
+        AcouFusedArgs even = acou_fused_args(h, w, w.lm, w.lm2), odd = acou_fused_args(h, w, w.lm2, w.lm);
+        even.grad = odd.grad = w.grad_pad;
+        even.res = odd.res = w.rec2;
+        const auto groups = launch_groups(h, B);
+        for (size_t gi = 0; gi < groups.size(); ++gi)
+            for (int it = h->g.nt - 1, k = 0; it >= 0; --it, ++k)
+                launch_acou_adj_fused(st, groups[gi], (k & 1) ? odd : even, w.chunks + w.gchunks[gi].first,
+                                      w.gchunks[gi].second, w.rmap, it, ACOU_THREADS);
+        return;
+    }
     for (const BatchShape& sh : launch_groups(h, B))
         for (int it = h->g.nt - 1; it >= 0; --it) {
             launch_acou_adj_p(st, sh, w.bg, w.lm, w.m, w.grad_pad, w.store ? w.hist : nullptr, w.rec2, it);

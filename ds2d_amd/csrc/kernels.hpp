@@ -82,6 +82,25 @@ void launch_acou_adj_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouFi
                        const float* hist, const float* res, int it);
 void launch_acou_adj_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m,
                        const float* wav, int it, int do_bg);
+struct ChunkDesc;
+// Fused plain acoustic steps (kernels_acoustic_fused.hip): state A -> B, each
+// 3 fields (vx, vz, p) of n floats from the base pointer.
+struct AcouFusedArgs {
+    const float* A;
+    float* B;
+    long long n;
+    const float *b, *k, *m;  // buoyancy, modulus, reflectivity window
+    const float* hist;       // resident background increments
+    float* rec;              // Born step: records of p (or null)
+    float* grad;             // adjoint step: padded gradient (read-modify-write)
+    const float* res;        // adjoint step: residual records
+    float ax, az;            // dt/dx, dt/dz
+};
+bool acou_fused_threads_ok(int threads);
+void launch_acou_born_fused(hipStream_t st, const BatchShape& b, const AcouFusedArgs& a, const ChunkDesc* chunks,
+                            int nchunks, const int* rmap, int it, int threads);
+void launch_acou_adj_fused(hipStream_t st, const BatchShape& b, const AcouFusedArgs& a, const ChunkDesc* chunks,
+                           int nchunks, const int* rmap, int it, int threads);
 void launch_acou_ref_step(hipStream_t st, const BatchShape& b, float* px, float* pz, float* p, float* vx, float* vz,
                           const float* k, const float* rho, const float* damp, const float* wav, int sxnum,
                           float* rec, int it);

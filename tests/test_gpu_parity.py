@@ -232,10 +232,20 @@ def test_lsrtm_cg_decreases_residual(bg_mode):
     assert res[-1] < 0.5
 
 
-def test_lsrtm_cg_matches_oracle_cg():
+@pytest.mark.parametrize("deep", [False, True])
+def test_lsrtm_cg_matches_oracle_cg(deep):
     """Three CG iterations on the GPU vs CGLS driven by the FP64 oracle's
-    Born/adjoint pair: same residual history and image."""
-    allnx, allnz, pml, nt, vp, rho, refl, wav, shots = _acou_case(nshots=3, nt=300)
+    Born/adjoint pair: same residual history and image.  deep: a 220-row
+    model, so the fused Born/adjoint steps run two row blocks per window."""
+    if deep:
+        allnx, allnz, pml, nt = 48, 220, 24, 250
+        vp = models.layered_model(allnx, allnz, 19)
+        _, rho = models.elastic_from_vp(vp)
+        refl = (0.05 * np.random.default_rng(9).standard_normal((allnx, allnz))).astype(np.float32)
+        wav = wavelet(nt)
+        shots = [full_window_shot(allnx, pml, 10 + 25 * i, shot_id=i, boundary=1) for i in range(2)]
+    else:
+        allnx, allnz, pml, nt, vp, rho, refl, wav, shots = _acou_case(nshots=3, nt=300)
     s = _solver(allnx, allnz, pml, nt, allnx, wav, physics=1, bg_mode=1)
     s.set_model(vp, None, rho)
     _, d_obs, _ = s.born_forward(shots, refl)
@@ -273,3 +283,19 @@ def test_lsrtm_cg_matches_oracle_cg():
     print("gpu", res, "oracle", hist)
     np.testing.assert_allclose(res, hist, rtol=1e-4)
     assert orc.rel_l2(img, x) <= 1e-4
+
+
+def test_one_shot_per_launch_matches_batched():
+    """ds2d_set_batch(1) runs the shots one after another with the same
+    kernels: the same CG iterates as one batched launch per step."""
+    allnx, allnz, pml, nt, vp, rho, refl, wav, shots = _acou_case(nshots=3, nt=250)
+    s = _solver(allnx, allnz, pml, nt, allnx, wav, physics=1, bg_mode=1)
+    s.set_model(vp, None, rho)
+    _, d_obs, _ = s.born_forward(shots, refl)
+    img_b, res_b, _ = s.lsrtm_cg(shots, d_obs, 3)
+    s.set_batch(1)
+    img_1, res_1, _ = s.lsrtm_cg(shots, d_obs, 3)
+    s.set_batch(0)
+    print("batched", res_b, "one shot per launch", res_1)
+    np.testing.assert_allclose(res_1, res_b, rtol=1e-6)
+    assert orc.rel_l2(img_1, img_b) <= 1e-6
