@@ -135,9 +135,9 @@ def run_forward(a, cfg):
                pml=cfg["pml"], nx_max=nx_max, wavelet=wav, pml_rr=1e-3)
     s.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
     s.set_graph(a.graph)
-    chunk = threads = None
+    chunk = threads = depth = None
     if a.mode == 2:  # untimed: pick the fused kernel's x-chunk width (or the two-kernel step) for this batch
-        chunk, threads, _ = s.autotune(cfg["shots"], nsteps=min(60, cfg["nt"]))
+        chunk, threads, depth, _ = s.autotune(cfg["shots"], nsteps=min(60, cfg["nt"]))
     for _ in range(a.warmup):
         s.batch_forward(cfg["shots"], mode=a.mode, fetch=False)
     stats = []
@@ -158,7 +158,8 @@ def run_forward(a, cfg):
             "workload": cfg["desc"], "shots": len(cfg["shots"]), "nt": cfg["nt"], "cell_updates_per_step": cu,
             "kernels": KERNELS[1 if (a.mode == 2 and chunk == 0) else a.mode] +
                        (" (autotuned)" if a.mode == 2 else ""), "parallelism": "shots batched on one GPU",
-            "autotuned_tile": None if chunk is None else {"x_width": chunk, "threads": threads},
+            "autotuned_tile": None if chunk is None else {"x_width": chunk, "threads": threads,
+                                                          "prefetch_columns": depth},
             "hipgraph": a.graph},
         "roofline": roofline(stats),
     }

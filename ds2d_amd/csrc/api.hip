@@ -76,6 +76,7 @@ struct ds2d_ctx {
     ds2d_stats last{};
     int chunk_w = 0;    // fused-kernel x-chunk width (0: default)
     int chunk_t = 0;    // fused-kernel workgroup size (256/512/768; 0: default)
+    int chunk_d = 0;    // fused-kernel prefetch depth in columns (2 or 4; 0: default 2)
     int use_graph = 0;  // capture time loops into one hipGraph per loop
     int per_launch = 0; // plain Born/adjoint: shots per time loop (0: all)
     int fwd_two = 0;    // autotuned: mode 2 runs the two-kernel step instead of the fused one
@@ -665,6 +666,7 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
         std::vector<ChunkDesc> ch;
         const int W = h->chunk_w > 0 ? h->chunk_w : 64;
         const int T = h->chunk_t > 0 ? h->chunk_t : 256;
+        const int D = h->chunk_d == 4 ? 4 : 2;
         const int rows = T - 2 * FUSED_HALO;
         for (int s = 0; s < nshots; ++s)
             for (int x0 = 0; x0 < B.slots[s].nxp; x0 += W)
@@ -691,14 +693,14 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
         if (h->use_graph) {
             // whole time loop as one graph; per-launch time = loop time / nt
             if ((rc = run_steps(h, 0, nt, 1, [&](int it) {
-                     launch_fused_step(st, sh, (it & 1) ? odd : even, d_ch, (int)ch.size(), rmap, it, T);
+                     launch_fused_step(st, sh, (it & 1) ? odd : even, d_ch, (int)ch.size(), rmap, it, T, D);
                  })))
                 return rc;
         } else {
             for (int it = 0; it < nt; ++it) {
                 const bool tk = smp.take(it);
                 if (tk) smp.mark(st, 0);
-                launch_fused_step(st, sh, (it & 1) ? odd : even, d_ch, (int)ch.size(), rmap, it, T);
+                launch_fused_step(st, sh, (it & 1) ? odd : even, d_ch, (int)ch.size(), rmap, it, T, D);
                 if (tk) { smp.mark(st, 1); smp.mark(st, 2); }
             }
         }
@@ -774,13 +776,15 @@ int ds2d_set_graph(ds2d_t* h, int on) {
     return 0;
 }
 
-int ds2d_set_chunk(ds2d_t* h, int width, int threads) {
+int ds2d_set_chunk(ds2d_t* h, int width, int threads, int depth) {
     int rc;
     if ((rc = check_h(h))) return rc;
     if (width < 0 || (width > 0 && width < 8)) return fail(DS2D_EINVAL, "chunk width must be 0 or >= 8");
     if (threads != 0 && !fused_threads_ok(threads)) return fail(DS2D_EINVAL, "threads must be 0, 256, 512 or 768");
+    if (depth != 0 && depth != 2 && depth != 4) return fail(DS2D_EINVAL, "depth must be 0, 2 or 4");
     h->chunk_w = width;
     h->chunk_t = threads;
+    h->chunk_d = depth;
     h->fwd_two = 0;
     return 0;
 }
@@ -789,26 +793,28 @@ int ds2d_set_chunk(ds2d_t* h, int width, int threads) {
 // batch, and of the two-kernel step (width 0), each after one untimed warm
 // run, and keep the fastest for mode 2.
 int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nsteps, int* best_width,
-                          int* best_threads, double* best_ms) {
+                          int* best_threads, int* best_depth, double* best_ms) {
     int rc;
     if ((rc = check_h(h))) return rc;
     struct Cand {
-        int w, t;
+        int w, t, d;
     };
-    static const Cand cand[] = {{0, 0},    {32, 256}, {64, 256}, {128, 256}, {32, 512},
-                                {64, 512}, {128, 512}, {64, 768}, {128, 768}};
-    Cand best{64, 256};
+    static const Cand cand[] = {{0, 0, 0},    {32, 256, 2},  {64, 256, 2}, {128, 256, 2}, {32, 256, 4},
+                                {64, 256, 4}, {128, 256, 4}, {64, 512, 2}, {64, 768, 2},  {64, 512, 4}};
+    Cand best{64, 256, 2};
     double bt = 1e300;
-    const int keep_w = h->chunk_w, keep_t = h->chunk_t, keep_two = h->fwd_two;
+    const int keep_w = h->chunk_w, keep_t = h->chunk_t, keep_d = h->chunk_d, keep_two = h->fwd_two;
     for (const Cand& c : cand) {
         h->fwd_two = c.w == 0;
         h->chunk_w = c.w;
         h->chunk_t = c.t;
+        h->chunk_d = c.d;
         ds2d_stats s{};
         for (int rep = 0; rep < 2; ++rep)
             if ((rc = batch_forward_n(h, nshots, shots, nullptr, nullptr, 2, &s, nsteps))) {
                 h->chunk_w = keep_w;
                 h->chunk_t = keep_t;
+                h->chunk_d = keep_d;
                 h->fwd_two = keep_two;
                 return rc;
             }
@@ -820,8 +826,10 @@ int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nst
     h->fwd_two = best.w == 0;
     h->chunk_w = best.w;
     h->chunk_t = best.t;
+    h->chunk_d = best.d;
     if (best_width) *best_width = best.w;
     if (best_threads) *best_threads = best.t;
+    if (best_depth) *best_depth = best.d;
     if (best_ms) *best_ms = bt;
     return 0;
 }

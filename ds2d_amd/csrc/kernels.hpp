@@ -144,7 +144,8 @@ constexpr long long FUSED_MAX_CELLS = (1LL << 30) - (1LL << 20);
 // so it owns threads - 2*FUSED_HALO rows.
 constexpr int FUSED_HALO = 8;
 bool fused_threads_ok(int threads);
+// depth: columns of operands in flight per lane (2 or 4)
 void launch_fused_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, const ChunkDesc* chunks,
-                       int nchunks, const int* rmap, int it, int threads);
+                       int nchunks, const int* rmap, int it, int threads, int depth);
 
 }  // namespace ds2d

@@ -17,10 +17,10 @@
 // column j-4.  The eight x taps of every derivative live in per-lane
 // register queues; the eight z taps are read from a double-buffered LDS copy
 // of the current column (one barrier per column).  The march is unrolled by
-// two with one operand set per phase: the operands of column j + 2 are
-// loaded right after column j is consumed, into the registers column j just
-// freed, so no loaded value is copied before use and HBM latency overlaps
-// two column iterations.  The eight operands every cell needs are loaded
+// the prefetch depth PF (2 or 4) with one operand set per phase: the operands
+// of column j + PF are loaded right after column j is consumed, into the
+// registers column j just freed, so no loaded value is copied before use and
+// HBM latency overlaps PF column iterations.  The eight operands every cell needs are loaded
 // unconditionally from clamped addresses (values of clamped cells are never
 // used), which keeps those loads free of branches.  The state is ping-ponged
 // (read A, write B), so the four halo columns of velocity on each side are
@@ -48,7 +48,7 @@ struct Col {
 };
 }  // namespace
 
-template <int T>
+template <int T, int PF>
 __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* __restrict__ chunks, int nchunks,
                                                   int per_xcd, const int* __restrict__ rmap,
                                                   const SlotDesc* __restrict__ slots, float idx, float idz, float dt,
@@ -292,32 +292,57 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
     };
 
     // even number of columns: an extra trailing column stores nothing
-    const int jend = j0 + ((x1 + 4 - j0 + 1) & ~1);
-    Col cA = fetch(j0), cB = fetch(j0 + 1);
-    for (int j = j0; j < jend; j += 2) {
-        column(j, cA, 0);
-        cA = fetch(j + 2);
-        column(j + 1, cB, 1);
-        cB = fetch(j + 3);
+    // a multiple of PF columns: up to PF-1 extra trailing columns store nothing
+    const int jend = j0 + (x1 + 4 - j0 + PF - 1) / PF * PF;
+    if (PF == 2) {
+        Col cA = fetch(j0), cB = fetch(j0 + 1);
+        for (int j = j0; j < jend; j += 2) {
+            column(j, cA, 0);
+            cA = fetch(j + 2);
+            column(j + 1, cB, 1);
+            cB = fetch(j + 3);
+        }
+    } else {
+        Col cA = fetch(j0), cB = fetch(j0 + 1), cC = fetch(j0 + 2), cD = fetch(j0 + 3);
+        for (int j = j0; j < jend; j += 4) {
+            column(j, cA, 0);
+            cA = fetch(j + 4);
+            column(j + 1, cB, 1);
+            cB = fetch(j + 5);
+            column(j + 2, cC, 0);
+            cC = fetch(j + 6);
+            column(j + 3, cD, 1);
+            cD = fetch(j + 7);
+        }
     }
 }
 
 bool fused_threads_ok(int threads) { return threads == 256 || threads == 512 || threads == 768; }
 
 void launch_fused_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, const ChunkDesc* chunks,
-                       int nchunks, const int* rmap, int it, int threads) {
+                       int nchunks, const int* rmap, int it, int threads, int depth) {
     const int per_xcd = (nchunks + 7) / 8;
     const dim3 grid(per_xcd * 8);
     const float idx = 1.f / b.dx, idz = 1.f / b.dz;
-    if (threads == 256)
-        hipLaunchKernelGGL(k_fused_step<256>, grid, dim3(256), 0, st, a, chunks, nchunks, per_xcd, rmap, b.slots, idx,
-                           idz, b.dt, b.nt, it);
-    else if (threads == 512)
-        hipLaunchKernelGGL(k_fused_step<512>, grid, dim3(512), 0, st, a, chunks, nchunks, per_xcd, rmap, b.slots, idx,
-                           idz, b.dt, b.nt, it);
-    else
-        hipLaunchKernelGGL(k_fused_step<768>, grid, dim3(768), 0, st, a, chunks, nchunks, per_xcd, rmap, b.slots, idx,
-                           idz, b.dt, b.nt, it);
+#define DS2D_FUSED_LAUNCH(TT, PP)                                                                               \
+    hipLaunchKernelGGL((k_fused_step<TT, PP>), grid, dim3(TT), 0, st, a, chunks, nchunks, per_xcd, rmap, b.slots, \
+                       idx, idz, b.dt, b.nt, it)
+    if (depth == 4) {
+        if (threads == 256)
+            DS2D_FUSED_LAUNCH(256, 4);
+        else if (threads == 512)
+            DS2D_FUSED_LAUNCH(512, 4);
+        else
+            DS2D_FUSED_LAUNCH(768, 4);
+    } else {
+        if (threads == 256)
+            DS2D_FUSED_LAUNCH(256, 2);
+        else if (threads == 512)
+            DS2D_FUSED_LAUNCH(512, 2);
+        else
+            DS2D_FUSED_LAUNCH(768, 2);
+    }
+#undef DS2D_FUSED_LAUNCH
 }
 
 }  // namespace ds2d

@@ -71,10 +71,10 @@ SIGNATURES = {
     "ds2d_take_gradients": (C.c_int, [_H, _F, _F, _F, _F, _F, _F, _F]),
     "ds2d_forward": (C.c_int, [_H, _F, _F]),
     "ds2d_batch_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.c_int, C.POINTER(Stats)]),
-    "ds2d_set_chunk": (C.c_int, [_H, C.c_int, C.c_int]),
+    "ds2d_set_chunk": (C.c_int, [_H, C.c_int, C.c_int, C.c_int]),
     "ds2d_set_graph": (C.c_int, [_H, C.c_int]),
     "ds2d_autotune_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), C.c_int, C.POINTER(C.c_int),
-                                        C.POINTER(C.c_int), _D]),
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int), _D]),
     "ds2d_acoustic_ref_forward": (C.c_int, [_H, C.c_int, C.c_int, _F]),
     "ds2d_born_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, _F, C.POINTER(Stats)]),
     "ds2d_born_adjoint": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.POINTER(Stats)]),
@@ -220,22 +220,21 @@ class Solver:
     def _rec(self, shots):
         return np.zeros(sum(s.ntr for s in shots) * self.nt, np.float32)
 
-    def set_chunk(self, width, threads=0):
-        """Fused-kernel tile: x-chunk width and workgroup lanes (0 = defaults)."""
-        _check(self.lib.ds2d_set_chunk(self.h, int(width), int(threads)))
+    def set_chunk(self, width, threads=0, depth=0):
+        """Fused-kernel tile: x-chunk width, workgroup lanes, prefetch depth (0 = defaults)."""
+        _check(self.lib.ds2d_set_chunk(self.h, int(width), int(threads), int(depth)))
 
     def set_graph(self, on):
         _check(self.lib.ds2d_set_graph(self.h, 1 if on else 0))
 
     def autotune(self, shots, nsteps=100):
-        """Pick the forward step's tile for this batch; returns (width, threads, ms);
+        """Pick the forward step's tile for this batch; returns (width, threads, depth, ms);
         width 0 means the two-kernel step won."""
-        w = C.c_int(0)
-        t = C.c_int(0)
+        w, t, d = C.c_int(0), C.c_int(0), C.c_int(0)
         ms = C.c_double(0)
         _check(self.lib.ds2d_autotune_forward(self.h, len(shots), _shots_array(shots), nsteps, C.byref(w),
-                                              C.byref(t), C.byref(ms)))
-        return w.value, t.value, ms.value
+                                              C.byref(t), C.byref(d), C.byref(ms)))
+        return w.value, t.value, d.value, ms.value
 
     def batch_forward(self, shots, mode=2, fetch=True):
         """Batched elastic forward; fetch=False leaves the records in HBM."""

@@ -124,17 +124,18 @@ int ds2d_forward(ds2d_t* h, float* rec_z, float* rec_x);
  * step (or what ds2d_autotune_forward picked). */
 int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec_z, float* rec_x,
                        int mode, ds2d_stats* stats);
-/* Tile of the fused kernel: x-chunk width (0 = default 64) and workgroup
- * size (256, 512 or 768 lanes, one z row each, owning threads-16 rows; 0 =
- * default 256); and an autotuner that times nsteps per candidate on this
- * batch -- the fused kernel at several tiles and the two-kernel step
- * (reported as width 0) -- and keeps the fastest for mode 2.  No reference
- * counterpart (new capability). */
-int ds2d_set_chunk(ds2d_t* h, int width, int threads);
+/* Tile of the fused kernel: x-chunk width (0 = default 64), workgroup size
+ * (256, 512 or 768 lanes, one z row each, owning threads-16 rows; 0 =
+ * default 256) and prefetch depth (columns of operands in flight, 2 or 4;
+ * 0 = 2); and an autotuner that times nsteps per candidate on this batch --
+ * the fused kernel at several tiles and the two-kernel step (reported as
+ * width 0) -- and keeps the fastest for mode 2.  No reference counterpart
+ * (new capability). */
+int ds2d_set_chunk(ds2d_t* h, int width, int threads, int depth);
 /* Capture each time loop into one hipGraph (1) or launch step by step (0). */
 int ds2d_set_graph(ds2d_t* h, int on);
 int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nsteps, int* best_width,
-                          int* best_threads, double* best_ms);
+                          int* best_threads, int* best_depth, double* best_ms);
 
 /* Reference acoustic step (split p, mirrored stagger): pressure at depth pml. */
 int ds2d_acoustic_ref_forward(ds2d_t* h, int sxnum, int ngx_min, float* rec_p);

@@ -57,8 +57,9 @@ def test_elastic_forward_batched_uneven(mode):
         off += n
 
 
-@pytest.mark.parametrize("width,threads", [(16, 256), (128, 256), (64, 512), (40, 768)])
-def test_fused_tiles_match_oracle(width, threads):
+@pytest.mark.parametrize("width,threads,depth", [(16, 256, 2), (128, 256, 2), (64, 512, 2), (40, 768, 2),
+                                                 (64, 256, 4), (24, 512, 4)])
+def test_fused_tiles_match_oracle(width, threads, depth):
     """Fused kernel with several tiles on a window deeper than one z block:
     the source sits on the first block boundary of the 256-lane tile (row 240)
     and the receivers in the second block, with a PML window and a random-
@@ -71,14 +72,14 @@ def test_fused_tiles_match_oracle(width, threads):
              ShotSpec(10, 41, pml + 20, pml + 219, pml + 231, pml + np.arange(0, 41, 2, dtype=np.int32), 1, 1)]
     s = _solver(allnx, allnz, pml, nt, 64, wav)
     s.set_model(vp, vs, rho)
-    s.set_chunk(width, threads)
+    s.set_chunk(width, threads, depth)
     rz, rx, _ = s.batch_forward(shots, mode=2)
     off = 0
     for sh in shots:
         n = sh.ntr * nt
         oz, ox = orc.elastic_forward(_orc_shot(sh, allnx, allnz, pml, nt), vp, vs, rho, wav)
         ez, ex = orc.rel_l2(rz[off:off + n], oz), orc.rel_l2(rx[off:off + n], ox)
-        print(f"tile {width}x{threads} boundary={sh.boundary}: rel_z={ez:.2e} rel_x={ex:.2e}")
+        print(f"tile {width}x{threads}x{depth} boundary={sh.boundary}: rel_z={ez:.2e} rel_x={ex:.2e}")
         assert np.abs(oz).max() > 0
         assert ez <= TOL and ex <= TOL
         off += n
