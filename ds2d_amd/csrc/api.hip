@@ -1024,10 +1024,10 @@ void acou_fwd(ds2d_ctx* h, const Batch& B, AcouWS& w, bool do_bg, bool do_born, 
         // Born field only, from the resident history: one fused launch per step
         AcouFusedArgs even = acou_fused_args(h, w, w.bn, w.bn2), odd = acou_fused_args(h, w, w.bn2, w.bn);
         even.rec = odd.rec = w.rec2;
-        const auto groups = launch_groups(h, B);
-        for (size_t gi = 0; gi < groups.size(); ++gi)
+        // chunk lists hold absolute slot indices: launch with the whole slot table
+        for (size_t gi = 0; gi < w.gchunks.size(); ++gi)
             for (int it = 0; it < h->g.nt; ++it)
-                launch_acou_born_fused(st, groups[gi], (it & 1) ? odd : even, w.chunks + w.gchunks[gi].first,
+                launch_acou_born_fused(st, B.shape, (it & 1) ? odd : even, w.chunks + w.gchunks[gi].first,
                                        w.gchunks[gi].second, w.rmap, it, ACOU_THREADS);
         return;
     }
@@ -1051,10 +1051,9 @@ void acou_bwd(ds2d_ctx* h, const Batch& B, AcouWS& w) {
         AcouFusedArgs even = acou_fused_args(h, w, w.lm, w.lm2), odd = acou_fused_args(h, w, w.lm2, w.lm);
         even.grad = odd.grad = w.grad_pad;
         even.res = odd.res = w.rec2;
-        const auto groups = launch_groups(h, B);
-        for (size_t gi = 0; gi < groups.size(); ++gi)
+        for (size_t gi = 0; gi < w.gchunks.size(); ++gi)
             for (int it = h->g.nt - 1, k = 0; it >= 0; --it, ++k)
-                launch_acou_adj_fused(st, groups[gi], (k & 1) ? odd : even, w.chunks + w.gchunks[gi].first,
+                launch_acou_adj_fused(st, B.shape, (k & 1) ? odd : even, w.chunks + w.gchunks[gi].first,
                                       w.gchunks[gi].second, w.rmap, it, ACOU_THREADS);
         return;
     }
