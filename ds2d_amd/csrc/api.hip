@@ -80,6 +80,7 @@ struct ds2d_ctx {
     int use_graph = 0;  // capture time loops into one hipGraph per loop
     int per_launch = 0; // plain Born/adjoint: shots per time loop (0: all)
     int fwd_two = 0;    // autotuned: mode 2 runs the two-kernel step instead of the fused one
+    int acou_fused = -1;  // fused Born/adjoint steps: -1 auto, 0 never, 1 whenever eligible
 
     ~ds2d_ctx() {
         for (auto& kv : ws)
@@ -910,7 +911,7 @@ struct AcouWS {
     long long n = 0, R = 0, W = 0;
 };
 
-constexpr int ACOU_CHUNK_W = 64, ACOU_THREADS = 256;
+constexpr int ACOU_CHUNK_W = 64, ACOU_THREADS = 256, ACOU_MIN_CHUNKS = 1024;
 
 int acou_prepare(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouWS& w) {
     int rc;
@@ -941,12 +942,18 @@ int acou_prepare(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouWS
     const double budget = h->g.hbm_budget_gb > 0 ? h->g.hbm_budget_gb : 64.0;
     w.store = h->g.bg_mode == 1 || (h->g.bg_mode == 0 && hist_gb <= budget);
     if (w.store && (rc = h->fbuf("hist", (size_t)B.total_hist, &w.hist))) return rc;
-    // fused Born / adjoint steps (store mode; 32-bit byte offsets within a field)
-    w.fused = w.store && n <= FUSED_MAX_CELLS;
+    // receiver map (column -> trace) of every slot, for the in-kernel recorder and injection
+    if ((rc = h->fbuf("armap", std::max<long long>(B.total_rmap, 1), &w.rmap))) return rc;
+    HIP_TRY(hipMemsetAsync(w.rmap, 0xff, B.total_rmap * 4, h->st));
+    launch_fast_rmap(h->st, B.shape, w.rmap);
+    // fused Born / adjoint steps: store mode, 32-bit byte offsets within a field, and enough
+    // x-chunks for every CU (small batches run faster on the per-cell kernels)
+    long long nch = 0;
+    for (auto& S : B.slots)
+        nch += (long long)((S.nxp + ACOU_CHUNK_W - 1) / ACOU_CHUNK_W) * ((S.nzp + ACOU_THREADS - 17) / (ACOU_THREADS - 16));
+    w.fused = w.store && n <= FUSED_MAX_CELLS &&
+              (h->acou_fused == 1 || (h->acou_fused < 0 && nch >= ACOU_MIN_CHUNKS));
     if (w.fused) {
-        if ((rc = h->fbuf("armap", std::max<long long>(B.total_rmap, 1), &w.rmap))) return rc;
-        HIP_TRY(hipMemsetAsync(w.rmap, 0xff, B.total_rmap * 4, h->st));
-        launch_fast_rmap(h->st, B.shape, w.rmap);
         std::vector<ChunkDesc> ch;
         const int ns = (int)B.slots.size();
         const int gs = h->per_launch > 0 ? std::min(h->per_launch, ns) : ns;
@@ -1036,7 +1043,7 @@ void acou_fwd(ds2d_ctx* h, const Batch& B, AcouWS& w, bool do_bg, bool do_born, 
         for (int it = 0; it < h->g.nt; ++it) {
             launch_acou_v(st, sh, w.bg, w.bn, w.m, h->d_wav, it, 1.f, do_bg, do_born);
             launch_acou_p(st, sh, w.bg, w.bn, w.m, w.hist, rec_bg ? w.rec : nullptr, do_born ? w.rec2 : nullptr, it,
-                          do_bg, do_born, hmode);
+                          do_bg, do_born, hmode, w.rmap);
         }
 }
 
@@ -1057,11 +1064,13 @@ void acou_bwd(ds2d_ctx* h, const Batch& B, AcouWS& w) {
                                       w.gchunks[gi].second, w.rmap, it, ACOU_THREADS);
         return;
     }
-    for (const BatchShape& sh : launch_groups(h, B))
+    for (const BatchShape& sh : launch_groups(h, B)) {
+        launch_acou_inj(st, sh, w.lm.p, w.rec2, h->g.nt - 1);
         for (int it = h->g.nt - 1; it >= 0; --it) {
-            launch_acou_adj_p(st, sh, w.bg, w.lm, w.m, w.grad_pad, w.store ? w.hist : nullptr, w.rec2, it);
-            launch_acou_adj_v(st, sh, w.bg, w.lm, w.m, h->d_wav, it, w.store ? 0 : 1);
+            launch_acou_adj_p(st, sh, w.bg, w.lm, w.m, w.grad_pad, w.store ? w.hist : nullptr, it);
+            launch_acou_adj_v(st, sh, w.bg, w.lm, w.m, h->d_wav, it, w.store ? 0 : 1, w.rmap, w.rec2);
         }
+    }
 }
 
 // Padded per-slot gradient -> global (deterministic slot order), overwrite.
@@ -1228,6 +1237,13 @@ int ds2d_lsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_
     HIP_TRY(hipMemcpyAsync(image, x, A * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int ds2d_set_fused(ds2d_t* h, int mode) {
+    if (!h) return fail(DS2D_EINVAL, "null handle");
+    if (mode < -1 || mode > 1) return fail(DS2D_EINVAL, "mode must be -1, 0 or 1");
+    h->acou_fused = mode;
     return 0;
 }
 

@@ -74,14 +74,19 @@ struct AcouMed {
 };
 void launch_acou_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, const float* wav,
                    int it, float sign, int do_bg, int do_born);
-// hist_mode: 0 none, 1 store the background increment, 2 read it (do_bg = 0)
+// hist_mode: 0 none, 1 store the background increment, 2 read it (do_bg = 0).
+// Records (rec_bg / rec_born, may be null) are taken inside, through rmap.
 void launch_acou_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, float* hist,
-                   float* rec_bg, float* rec_born, int it, int do_bg, int do_born, int hist_mode);
+                   float* rec_bg, float* rec_born, int it, int do_bg, int do_born, int hist_mode, const int* rmap);
+// Backward step it: (injection of res[it] already in lam.p) adj_p, then adj_v,
+// which also injects res[it - 1] for the next step.  Step nt - 1 is injected
+// by launch_acou_inj.
+void launch_acou_inj(hipStream_t st, const BatchShape& b, float* lp, const float* res, int it);
 // hist != null: gradient from the stored increment, background untouched
 void launch_acou_adj_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m, float* grad,
-                       const float* hist, const float* res, int it);
+                       const float* hist, int it);
 void launch_acou_adj_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m,
-                       const float* wav, int it, int do_bg);
+                       const float* wav, int it, int do_bg, const int* rmap, const float* res);
 struct ChunkDesc;
 // Fused plain acoustic steps (kernels_acoustic_fused.hip): state A -> B, each
 // 3 fields (vx, vz, p) of n floats from the base pointer.

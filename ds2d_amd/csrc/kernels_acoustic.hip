@@ -79,7 +79,8 @@ __global__ __launch_bounds__(256) void k_acou_v(AcouField bg, AcouField bn, Acou
 // born: dpb += k D- dv + 2 m dp.
 __global__ __launch_bounds__(256) void k_acou_p(AcouField bg, AcouField bn, AcouMed m, float* hist,
                                                 const SlotDesc* __restrict__ slots, float ax, float az, int it,
-                                                int do_bg, int do_born, int hist_mode) {
+                                                int do_bg, int do_born, int hist_mode, const int* __restrict__ rmap,
+                                                float* rbg, float* rbn, int nt) {
     const SlotDesc S = slots[blockIdx.z];
     Tile t;
     if (!tile_cell(S, t)) return;
@@ -99,6 +100,15 @@ __global__ __launch_bounds__(256) void k_acou_p(AcouField bg, AcouField bn, Acou
     if (do_born) {
         const float bd = kk * (ax * dmx(bn.vx, c, s) + az * dmx(bn.vz, c, 1));
         bn.p[c] += bd + 2 * m.m[c] * dp;
+    }
+    // surface recorder (FD2DGPU.cu:3093-3101): p after this step at (rec_ix, igz)
+    if (t.iz == S.igz && (rbg || rbn)) {
+        const int r = rmap[S.rmap_off + t.ix];
+        if (r >= 0) {
+            const size_t o = (size_t)S.rec_off + (size_t)r * nt + it;
+            if (rbg) rbg[o] = bg.p[c];
+            if (rbn) rbn[o] = bn.p[c];
+        }
     }
 }
 
@@ -141,7 +151,9 @@ __global__ __launch_bounds__(256) void k_acou_adj_p(AcouField bg, AcouField lm, 
 //   lp += -(dt/d) D- (b lv) with b lv masked to the update region.
 __global__ __launch_bounds__(256) void k_acou_adj_v(AcouField bg, AcouField lm, AcouMed m,
                                                     const float* __restrict__ wav, const SlotDesc* __restrict__ slots,
-                                                    float ax, float az, int it, int do_bg) {
+                                                    float ax, float az, int it, int do_bg,
+                                                    const int* __restrict__ rmap, const float* __restrict__ res,
+                                                    int nt) {
     const SlotDesc S = slots[blockIdx.z];
     Tile t;
     if (!tile_cell(S, t)) return;
@@ -172,19 +184,15 @@ __global__ __launch_bounds__(256) void k_acou_adj_v(AcouField bg, AcouField lm, 
     const float gz = DS2D_C1 * (UZ(ix, iz) - UZ(ix, iz - 1)) + DS2D_C2 * (UZ(ix, iz + 1) - UZ(ix, iz - 2)) +
                      DS2D_C3 * (UZ(ix, iz + 2) - UZ(ix, iz - 3)) + DS2D_C4 * (UZ(ix, iz + 3) - UZ(ix, iz - 4));
     lm.p[c] += -(ax * gx + az * gz);
+    // receiver injection of the next backward step (it - 1), fused here: this
+    // cell's lp is final for step it and no other cell reads it in this kernel
+    if (res && it > 0 && iz == S.igz) {
+        const int r = rmap[S.rmap_off + ix];
+        if (r >= 0) lm.p[c] += res[(size_t)S.rec_off + (size_t)r * nt + (it - 1)];
+    }
 }
 
-// rec[r][it] = p, born p at (rec_ix, igz)  /  lp[rec] += res[r][it].
-__global__ void k_acou_rec(const float* __restrict__ p, const float* __restrict__ bp, float* rbg, float* rbn,
-                           const int* __restrict__ rix, const SlotDesc* __restrict__ slots, int nt, int it) {
-    const SlotDesc S = slots[blockIdx.y];
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= S.ntr) return;
-    const size_t c = (size_t)S.off + (size_t)rix[S.rix_off + r] * S.nzp + S.igz;
-    const size_t o = (size_t)S.rec_off + (size_t)r * nt + it;
-    if (rbg) rbg[o] = p[c];
-    if (rbn) rbn[o] = bp[c];
-}
+// lp[rec] += res[r][it] (first backward step; later steps inject inside k_acou_adj_v).
 __global__ void k_acou_inj(float* lp, const float* __restrict__ res, const int* __restrict__ rix,
                            const SlotDesc* __restrict__ slots, int nt, int it) {
     const SlotDesc S = slots[blockIdx.y];
@@ -202,24 +210,23 @@ void launch_acou_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField 
                        it, sign, do_bg, do_born);
 }
 void launch_acou_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, float* hist,
-                   float* rec_bg, float* rec_born, int it, int do_bg, int do_born, int hist_mode) {
+                   float* rec_bg, float* rec_born, int it, int do_bg, int do_born, int hist_mode, const int* rmap) {
     hipLaunchKernelGGL(k_acou_p, cgrid(b), dim3(BZ, BX), 0, st, bg, born, m, hist, b.slots, b.dt / b.dx, b.dt / b.dz,
-                       it, do_bg, do_born, hist_mode);
-    if (rec_bg || rec_born)
-        hipLaunchKernelGGL(k_acou_rec, dim3(ceil_div(b.max_ntr, 64), b.nslots), dim3(64), 0, st, bg.p, born.p, rec_bg,
-                           rec_born, b.rix, b.slots, b.nt, it);
+                       it, do_bg, do_born, hist_mode, rmap, rec_bg, rec_born, b.nt);
+}
+void launch_acou_inj(hipStream_t st, const BatchShape& b, float* lp, const float* res, int it) {
+    hipLaunchKernelGGL(k_acou_inj, dim3(ceil_div(b.max_ntr, 64), b.nslots), dim3(64), 0, st, lp, res, b.rix,
+                       b.slots, b.nt, it);
 }
 void launch_acou_adj_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m, float* grad,
-                       const float* hist, const float* res, int it) {
-    hipLaunchKernelGGL(k_acou_inj, dim3(ceil_div(b.max_ntr, 64), b.nslots), dim3(64), 0, st, lam.p, res, b.rix,
-                       b.slots, b.nt, it);
+                       const float* hist, int it) {
     hipLaunchKernelGGL(k_acou_adj_p, cgrid(b), dim3(BZ, BX), 0, st, bg, lam, m, grad, hist, b.slots, b.dt / b.dx,
                        b.dt / b.dz, it);
 }
 void launch_acou_adj_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m,
-                       const float* wav, int it, int do_bg) {
+                       const float* wav, int it, int do_bg, const int* rmap, const float* res) {
     hipLaunchKernelGGL(k_acou_adj_v, cgrid(b), dim3(BZ, BX), 0, st, bg, lam, m, wav, b.slots, b.dt / b.dx,
-                       b.dt / b.dz, it, do_bg);
+                       b.dt / b.dz, it, do_bg, rmap, res, b.nt);
 }
 
 // ---------------------------------------------------------------------------

@@ -53,8 +53,10 @@ void launch_conv(hipStream_t st, const ConvArgs& a);
 // partial buffer may hold at most max_part floats.
 void wgrad_plan(int n, int h, int w, int cout, int cin, long long max_part, int* rows, int* nchunk);
 void launch_wgrad(hipStream_t st, const WgradArgs& a);
-// out[e] = sum_c part[c * len + e]  (deterministic order), optionally += into out.
-void launch_reduce(hipStream_t st, const float* part, int nchunk, long long len, float* out, int accumulate);
+// out[e] = sum_c part[c * len + e] in a fixed order (two levels when the
+// element count is small; the first level works in place, so part is
+// clobbered), optionally += into out.
+void launch_reduce(hipStream_t st, float* part, int nchunk, long long len, float* out, int accumulate);
 
 // First layer (one input channel, fp32 image): y = relu(conv(x) + b) -> bf16 [npix][cout]
 void launch_conv1(hipStream_t st, const float* x, const float* w, const float* b, bf16_t* y, int n, int h, int wd,

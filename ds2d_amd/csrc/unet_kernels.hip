@@ -246,7 +246,7 @@ void wgrad_plan(int n, int h, int w, int cout, int cin, long long max_part, int*
     const int tco = cout % 64 == 0 ? 64 : 32, tci = cin % 64 == 0 ? 64 : 32;
     const long long tiles = (long long)(cout / tco) * (cin / tci);
     const int nseg = (w + ZS - 1) / ZS;
-    const long long want = std::max<long long>(1, (1024 + tiles - 1) / tiles);  // chunks for ~1024 blocks
+    const long long want = std::max<long long>(1, (512 + tiles - 1) / tiles);  // chunks for ~512 blocks
     long long per = std::max<long long>(1, (want + (long long)n * nseg - 1) / ((long long)n * nseg));
     int r = (int)std::max<long long>(1, (h + per - 1) / per);
     r = std::max(r, std::min(4, h));
@@ -284,9 +284,27 @@ __global__ void k_reduce(const float* part, int nchunk, long long len, float* ou
     out[e] = accumulate ? out[e] + s : s;
 }
 
-void launch_reduce(hipStream_t st, const float* part, int nchunk, long long len, float* out, int accumulate) {
-    hipLaunchKernelGGL(k_reduce, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, nchunk, len, out,
-                       accumulate);
+// Level 1 of a two-level reduction, in place: part[g][e] = sum over chunks
+// c = g, g + G, g + 2G, ... of part[c][e] (set g is read only by block row g).
+__global__ void k_reduce_strided(float* part, int nchunk, int G, long long len) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = blockIdx.y;
+    if (e >= len) return;
+    float s = 0.f;
+    for (int c = g; c < nchunk; c += G) s += part[(long long)c * len + e];
+    part[(long long)g * len + e] = s;
+}
+
+void launch_reduce(hipStream_t st, float* part, int nchunk, long long len, float* out, int accumulate) {
+    const long long eb = (len + 255) / 256;
+    // enough blocks to keep every CU busy: split the chunk loop when the element count is small
+    const int G = (int)std::min<long long>(nchunk, std::max<long long>(1, (2048 + eb - 1) / eb));
+    if (G > 1 && nchunk >= 4 * G) {
+        hipLaunchKernelGGL(k_reduce_strided, dim3((unsigned)eb, G), dim3(256), 0, st, part, nchunk,
+                           G, len);
+        nchunk = G;
+    }
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)eb), dim3(256), 0, st, part, nchunk, len, out, accumulate);
 }
 
 // ---------------------------------------------------------------------------

@@ -82,6 +82,7 @@ SIGNATURES = {
     "ds2d_lsrtm_learned": (C.c_int, [_H, _H, C.c_int, C.POINTER(Shot), _F, C.c_int, _F, _D, C.POINTER(C.c_int),
                                      C.POINTER(Stats)]),
     "ds2d_set_batch": (C.c_int, [_H, C.c_int]),
+    "ds2d_set_fused": (C.c_int, [_H, C.c_int]),
     "ds2d_unet_create": (C.c_int, [C.POINTER(UnetCfg), C.POINTER(_H)]),
     "ds2d_unet_destroy": (C.c_int, [_H]),
     "ds2d_unet_num_params": (C.c_int, [_H, C.POINTER(C.c_longlong)]),
@@ -279,6 +280,10 @@ class Solver:
         _check(self.lib.ds2d_born_adjoint(self.h, len(shots), _shots_array(shots), fptr(res), fptr(g),
                                           C.byref(st)))
         return g.reshape(self.allnx, self.allnz), st.as_dict()
+
+    def set_fused(self, mode):
+        """Fused Born/adjoint steps of the resident-history passes: -1 auto, 0 never, 1 always."""
+        _check(self.lib.ds2d_set_fused(self.h, int(mode)))
 
     def set_batch(self, shots_per_launch):
         """Shots per time loop for the plain Born/adjoint/CG entries (0 = all)."""

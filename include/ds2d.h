@@ -216,6 +216,10 @@ int ds2d_lsrtm_learned(ds2d_t* h, ds2d_unet_t* u, int nshots, const ds2d_shot* s
  * batched launch, the default; 1 = one shot after another as the reference's
  * per-rank shot loop does).  Applies to the plain Born/adjoint/CG entries. */
 int ds2d_set_batch(ds2d_t* h, int shots_per_launch);
+/* Resident-history Born and adjoint passes as one fused launch per step
+ * (x-marching workgroups): -1 = auto (when the batch gives every CU several
+ * x-chunks; default), 0 = never (per-cell kernels), 1 = always. */
+int ds2d_set_fused(ds2d_t* h, int mode);
 
 int ds2d_laplace_filter(int nz, int nx, int order, float dx, float* dat);
 int ds2d_laplace_band(int nz, int nx, float* dat);

@@ -189,12 +189,14 @@ def _acou_case(nshots=3, nt=300):
     return allnx, allnz, pml, nt, vp, rho, refl, wav, shots
 
 
-@pytest.mark.parametrize("bg_mode", [1, 2])
-def test_born_forward_and_adjoint(bg_mode):
-    """bg_mode 1: background increments resident in HBM; 2: background
-    rebuilt by reversal through the random boundary (FP32 reversal drift)."""
+@pytest.mark.parametrize("bg_mode,fused", [(1, 0), (1, 1), (2, 0)])
+def test_born_forward_and_adjoint(bg_mode, fused):
+    """bg_mode 1: background increments resident in HBM (per-cell or fused
+    adjoint kernels); 2: background rebuilt by reversal through the random
+    boundary (FP32 reversal drift)."""
     allnx, allnz, pml, nt, vp, rho, refl, wav, shots = _acou_case()
     s = _solver(allnx, allnz, pml, nt, allnx, wav, physics=1, bg_mode=bg_mode)
+    s.set_fused(fused)
     s.set_model(vp, None, rho)
     rbg, rbn, _ = s.born_forward(shots, refl)
     res = np.random.default_rng(8).standard_normal(rbn.size).astype(np.float32)
@@ -233,8 +235,8 @@ def test_lsrtm_cg_decreases_residual(bg_mode):
     assert res[-1] < 0.5
 
 
-@pytest.mark.parametrize("deep", [False, True])
-def test_lsrtm_cg_matches_oracle_cg(deep):
+@pytest.mark.parametrize("deep,fused", [(False, 0), (False, 1), (True, 1)])
+def test_lsrtm_cg_matches_oracle_cg(deep, fused):
     """Three CG iterations on the GPU vs CGLS driven by the FP64 oracle's
     Born/adjoint pair: same residual history and image.  deep: a 220-row
     model, so the fused Born/adjoint steps run two row blocks per window."""
@@ -248,6 +250,7 @@ def test_lsrtm_cg_matches_oracle_cg(deep):
     else:
         allnx, allnz, pml, nt, vp, rho, refl, wav, shots = _acou_case(nshots=3, nt=300)
     s = _solver(allnx, allnz, pml, nt, allnx, wav, physics=1, bg_mode=1)
+    s.set_fused(fused)
     s.set_model(vp, None, rho)
     _, d_obs, _ = s.born_forward(shots, refl)
     img, res, _ = s.lsrtm_cg(shots, d_obs, 3)
@@ -286,11 +289,13 @@ def test_lsrtm_cg_matches_oracle_cg(deep):
     assert orc.rel_l2(img, x) <= 1e-4
 
 
-def test_one_shot_per_launch_matches_batched():
+@pytest.mark.parametrize("fused", [0, 1])
+def test_one_shot_per_launch_matches_batched(fused):
     """ds2d_set_batch(1) runs the shots one after another with the same
     kernels: the same CG iterates as one batched launch per step."""
     allnx, allnz, pml, nt, vp, rho, refl, wav, shots = _acou_case(nshots=3, nt=250)
     s = _solver(allnx, allnz, pml, nt, allnx, wav, physics=1, bg_mode=1)
+    s.set_fused(fused)
     s.set_model(vp, None, rho)
     _, d_obs, _ = s.born_forward(shots, refl)
     img_b, res_b, _ = s.lsrtm_cg(shots, d_obs, 3)
