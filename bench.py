@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "unet"])
     ap.add_argument("--train-models", type=int, default=64, help="c4: training models (seeds 1000..)")
     ap.add_argument("--eval-models", type=int, default=4, help="c4: held-out models (seeds 2000..)")
     ap.add_argument("--train-steps", type=int, default=1500, help="c4: Adam steps")
@@ -55,8 +55,13 @@ def parse():
     ap.add_argument("--graph", action="store_true", help="capture each time loop in one hipGraph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--traffic-json", default=None,
-                    help="rocprof FETCH/WRITE summary (JSON) to embed as roofline.traffic")
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "k_fused_step_c3_traffic.json"),
+                    help="rocprof FETCH/WRITE summary (JSON) to embed as roofline.traffic when its kernel and "
+                         "workload match the run")
+    ap.add_argument("--tile", default=None,
+                    help="mode 2: fixed fused tile 'width,threads,depth' instead of the autotuner")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB device copy 5 times (calibrates rocprof FETCH/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -136,7 +141,10 @@ def run_forward(a, cfg):
     s.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
     s.set_graph(a.graph)
     chunk = threads = depth = None
-    if a.mode == 2:  # untimed: pick the fused kernel's x-chunk width (or the two-kernel step) for this batch
+    if a.mode == 2 and a.tile:
+        chunk, threads, depth = (int(v) for v in a.tile.split(","))
+        s.set_chunk(chunk, threads, depth)
+    elif a.mode == 2:  # untimed: pick the fused kernel's tile (or the two-kernel step) for this batch
         chunk, threads, depth, _ = s.autotune(cfg["shots"], nsteps=min(60, cfg["nt"]))
     for _ in range(a.warmup):
         s.batch_forward(cfg["shots"], mode=a.mode, fetch=False)
@@ -306,8 +314,45 @@ def run_learned(a):
     return out
 
 
+def run_unet(a):
+    """U-Net training throughput: Adam steps (batch 8) of the 4-level, 32-channel
+    U-Net on 512x256 images with the BF16 MFMA kernels (synthetic data)."""
+    from ds2d_amd import Unet
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((16, 512, 256)).astype(np.float32)
+    t = rng.standard_normal((16, 512, 256)).astype(np.float32)
+    u = Unet(levels=4, base=32, max_batch=8, max_h=512, max_w=256, seed=1)
+    u.set_data(x, t)
+    u.train(max(1, a.warmup), batch=8, lr=1e-3, seed=2)
+    n = max(1, a.steps) * 10
+    _, st = u.train(n, batch=8, lr=1e-3, seed=3)
+    tf = st["flops_per_step"] / st["ms_per_step"] / 1e9
+    return {"metric": "U-Net training TFLOP/s", "value": round(tf, 2), "unit": "TFLOP/s", "n_gpus": 1, "steps": n,
+            "warmup": max(1, a.warmup), "ms_per_step": round(st["ms_per_step"], 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
+            "data": "synthetic: Gaussian 512x256 images",
+            "config": {"workload": "unet_train: 4 levels x 32 channels, batch 8, 512x256, Adam",
+                       "flops_per_step": st["flops_per_step"]},
+            "roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                         "frac": round(tf / 2500.0, 4), "traffic": None}}
+
+
+def copy_probe():
+    """A plain device copy of known size: 256 MiB read + 256 MiB written per launch."""
+    import torch
+    x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": x.numel() * 4, "copy_probe_bytes_written": y.numel() * 4}))
+
+
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe()
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {a.gpus} is not supported by this build "
               "(shots shard across GPUs by design, but no multi-GPU path is built)", file=sys.stderr)
@@ -329,6 +374,8 @@ def main():
         out = run_forward(a, cfg)
     elif a.workload == "c4":
         out = run_learned(a)
+    elif a.workload == "unet":
+        out = run_unet(a)
     else:
         cfg = models.c2_acoustic_lsrtm(nt=a.nt or 2500, nshots=a.nshots or 26)
         cfg["seed"] = 2
@@ -338,7 +385,13 @@ def main():
     name, ncu, mem = device_info()
     out["device"] = {"arch": name, "cus": ncu, "hbm_gb": round(mem, 1)}
     if a.traffic_json and os.path.exists(a.traffic_json) and out.get("roofline"):
-        out["roofline"]["traffic"] = json.load(open(a.traffic_json)).get("traffic_bytes_per_launch")
+        tr = json.load(open(a.traffic_json))
+        cfgd = out.get("config", {})
+        tile = cfgd.get("autotuned_tile") or {}
+        if (tr.get("workload_prefix") and str(cfgd.get("workload", "")).startswith(tr["workload_prefix"])
+                and tr.get("nt") == cfgd.get("nt") and tr.get("tile") == [tile.get("x_width"), tile.get("threads"),
+                                                                           tile.get("prefetch_columns")]):
+            out["roofline"]["traffic"] = tr.get("traffic_bytes_per_launch")
     print(json.dumps(out), flush=True)
 
 
