@@ -409,9 +409,10 @@ def main():
         tr = json.load(open(a.traffic_json))
         cfgd = out.get("config", {})
         tile = cfgd.get("autotuned_tile") or {}
+        # per-launch traffic depends on the batch and the tile, not on the number of steps
         if (tr.get("workload_prefix") and str(cfgd.get("workload", "")).startswith(tr["workload_prefix"])
-                and tr.get("nt") == cfgd.get("nt") and tr.get("tile") == [tile.get("x_width"), tile.get("threads"),
-                                                                           tile.get("prefetch_columns")]):
+                and tr.get("shots") == cfgd.get("shots")
+                and tr.get("tile") == [tile.get("x_width"), tile.get("threads"), tile.get("prefetch_columns")]):
             out["roofline"]["traffic"] = tr.get("traffic_bytes_per_launch")
     print(json.dumps(out), flush=True)
 

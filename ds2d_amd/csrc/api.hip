@@ -81,6 +81,7 @@ struct ds2d_ctx {
     int per_launch = 0; // plain Born/adjoint: shots per time loop (0: all)
     int fwd_two = 0;    // autotuned: mode 2 runs the two-kernel step instead of the fused one
     int acou_fused = -1;  // fused Born/adjoint steps: -1 auto, 0 never, 1 whenever eligible
+    std::map<std::pair<long long, int>, int> acou_tuned;  // (cells, slots) -> 0 per-cell kernels, else x width
 
     ~ds2d_ctx() {
         for (auto& kv : ws)
@@ -908,10 +909,32 @@ struct AcouWS {
     int* rmap = nullptr;        // column -> trace of every slot (-1: none)
     ChunkDesc* chunks = nullptr;
     std::vector<std::pair<int, int>> gchunks;  // per launch group: first chunk, count
+    bool tune = false;          // choose per-cell vs fused (and the x width) at the first Born pass
     long long n = 0, R = 0, W = 0;
 };
 
-constexpr int ACOU_CHUNK_W = 64, ACOU_THREADS = 256, ACOU_MIN_CHUNKS = 1024;
+constexpr int ACOU_CHUNK_W = 64, ACOU_THREADS = 256, ACOU_TUNE_STEPS = 48;
+
+// Chunk list of the fused acoustic steps for x width W (per launch group).
+int acou_build_chunks(ds2d_ctx* h, const Batch& B, AcouWS& w, int W) {
+    std::vector<ChunkDesc> ch;
+    const int ns = (int)B.slots.size();
+    const int gs = h->per_launch > 0 ? std::min(h->per_launch, ns) : ns;
+    const int rows = ACOU_THREADS - 16;
+    w.gchunks.clear();
+    for (int s0 = 0; s0 < ns; s0 += gs) {
+        const int first = (int)ch.size();
+        for (int s = s0; s < std::min(ns, s0 + gs); ++s)
+            for (int x0 = 0; x0 < B.slots[s].nxp; x0 += W)
+                for (int z0 = 0; z0 < B.slots[s].nzp; z0 += rows)
+                    ch.push_back(ChunkDesc{s, x0, std::min(x0 + W, B.slots[s].nxp), z0});
+        w.gchunks.push_back({first, (int)ch.size() - first});
+    }
+    int rc;
+    if ((rc = h->fbuf("achunks", ch.size(), &w.chunks))) return rc;
+    HIP_TRY(hipMemcpyAsync(w.chunks, ch.data(), ch.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice, h->st));
+    return 0;
+}
 
 int acou_prepare(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouWS& w) {
     int rc;
@@ -946,33 +969,28 @@ int acou_prepare(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouWS
     if ((rc = h->fbuf("armap", std::max<long long>(B.total_rmap, 1), &w.rmap))) return rc;
     HIP_TRY(hipMemsetAsync(w.rmap, 0xff, B.total_rmap * 4, h->st));
     launch_fast_rmap(h->st, B.shape, w.rmap);
-    // fused Born / adjoint steps: store mode, 32-bit byte offsets within a field, and enough
-    // x-chunks for every CU (small batches run faster on the per-cell kernels)
-    long long nch = 0;
-    for (auto& S : B.slots)
-        nch += (long long)((S.nxp + ACOU_CHUNK_W - 1) / ACOU_CHUNK_W) * ((S.nzp + ACOU_THREADS - 17) / (ACOU_THREADS - 16));
-    w.fused = w.store && n <= FUSED_MAX_CELLS &&
-              (h->acou_fused == 1 || (h->acou_fused < 0 && nch >= ACOU_MIN_CHUNKS));
-    if (w.fused) {
-        std::vector<ChunkDesc> ch;
-        const int ns = (int)B.slots.size();
-        const int gs = h->per_launch > 0 ? std::min(h->per_launch, ns) : ns;
-        const int rows = ACOU_THREADS - 16;
-        w.gchunks.clear();
-        for (int s0 = 0; s0 < ns; s0 += gs) {
-            const int first = (int)ch.size();
-            for (int s = s0; s < std::min(ns, s0 + gs); ++s)
-                for (int x0 = 0; x0 < B.slots[s].nxp; x0 += ACOU_CHUNK_W)
-                    for (int z0 = 0; z0 < B.slots[s].nzp; z0 += rows)
-                        ch.push_back(ChunkDesc{s, x0, std::min(x0 + ACOU_CHUNK_W, B.slots[s].nxp), z0});
-            w.gchunks.push_back({first, (int)ch.size() - first});
+    // fused Born / adjoint steps: store mode, 32-bit byte offsets within a field; the choice
+    // (per-cell kernels or fused at some x width) is timed once per batch geometry
+    w.fused = false;
+    w.tune = false;
+    if (w.store && n <= FUSED_MAX_CELLS && h->acou_fused != 0) {
+        int width = ACOU_CHUNK_W;
+        if (h->acou_fused < 0) {
+            auto it = h->acou_tuned.find({n, (int)B.slots.size()});
+            if (it == h->acou_tuned.end()) {
+                w.tune = true;
+                width = 0;
+            } else {
+                width = it->second;
+            }
         }
-        if ((rc = h->fbuf("achunks", ch.size(), &w.chunks))) return rc;
-        HIP_TRY(hipMemcpyAsync(w.chunks, ch.data(), ch.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice, h->st));
+        if (width > 0) {
+            w.fused = true;
+            if ((rc = acou_build_chunks(h, B, w, width))) return rc;
+        }
     }
     return 0;
 }
-
 AcouFusedArgs acou_fused_args(ds2d_ctx* h, AcouWS& w, const AcouField& A, const AcouField& Bf) {
     AcouFusedArgs q{};
     q.A = A.vx;
@@ -1024,8 +1042,60 @@ std::vector<BatchShape> launch_groups(const ds2d_ctx* h, const Batch& B) {
 // Forward pass from rest.  do_bg: propagate the background (storing its
 // increment when w.store); otherwise the Born field reads the stored
 // increment.  Records: w.rec (bg), w.rec2 (born).
+void acou_fwd(ds2d_ctx* h, const Batch& B, AcouWS& w, bool do_bg, bool do_born, bool rec_bg);
+
+// Autotuner of the resident-history Born pass: time ACOU_TUNE_STEPS steps of
+// the per-cell kernels and of the fused step at x widths 16, 32 and 64, keep
+// the fastest for this batch geometry (cached in the handle).
+void acou_tune(ds2d_ctx* h, const Batch& B, AcouWS& w) {
+    hipStream_t st = h->st;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return;
+    const int steps = std::min(h->g.nt, ACOU_TUNE_STEPS);
+    auto time_pass = [&](int width) -> float {
+        (void)hipMemsetAsync(w.bn.vx, 0, (size_t)w.n * 3 * 4, st);
+        if (width > 0 && acou_build_chunks(h, B, w, width)) return 1e30f;
+        (void)hipEventRecord(e0, st);
+        if (width == 0) {
+            for (const BatchShape& sh : launch_groups(h, B))
+                for (int it = 0; it < steps; ++it) {
+                    launch_acou_v(st, sh, w.bg, w.bn, w.m, h->d_wav, it, 1.f, 0, 1);
+                    launch_acou_p(st, sh, w.bg, w.bn, w.m, w.hist, nullptr, w.rec2, it, 0, 1, 2, w.rmap);
+                }
+        } else {
+            AcouFusedArgs even = acou_fused_args(h, w, w.bn, w.bn2), odd = acou_fused_args(h, w, w.bn2, w.bn);
+            even.rec = odd.rec = w.rec2;
+            for (size_t gi = 0; gi < w.gchunks.size(); ++gi)
+                for (int it = 0; it < steps; ++it)
+                    launch_acou_born_fused(st, B.shape, (it & 1) ? odd : even, w.chunks + w.gchunks[gi].first,
+                                           w.gchunks[gi].second, w.rmap, it, ACOU_THREADS);
+        }
+        (void)hipEventRecord(e1, st);
+        float ms = 1e30f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return 1e30f;
+        return ms;
+    };
+    int best = 0;
+    float bt = time_pass(0);
+    for (int width : {16, 32, 64}) {
+        const float t = time_pass(width);
+        if (t < bt) {
+            bt = t;
+            best = width;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->acou_tuned[{w.n, (int)B.slots.size()}] = best;
+    w.fused = best > 0 && acou_build_chunks(h, B, w, best) == 0;
+}
+
 void acou_fwd(ds2d_ctx* h, const Batch& B, AcouWS& w, bool do_bg, bool do_born, bool rec_bg) {
     hipStream_t st = h->st;
+    if (w.tune && !do_bg && do_born && !rec_bg) {
+        w.tune = false;
+        acou_tune(h, B, w);
+    }
     (void)hipMemsetAsync(w.bg.vx, 0, (size_t)w.n * 6 * 4, st);  // bg + bn
     if (w.fused && !do_bg && do_born && !rec_bg) {
         // Born field only, from the resident history: one fused launch per step
