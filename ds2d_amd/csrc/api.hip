@@ -14,6 +14,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/ds2d.h"
@@ -81,7 +82,8 @@ struct ds2d_ctx {
     int per_launch = 0; // plain Born/adjoint: shots per time loop (0: all)
     int fwd_two = 0;    // autotuned: mode 2 runs the two-kernel step instead of the fused one
     int acou_fused = -1;  // fused Born/adjoint steps: -1 auto, 0 never, 1 whenever eligible
-    std::map<std::pair<long long, int>, int> acou_tuned;  // (cells, slots) -> 0 per-cell kernels, else x width
+    // (cells, slots, shots per launch) -> 0 per-cell kernels, else fused x width
+    std::map<std::tuple<long long, int, int>, int> acou_tuned;
 
     ~ds2d_ctx() {
         for (auto& kv : ws)
@@ -976,7 +978,7 @@ int acou_prepare(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouWS
     if (w.store && n <= FUSED_MAX_CELLS && h->acou_fused != 0) {
         int width = ACOU_CHUNK_W;
         if (h->acou_fused < 0) {
-            auto it = h->acou_tuned.find({n, (int)B.slots.size()});
+            auto it = h->acou_tuned.find({n, (int)B.slots.size(), h->per_launch});
             if (it == h->acou_tuned.end()) {
                 w.tune = true;
                 width = 0;
@@ -1086,7 +1088,7 @@ void acou_tune(ds2d_ctx* h, const Batch& B, AcouWS& w) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    h->acou_tuned[{w.n, (int)B.slots.size()}] = best;
+    h->acou_tuned[{w.n, (int)B.slots.size(), h->per_launch}] = best;
     w.fused = best > 0 && acou_build_chunks(h, B, w, best) == 0;
 }
 
