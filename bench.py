@@ -145,7 +145,7 @@ def run_forward(a, cfg):
         chunk, threads, depth = (int(v) for v in a.tile.split(","))
         s.set_chunk(chunk, threads, depth)
     elif a.mode == 2:  # untimed: pick the fused kernel's tile (or the two-kernel step) for this batch
-        chunk, threads, depth, _ = s.autotune(cfg["shots"], nsteps=min(60, cfg["nt"]))
+        chunk, threads, depth, _ = s.autotune(cfg["shots"], nsteps=min(120, cfg["nt"]))
     for _ in range(a.warmup):
         s.batch_forward(cfg["shots"], mode=a.mode, fetch=False)
     stats = []

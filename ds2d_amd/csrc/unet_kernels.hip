@@ -72,49 +72,32 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[m][j][r] = 0.f;
     const u32x4 zero = {0u, 0u, 0u, 0u};
-    // K runs over (tap, 16-channel step).  Operands of step k + 1 are loaded
-    // into the second register set while the MFMAs of step k run.
-    const int nck = ctot / 16, nks = 9 * nck;
-    const bf16_t* s1[WM];
-    const bf16_t* s2[WM];
-    const bf16_t* wt = nullptr;
-    struct Frag {
-        u32x4 A[WM], B[WN];
-    };
-    auto load = [&](int ks, Frag& f) {
-        const int tap = ks / nck, c = (ks - tap * nck) * 16;
-        if (c == 0) {  // first step of a tap: per-lane source pixels of this tap
-            const int dx = tap / 3 - 1, dz = tap % 3 - 1;
+    for (int tap = 0; tap < 9; ++tap) {
+        const int dx = tap / 3 - 1, dz = tap % 3 - 1;
+        const bf16_t* s1[WM];
+        const bf16_t* s2[WM];
 #pragma unroll
-            for (int m = 0; m < WM; ++m) {
-                const int xs = px[m] + dx, zs = pz[m] + dz;
-                const bool ok = pv[m] && xs >= 0 && xs < a.h && zs >= 0 && zs < a.w;
-                s1[m] = ok ? a.x1 + (((long long)pn[m] * a.h + xs) * a.w + zs) * a.c1 + 8 * kh : nullptr;
-                s2[m] = (ok && a.c2) ? a.x2 + (((long long)pn[m] * h2 + (xs >> a.up)) * w2 + (zs >> a.up)) * a.c2 +
-                                           8 * kh
-                                     : nullptr;
-            }
-            wt = a.wt + ((long long)tap * a.cout + co0 + li) * ctot + 8 * kh;
+        for (int m = 0; m < WM; ++m) {
+            const int xs = px[m] + dx, zs = pz[m] + dz;
+            const bool ok = pv[m] && xs >= 0 && xs < a.h && zs >= 0 && zs < a.w;
+            s1[m] = ok ? a.x1 + (((long long)pn[m] * a.h + xs) * a.w + zs) * a.c1 + 8 * kh : nullptr;
+            s2[m] = (ok && a.c2) ? a.x2 + (((long long)pn[m] * h2 + (xs >> a.up)) * w2 + (zs >> a.up)) * a.c2 + 8 * kh
+                                 : nullptr;
         }
+        const bf16_t* wt = a.wt + ((long long)tap * a.cout + co0 + li) * ctot + 8 * kh;
+#pragma unroll 2
+        for (int c = 0; c < ctot; c += 16) {
+            u32x4 A[WM], B[WN];
 #pragma unroll
-        for (int m = 0; m < WM; ++m)
-            f.A[m] = !s1[m] ? zero : (c < a.c1 ? ld16(s1[m] + c) : ld16(s2[m] + (c - a.c1)));
+            for (int m = 0; m < WM; ++m)
+                A[m] = !s1[m] ? zero : (c < a.c1 ? ld16(s1[m] + c) : ld16(s2[m] + (c - a.c1)));
 #pragma unroll
-        for (int j = 0; j < WN; ++j) f.B[j] = ld16(wt + (long long)j * 32 * ctot + c);
-    };
-    auto compute = [&](const Frag& f) {
+            for (int j = 0; j < WN; ++j) B[j] = ld16(wt + (long long)j * 32 * ctot + c);
 #pragma unroll
-        for (int m = 0; m < WM; ++m)
+            for (int m = 0; m < WM; ++m)
 #pragma unroll
-            for (int j = 0; j < WN; ++j) acc[m][j] = mfma(f.A[m], f.B[j], acc[m][j]);
-    };
-    Frag fx, fy;
-    load(0, fx);
-    for (int ks = 0; ks < nks; ks += 2) {
-        if (ks + 1 < nks) load(ks + 1, fy);
-        compute(fx);
-        if (ks + 2 < nks) load(ks + 2, fx);
-        if (ks + 1 < nks) compute(fy);
+                for (int j = 0; j < WN; ++j) acc[m][j] = mfma(A[m], B[j], acc[m][j]);
+        }
     }
     const int c2o = a.cout - a.split;
 #pragma unroll
@@ -312,7 +295,27 @@ __global__ void k_reduce_strided(float* part, int nchunk, int G, long long len) 
     part[(long long)g * len + e] = s;
 }
 
+// Few elements, many chunks: one block per element, fixed-order tree sum.
+__global__ __launch_bounds__(256) void k_reduce_cols(const float* part, int nchunk, long long len, float* out,
+                                                     int accumulate) {
+    __shared__ float red[256];
+    const long long e = blockIdx.x;
+    float s = 0.f;
+    for (int c = threadIdx.x; c < nchunk; c += 256) s += part[(long long)c * len + e];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[e] = accumulate ? out[e] + red[0] : red[0];
+}
+
 void launch_reduce(hipStream_t st, float* part, int nchunk, long long len, float* out, int accumulate) {
+    if (len <= 4096 && nchunk >= 64) {
+        hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)len), dim3(256), 0, st, part, nchunk, len, out, accumulate);
+        return;
+    }
     const long long eb = (len + 255) / 256;
     // enough blocks to keep every CU busy: split the chunk loop when the element count is small
     const int G = (int)std::min<long long>(nchunk, std::max<long long>(1, (2048 + eb - 1) / eb));
