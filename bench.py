@@ -363,10 +363,16 @@ def copy_probe():
     import torch
     x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
+    y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(5):
         y.copy_(x)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_read": x.numel() * 4, "copy_probe_bytes_written": y.numel() * 4}))
+    ms = e0.elapsed_time(e1) / 5
+    print(json.dumps({"copy_probe_bytes_read": x.numel() * 4, "copy_probe_bytes_written": y.numel() * 4,
+                      "ms_per_copy": round(ms, 4), "GB_per_s": round(2 * x.numel() * 4 / (ms * 1e-3) / 1e9, 1)}))
 
 
 def main():

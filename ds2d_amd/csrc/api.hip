@@ -1495,6 +1495,42 @@ int ds2d_lsrtm_learned(ds2d_t* h, ds2d_unet_t* u, int nshots, const ds2d_shot* s
     return 0;
 }
 
+// main.cpp:1232-1303 sjcgbeta / sjcglength / sjcgsolver: conjugate direction
+// cg = -g1 + beta cg with beta = max(0, min(beta_HS, beta_DY)) (0 at iter 0),
+// g0 <- g1, step alpha = 1e-4 at iter 0, else 0.1 max|m| / max|cg| (first
+// index of the largest magnitude), m += alpha cg.  FP32 sums, as there.
+int ds2d_sjcg_update(int n, float* m, float* cg, const float* g1, float* g0, int iter, float* alpha_out,
+                     float* beta_out) {
+    if (n <= 0 || !m || !cg || !g1 || !g0 || iter < 0) return fail(DS2D_EINVAL, "bad CG update arguments");
+    float beta = 0.f;
+    if (iter > 0) {
+        float a = 0.f, b = 0.f, c = 0.f;
+        for (int i = 0; i < n; ++i) {
+            a += g1[i] * (g1[i] - g0[i]);
+            b += cg[i] * (g1[i] - g0[i]);
+            c += g1[i] * g1[i];
+        }
+        const float hs = std::fabs(b) > 0.f ? a / b : 0.f;
+        const float dy = std::fabs(b) > 0.f ? c / b : 0.f;
+        beta = std::max(0.f, std::min(hs, dy));
+    }
+    for (int i = 0; i < n; ++i) cg[i] = -1.f * g1[i] + beta * cg[i];
+    std::memcpy(g0, g1, (size_t)n * sizeof(float));
+    float alpha = 1e-4f;
+    if (iter > 0) {
+        int is = 0, ix = 0;
+        for (int i = 0; i < n; ++i) {
+            if (std::fabs(cg[i]) > std::fabs(cg[is])) is = i;
+            if (std::fabs(m[i]) > std::fabs(m[ix])) ix = i;
+        }
+        alpha = 0.1f * std::fabs(m[ix]) / std::fabs(cg[is]);
+    }
+    for (int i = 0; i < n; ++i) m[i] = 1.f * m[i] + alpha * cg[i];
+    if (alpha_out) *alpha_out = alpha;
+    if (beta_out) *beta_out = beta;
+    return 0;
+}
+
 // laplace.cpp:82-106: (1/dx^2)(5-point Laplacian) on [2, n-2), zero frame.
 int ds2d_laplace_filter(int nz, int nx, int order, float dx, float* dat) {
     (void)order;

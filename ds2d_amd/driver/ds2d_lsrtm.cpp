@@ -368,6 +368,11 @@ int run_lsrtm(const Params& p) {
     std::vector<float> ipp(A, 0.f), ips(A, 0.f), s0p(A, 0.f), s0s(A, 0.f), r0p(A, 0.f), r0s(A, 0.f);
     std::vector<float> gpp(A), gps(A), il(A), bpp(A), bps(A), b2pp(A), b2ps(A), sump(A), sums(A);
     std::vector<float> mis(p.maxiter);
+    // opt = adam (main.cpp:1092-1130, the reference's active update) or
+    // cg (sjcgsolver, main.cpp:1278, present in the reference but not called)
+    const bool use_cg = p.get("opt", "adam") == "cg";
+    if (!use_cg && p.get("opt", "adam") != "adam") die("opt must be adam or cg");
+    std::vector<float> cgp(use_cg ? A : 0), cgs(use_cg ? A : 0), g0p(use_cg ? A : 0), g0s(use_cg ? A : 0);
     const size_t R = (size_t)nt;
     for (int iter = 0; iter < p.maxiter; ++iter) {
         check(ds2d_set_image(h, ipp.data(), ips.data()), "set_image");
@@ -384,8 +389,19 @@ int run_lsrtm(const Params& p) {
         if (iter == 0) misfit = 0;  // main.cpp:1065-1067
         mis[iter] = 1.0f - (float)std::fabs((1.0 / table.size()) * misfit);
         std::printf("iter = %d  misfit = %g  rmsmisfit = %g\n", iter, misfit, mis[iter]);
-        adam((int)A, iter, gpp.data(), bpp.data(), b2pp.data(), s0p.data(), r0p.data(), ipp.data(), sump.data());
-        adam((int)A, iter, gps.data(), bps.data(), b2ps.data(), s0s.data(), r0s.data(), ips.data(), sums.data());
+        if (use_cg) {  // the reference's nonlinear CG (sjcgsolver) on g + b_g + b_g2
+            for (size_t i = 0; i < A; ++i) {
+                sump[i] = gpp[i] + bpp[i] + b2pp[i];
+                sums[i] = gps[i] + bps[i] + b2ps[i];
+            }
+            check(ds2d_sjcg_update((int)A, ipp.data(), cgp.data(), sump.data(), g0p.data(), iter, nullptr, nullptr),
+                  "cg pp");
+            check(ds2d_sjcg_update((int)A, ips.data(), cgs.data(), sums.data(), g0s.data(), iter, nullptr, nullptr),
+                  "cg ps");
+        } else {
+            adam((int)A, iter, gpp.data(), bpp.data(), b2pp.data(), s0p.data(), r0p.data(), ipp.data(), sump.data());
+            adam((int)A, iter, gps.data(), bps.data(), b2ps.data(), s0s.data(), r0s.data(), ips.data(), sums.data());
+        }
         const struct { const char* tag; const float* v; } outs[] = {
             {"sum_bgrad2_pp", b2pp.data()}, {"sum_bgrad2_ps", b2ps.data()}, {"all_grad_pp", sump.data()},
             {"all_grad_ps", sums.data()},   {"sum_image_pp", ipp.data()},   {"sum_image_ps", ips.data()}};

@@ -93,6 +93,7 @@ SIGNATURES = {
     "ds2d_unet_set_data": (C.c_int, [_H, C.c_int, C.c_int, C.c_int, _F, _F]),
     "ds2d_unet_train": (C.c_int, [_H, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _F, C.POINTER(UnetStats)]),
     "ds2d_conv3x3_bf16": (C.c_int, [C.c_int] * 8 + [_F, _F, _F, _F, _F, C.c_int, C.c_int, _F, _D]),
+    "ds2d_sjcg_update": (C.c_int, [C.c_int, _F, _F, _F, _F, C.c_int, _F, _F]),
     "ds2d_laplace_filter": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_float, _F]),
     "ds2d_laplace_band": (C.c_int, [C.c_int, C.c_int, _F]),
     "ds2d_ricker": (C.c_int, [C.c_int, C.c_float, C.c_float, _F]),
@@ -405,3 +406,13 @@ def conv3x3_bf16(op, x1, wt, *, x2=None, up=0, dy=None, bias=None, relu=False, r
                                  fptr(wt), fptr(ba) if ba is not None else None, int(relu), repeat, fptr(out),
                                  C.byref(ms)))
     return out, ms.value
+
+
+def sjcg_update(m, cg, g1, g0, it):
+    """The reference's nonlinear CG update (in place on m, cg, g0); returns (alpha, beta)."""
+    lib = load()
+    a, b = C.c_float(0), C.c_float(0)
+    for v in (m, cg, g1, g0):
+        assert v.dtype == np.float32 and v.flags["C_CONTIGUOUS"]
+    _check(lib.ds2d_sjcg_update(m.size, fptr(m), fptr(cg), fptr(g1), fptr(g0), int(it), C.byref(a), C.byref(b)))
+    return a.value, b.value

@@ -35,6 +35,7 @@
  *   ds2d_laplace_filter    laplace_filter_2d(nz,nx,order,dx,dat) laplace.cpp:82-106
  *   ds2d_laplace_band      laplace_filter_2d(nz,nx,dat)          laplace.cpp:3-57
  *   ds2d_ricker            ricker1                               main.cpp:461-473
+ *   ds2d_sjcg_update       sjcgbeta + sjcglength + sjcgsolver    main.cpp:1232-1303
  */
 #ifndef DS2D_H
 #define DS2D_H
@@ -220,6 +221,12 @@ int ds2d_set_batch(ds2d_t* h, int shots_per_launch);
  * (x-marching workgroups): -1 = auto (when the batch gives every CU several
  * x-chunks; default), 0 = never (per-cell kernels), 1 = always. */
 int ds2d_set_fused(ds2d_t* h, int mode);
+
+/* Host-side nonlinear CG update of the reference (sjcgsolver, main.cpp:1232-1303):
+ * cg = -g1 + beta cg, beta = max(0, min(HS, DY)) (0 at iter 0); g0 <- g1;
+ * alpha = 1e-4 at iter 0, else 0.1 max|m| / max|cg|; m += alpha cg. */
+int ds2d_sjcg_update(int n, float* m, float* cg, const float* g1, float* g0, int iter, float* alpha,
+                     float* beta);
 
 int ds2d_laplace_filter(int nz, int nx, int order, float dx, float* dat);
 int ds2d_laplace_band(int nz, int nx, float* dat);
