@@ -290,6 +290,7 @@ int stats_end(ds2d_ctx* h, ds2d_stats* out, long long cell_updates, long long la
     h->last.cell_updates = cell_updates;
     h->last.stencil_launches = launches;
     h->last.alg_bytes = alg_bytes;
+    h->last.aux_alg_bytes = h->last.aux_kernel_avg_ms = 0;
     if (out) *out = h->last;
     return 0;
 }
@@ -697,14 +698,14 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
         if (h->use_graph) {
             // whole time loop as one graph; per-launch time = loop time / nt
             if ((rc = run_steps(h, 0, nt, 1, [&](int it) {
-                     launch_fused_step(st, sh, (it & 1) ? odd : even, d_ch, (int)ch.size(), rmap, it, T, D);
+                     launch_fused_step(st, sh, (it & 1) ? odd : even, d_ch, (int)ch.size(), rmap, it, T, D, B.any_pml);
                  })))
                 return rc;
         } else {
             for (int it = 0; it < nt; ++it) {
                 const bool tk = smp.take(it);
                 if (tk) smp.mark(st, 0);
-                launch_fused_step(st, sh, (it & 1) ? odd : even, d_ch, (int)ch.size(), rmap, it, T, D);
+                launch_fused_step(st, sh, (it & 1) ? odd : even, d_ch, (int)ch.size(), rmap, it, T, D, B.any_pml);
                 if (tk) { smp.mark(st, 1); smp.mark(st, 2); }
             }
         }
@@ -754,9 +755,13 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
         const bool s_dom = ks >= kv && bytes_s > 0;
         h->last.kernel_avg_ms = s_dom ? ks : kv;
         h->last.alg_bytes = s_dom ? bytes_s : bytes_v;
+        // the other kernel of a two-kernel step
+        h->last.aux_kernel_avg_ms = bytes_s > 0 && bytes_v > 0 ? (s_dom ? kv : ks) : 0;
+        h->last.aux_alg_bytes = bytes_s > 0 && bytes_v > 0 ? (s_dom ? bytes_v : bytes_s) : 0;
         h->last.kernel_samples = smp.n;
         if (smp.n == 0 && launches > 0) {  // graph mode: one kernel per step, no gaps
             h->last.kernel_avg_ms = h->last.stencil_ms / (double)launches;
+            h->last.aux_kernel_avg_ms = h->last.aux_alg_bytes = 0;
             h->last.kernel_samples = launches;
         }
         if (stats) *stats = h->last;

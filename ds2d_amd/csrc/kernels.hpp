@@ -150,7 +150,9 @@ constexpr long long FUSED_MAX_CELLS = (1LL << 30) - (1LL << 20);
 constexpr int FUSED_HALO = 8;
 bool fused_threads_ok(int threads);
 // depth: columns of operands in flight per lane (2 or 4)
+// pml = false compiles the damping-strip code out; valid only when no window
+// of the batch has a PML (boundary != 0 for every slot).
 void launch_fused_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, const ChunkDesc* chunks,
-                       int nchunks, const int* rmap, int it, int threads, int depth);
+                       int nchunks, const int* rmap, int it, int threads, int depth, bool pml);
 
 }  // namespace ds2d

@@ -48,7 +48,7 @@ struct Col {
 };
 }  // namespace
 
-template <int T, int PF>
+template <int T, int PF, bool PML>
 __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* __restrict__ chunks, int nchunks,
                                                   int per_xcd, const int* __restrict__ rmap,
                                                   const SlotDesc* __restrict__ slots, float idx, float idz, float dt,
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
         o.mu = ldc(Pmu, cs);
         // damping-strip operands: one branch per side (vupd/supd imply an in-window cell)
         o.dxv = o.dzv = o.vxx = o.vxz = o.vzx = o.vzz = 0.f;
-        if (vupd && damp_cell_f(S, j, z)) {
+        if (PML && vupd && damp_cell_f(S, j, z)) {
             const uint32_t c = zo4 + (uint32_t)j * colb;
             o.dxv = at(a.dampx, c);
             o.dzv = at(a.dampz, c);
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
             o.vzz = at(split(a.A, 3), c);
         }
         o.dxs = o.dzs = o.sxx = o.sxz = o.szx = o.szz = o.sex = o.sez = 0.f;
-        if (supd && damp_cell_f(S, cs, z)) {
+        if (PML && supd && damp_cell_f(S, cs, z)) {
             const uint32_t c = zo4 + (uint32_t)cs * colb;
             o.dxs = at(a.dampx, c);
             o.dzs = at(a.dampz, c);
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
             if (z == S.isz) ax += w * tap(S.isx - j);
             if (j == S.isx) cz += w * tap(S.isz - z);
             const float bx = o.bx;
-            if (damp_cell_f(S, j, z)) {
+            if (PML && damp_cell_f(S, j, z)) {
                 const float d1 = 1.f - dt * o.dxv / 2, d2 = 1.f + dt * o.dxv / 2;
                 const float d3 = 1.f - dt * o.dzv / 2, d4 = 1.f + dt * o.dzv / 2;
                 const float vxx = (d1 * o.vxx + bx * idx * ax) / d2;
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
                 const float dvxz = (c1 * (L2[L + 1] - L2[L]) + c2 * (L2[L + 2] - L2[L - 1]) +
                                     c3 * (L2[L + 3] - L2[L - 2]) + c4 * (L2[L + 4] - L2[L - 3])) * idz;
                 const float l = o.lam, mu = o.mu, l2m = l + 2 * mu;
-                if (damp_cell_f(S, cs, z)) {
+                if (PML && damp_cell_f(S, cs, z)) {
                     const float d1 = 1.f - dt * o.dxs / 2, d2 = 1.f + dt * o.dxs / 2;
                     const float d3 = 1.f - dt * o.dzs / 2, d4 = 1.f + dt * o.dzs / 2;
                     const float a1 = (d1 * o.sxx + l2m * dvx) / d2;
@@ -291,28 +291,32 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
         }
     };
 
-    // even number of columns: an extra trailing column stores nothing
     // a multiple of PF columns: up to PF-1 extra trailing columns store nothing
     const int jend = j0 + (x1 + 4 - j0 + PF - 1) / PF * PF;
+    // Without damping strips the last pass prefetches nothing (a uniform
+    // branch; with them, the unguarded march measured faster).
+    auto more = [&](int j) { return PML || j + PF < jend; };
     if (PF == 2) {
         Col cA = fetch(j0), cB = fetch(j0 + 1);
         for (int j = j0; j < jend; j += 2) {
+            const bool m = more(j);
             column(j, cA, 0);
-            cA = fetch(j + 2);
+            if (m) cA = fetch(j + 2);
             column(j + 1, cB, 1);
-            cB = fetch(j + 3);
+            if (m) cB = fetch(j + 3);
         }
     } else {
         Col cA = fetch(j0), cB = fetch(j0 + 1), cC = fetch(j0 + 2), cD = fetch(j0 + 3);
         for (int j = j0; j < jend; j += 4) {
+            const bool m = more(j);
             column(j, cA, 0);
-            cA = fetch(j + 4);
+            if (m) cA = fetch(j + 4);
             column(j + 1, cB, 1);
-            cB = fetch(j + 5);
+            if (m) cB = fetch(j + 5);
             column(j + 2, cC, 0);
-            cC = fetch(j + 6);
+            if (m) cC = fetch(j + 6);
             column(j + 3, cD, 1);
-            cD = fetch(j + 7);
+            if (m) cD = fetch(j + 7);
         }
     }
 }
@@ -320,28 +324,34 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
 bool fused_threads_ok(int threads) { return threads == 256 || threads == 512 || threads == 768; }
 
 void launch_fused_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, const ChunkDesc* chunks,
-                       int nchunks, const int* rmap, int it, int threads, int depth) {
+                       int nchunks, const int* rmap, int it, int threads, int depth, bool pml) {
     const int per_xcd = (nchunks + 7) / 8;
     const dim3 grid(per_xcd * 8);
     const float idx = 1.f / b.dx, idz = 1.f / b.dz;
-#define DS2D_FUSED_LAUNCH(TT, PP)                                                                               \
-    hipLaunchKernelGGL((k_fused_step<TT, PP>), grid, dim3(TT), 0, st, a, chunks, nchunks, per_xcd, rmap, b.slots, \
+#define DS2D_FUSED_LAUNCH(TT, PP, MM)                                                                             \
+    hipLaunchKernelGGL((k_fused_step<TT, PP, MM>), grid, dim3(TT), 0, st, a, chunks, nchunks, per_xcd, rmap, b.slots, \
                        idx, idz, b.dt, b.nt, it)
-    if (depth == 4) {
-        if (threads == 256)
-            DS2D_FUSED_LAUNCH(256, 4);
-        else if (threads == 512)
-            DS2D_FUSED_LAUNCH(512, 4);
-        else
-            DS2D_FUSED_LAUNCH(768, 4);
+#define DS2D_FUSED_THREADS(PP, MM)       \
+    if (threads == 256)                  \
+        DS2D_FUSED_LAUNCH(256, PP, MM);  \
+    else if (threads == 512)             \
+        DS2D_FUSED_LAUNCH(512, PP, MM);  \
+    else                                 \
+        DS2D_FUSED_LAUNCH(768, PP, MM);
+    if (pml) {
+        if (depth == 4) {
+            DS2D_FUSED_THREADS(4, true)
+        } else {
+            DS2D_FUSED_THREADS(2, true)
+        }
     } else {
-        if (threads == 256)
-            DS2D_FUSED_LAUNCH(256, 2);
-        else if (threads == 512)
-            DS2D_FUSED_LAUNCH(512, 2);
-        else
-            DS2D_FUSED_LAUNCH(768, 2);
+        if (depth == 4) {
+            DS2D_FUSED_THREADS(4, false)
+        } else {
+            DS2D_FUSED_THREADS(2, false)
+        }
     }
+#undef DS2D_FUSED_THREADS
 #undef DS2D_FUSED_LAUNCH
 }
 

@@ -96,6 +96,8 @@ typedef struct {
     double alg_bytes;          /* algorithmic HBM bytes of one such launch         */
     double kernel_avg_ms;      /* its average launch duration (sampled events)     */
     long long kernel_samples;  /* launches that were bracketed by events           */
+    double aux_alg_bytes;      /* the other kernel of a two-kernel step, per launch */
+    double aux_kernel_avg_ms;  /* (0 when the step is one kernel)                  */
 } ds2d_stats;
 
 const char* ds2d_last_error(void);
