@@ -1336,6 +1336,292 @@ int ds2d_set_batch(ds2d_t* h, int shots_per_launch) {
 
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// Plain-L2 elastic Born pair (PP/PS) and its CG LSRTM
+// ---------------------------------------------------------------------------
+namespace {
+
+struct EbWS {
+    EbField bg{}, bn{}, lm{};
+    EbMed m{};
+    float *gpp = nullptr, *gps = nullptr;  // per-slot padded gradients (n each)
+    float* win = nullptr;                  // per-slot window images, PP then PS (2 W)
+    float* rec = nullptr;                  // scattered records: z then x (2 R)
+    float* res = nullptr;                  // residual records: z then x (2 R)
+    float* hist = nullptr;                 // background strains exx | ezz | exz (3 H)
+    bool store = false;
+    int* rmap = nullptr;
+    long long n = 0, R = 0, W = 0, H = 0;
+};
+
+EbField eb_field(float* base, long long n) {
+    return EbField{base, base + n, base + 2 * n, base + 3 * n, base + 4 * n};
+}
+
+int eb_begin(ds2d_ctx* h, int nshots, const ds2d_shot* shots, Batch& B) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!h->have_model || !h->d_vs) return fail(DS2D_ESTATE, "no elastic model (vp, vs)");
+    if (h->g.physics != 0) return fail(DS2D_EINVAL, "the elastic Born pair needs physics = 0");
+    if (!shots) return fail(DS2D_EINVAL, "null shots");
+    for (int s = 0; s < nshots; ++s)
+        if (shots[s].boundary != 1)
+            return fail(DS2D_EINVAL, "plain elastic Born/adjoint needs the random boundary (shot %d)", s);
+    return build_batch(h, nshots, shots, B);
+}
+
+// Medium (with the PP/PS image windows when given) and workspace.
+int eb_set_image(ds2d_ctx* h, const Batch& B, const float* pp_dev, const float* ps_dev, EbWS& w) {
+    Med m;
+    int rc;
+    if ((rc = alloc_med(h, w.n, m))) return rc;
+    SetupArgs a = setup_args(h, false);
+    a.img_pp = pp_dev;
+    a.img_ps = ps_dev;
+    launch_setup(h->st, B.shape, a, m);
+    if (!pp_dev) HIP_TRY(hipMemsetAsync(m.dmp, 0, (size_t)w.n * 4, h->st));
+    if (!ps_dev) HIP_TRY(hipMemsetAsync(m.dms, 0, (size_t)w.n * 4, h->st));
+    w.m = EbMed{m.rho, m.lam, m.mu, m.dmp, m.dms};
+    return 0;
+}
+
+int eb_prepare(ds2d_ctx* h, const Batch& B, const float* pp_dev, const float* ps_dev, bool want_store, EbWS& w) {
+    int rc;
+    const long long n = B.total_cells;
+    w.n = n;
+    w.R = std::max<long long>(B.total_rec, 1);
+    w.W = std::max<long long>(B.total_win, 1);
+    w.H = std::max<long long>(B.total_hist, 1);
+    float* base;
+    if ((rc = h->fbuf("eborn", (size_t)n * 17, &base))) return rc;
+    w.bg = eb_field(base, n);
+    w.bn = eb_field(base + 5 * n, n);
+    w.lm = eb_field(base + 10 * n, n);
+    w.gpp = base + 15 * n;
+    w.gps = base + 16 * n;
+    if ((rc = eb_set_image(h, B, pp_dev, ps_dev, w))) return rc;
+    if ((rc = h->fbuf("ebrec", (size_t)w.R * 4, &w.rec))) return rc;
+    w.res = w.rec + 2 * w.R;
+    if ((rc = h->fbuf("ebwin", (size_t)w.W * 2, &w.win))) return rc;
+    // resident background strains when they fit (3 floats per physical cell and step)
+    const double hist_gb = (double)B.total_hist * 3 * 4 / 1e9;
+    const double budget = h->g.hbm_budget_gb > 0 ? h->g.hbm_budget_gb : 64.0;
+    w.store = want_store && (h->g.bg_mode == 1 || (h->g.bg_mode == 0 && hist_gb <= budget));
+    if (w.store && (rc = h->fbuf("ehist", (size_t)w.H * 3, &w.hist))) return rc;
+    if ((rc = h->fbuf("ebrmap", std::max<long long>(B.total_rmap, 1), &w.rmap))) return rc;
+    HIP_TRY(hipMemsetAsync(w.rmap, 0xff, B.total_rmap * 4, h->st));
+    launch_fast_rmap(h->st, B.shape, w.rmap);
+    return 0;
+}
+
+// Forward pass from rest into w.rec (z then x).  do_bg: propagate the
+// background (storing its strains when w.store); otherwise the scattered
+// field reads the stored strains.
+void eb_fwd(ds2d_ctx* h, const Batch& B, EbWS& w, bool do_bg) {
+    hipStream_t st = h->st;
+    (void)hipMemsetAsync(w.bg.vx, 0, (size_t)w.n * 10 * 4, st);  // bg + bn
+    const int hmode = do_bg ? (w.store ? 1 : 0) : 2;
+    for (const BatchShape& sh : launch_groups(h, B))
+        for (int it = 0; it < h->g.nt; ++it) {
+            launch_eb_v(st, sh, w.bg, w.bn, w.m.rho, h->wav[it], 1.f, do_bg, 1);
+            launch_eb_s(st, sh, w.bg, w.bn, w.m, w.hist, w.H, it, 1.f, do_bg, 1, hmode, w.rmap, w.rec, w.rec + w.R);
+        }
+}
+
+// Background only (fills the strain history), from rest to t = nt.
+void eb_bg(ds2d_ctx* h, const Batch& B, EbWS& w) {
+    hipStream_t st = h->st;
+    (void)hipMemsetAsync(w.bg.vx, 0, (size_t)w.n * 5 * 4, st);
+    for (const BatchShape& sh : launch_groups(h, B))
+        for (int it = 0; it < h->g.nt; ++it) {
+            launch_eb_v(st, sh, w.bg, w.bn, w.m.rho, h->wav[it], 1.f, 1, 0);
+            launch_eb_s(st, sh, w.bg, w.bn, w.m, w.hist, w.H, it, 1.f, 1, 0, w.store ? 1 : 0, w.rmap, nullptr,
+                        nullptr);
+        }
+}
+
+// Backward pass (residual in w.res): gradients in w.gpp / w.gps.  Without the
+// history the background, left at t = nt by the forward pass, is reversed.
+void eb_bwd(ds2d_ctx* h, const Batch& B, EbWS& w) {
+    hipStream_t st = h->st;
+    (void)hipMemsetAsync(w.lm.vx, 0, (size_t)w.n * 7 * 4, st);  // lm + gpp + gps
+    const int nt = h->g.nt;
+    for (const BatchShape& sh : launch_groups(h, B))
+        for (int it = nt - 2; it >= 0; --it) {
+            if (!w.store) {  // background of step it from that of step it + 1
+                launch_eb_s(st, sh, w.bg, w.bn, w.m, nullptr, w.H, it + 1, -1.f, 1, 0, 0, w.rmap, nullptr, nullptr);
+                launch_eb_v(st, sh, w.bg, w.bn, w.m.rho, h->wav[it + 1], -1.f, 1, 0);
+            }
+            launch_eb_adj_v(st, sh, w.lm, w.m, w.rmap, w.res, w.res + w.R, it + 1);
+            launch_eb_adj_s(st, sh, w.lm, w.bg, w.m, w.store ? w.hist : nullptr, w.H, w.gpp, w.gps, it);
+        }
+}
+
+// Padded per-slot gradients -> global PP then PS (deterministic slot order), overwrite.
+int eb_grad_to_global(ds2d_ctx* h, const Batch& B, EbWS& w, float* d_glob) {
+    for (size_t s = 0; s < B.slots.size(); ++s) {
+        const SlotDesc& S = B.slots[s];
+        const size_t o = S.off + (size_t)S.pml * S.nzp + S.pml;
+        HIP_TRY(hipMemcpy2DAsync(w.win + S.win_off, (size_t)S.nz * 4, w.gpp + o, (size_t)S.nzp * 4, (size_t)S.nz * 4,
+                                 S.nx, hipMemcpyDeviceToDevice, h->st));
+        HIP_TRY(hipMemcpy2DAsync(w.win + w.W + S.win_off, (size_t)S.nz * 4, w.gps + o, (size_t)S.nzp * 4,
+                                 (size_t)S.nz * 4, S.nx, hipMemcpyDeviceToDevice, h->st));
+    }
+    const long long A = (long long)h->g.allnx * h->g.allnz;
+    HIP_TRY(hipMemsetAsync(d_glob, 0, 2 * A * 4, h->st));
+    launch_gather(h->st, B.shape, w.win, w.W, d_glob, A, 2, h->g.allnx, h->g.allnz);
+    return 0;
+}
+
+// Padded cell-updates of one pass over the batch per propagated wavefield.
+long long eb_cells(const Batch& B, int nt) {
+    long long cu = 0;
+    for (auto& S : B.slots) cu += (long long)S.nxp * S.nzp;
+    return cu * nt;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ds2d_eborn_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* img_pp, const float* img_ps,
+                       float* rec_z, float* rec_x, ds2d_stats* stats) {
+    int rc;
+    Batch B;
+    if ((rc = eb_begin(h, nshots, shots, B))) return rc;
+    if (!img_pp || !img_ps || !rec_z || !rec_x) return fail(DS2D_EINVAL, "null image or record pointer");
+    const size_t A = (size_t)h->g.allnx * h->g.allnz;
+    float* d_img;
+    if ((rc = h->fbuf("ebimg", 2 * A, &d_img))) return rc;
+    HIP_TRY(hipMemcpyAsync(d_img, img_pp, A * 4, hipMemcpyHostToDevice, h->st));
+    HIP_TRY(hipMemcpyAsync(d_img + A, img_ps, A * 4, hipMemcpyHostToDevice, h->st));
+    EbWS w;
+    if ((rc = eb_prepare(h, B, d_img, d_img + A, false, w))) return rc;
+    stats_begin(h);
+    stats_loop_begin(h);
+    eb_fwd(h, B, w, true);
+    stats_loop_end(h);
+    if ((rc = stats_end(h, stats, 2 * eb_cells(B, h->g.nt), 2LL * h->g.nt, 0))) return rc;
+    HIP_TRY(hipMemcpyAsync(rec_z, w.rec, B.total_rec * 4, hipMemcpyDeviceToHost, h->st));
+    HIP_TRY(hipMemcpyAsync(rec_x, w.rec + w.R, B.total_rec * 4, hipMemcpyDeviceToHost, h->st));
+    HIP_TRY(hipStreamSynchronize(h->st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int ds2d_eborn_adjoint(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* res_z, const float* res_x,
+                       float* g_pp, float* g_ps, ds2d_stats* stats) {
+    int rc;
+    Batch B;
+    if ((rc = eb_begin(h, nshots, shots, B))) return rc;
+    if (!res_z || !res_x || !g_pp || !g_ps) return fail(DS2D_EINVAL, "null residual or gradient pointer");
+    EbWS w;
+    if ((rc = eb_prepare(h, B, nullptr, nullptr, true, w))) return rc;
+    stats_begin(h);
+    stats_loop_begin(h);
+    eb_bg(h, B, w);
+    HIP_TRY(hipMemcpyAsync(w.res, res_z, B.total_rec * 4, hipMemcpyHostToDevice, h->st));
+    HIP_TRY(hipMemcpyAsync(w.res + w.R, res_x, B.total_rec * 4, hipMemcpyHostToDevice, h->st));
+    eb_bwd(h, B, w);
+    stats_loop_end(h);
+    const size_t A = (size_t)h->g.allnx * h->g.allnz;
+    float* d_g;
+    if ((rc = h->fbuf("ebglob", 2 * A, &d_g))) return rc;
+    if ((rc = eb_grad_to_global(h, B, w, d_g))) return rc;
+    if ((rc = stats_end(h, stats, eb_cells(B, h->g.nt) * (w.store ? 2 : 3), 2LL * h->g.nt, 0))) return rc;
+    HIP_TRY(hipMemcpyAsync(g_pp, d_g, A * 4, hipMemcpyDeviceToHost, h->st));
+    HIP_TRY(hipMemcpyAsync(g_ps, d_g + A, A * 4, hipMemcpyDeviceToHost, h->st));
+    HIP_TRY(hipStreamSynchronize(h->st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// CGLS on ||L [pp; ps] - [d_z; d_x]||^2 with the plain elastic Born pair.
+// Store mode: one background pass fills the strain history, then every
+// iteration is one scattered-field pass and one adjoint pass; otherwise each
+// pass carries the background (forward) or reverses it (adjoint).
+int ds2d_elsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_z, const float* d_x, int niter,
+                   float* img_pp, float* img_ps, double* resid, ds2d_stats* stats) {
+    int rc;
+    Batch B;
+    if ((rc = eb_begin(h, nshots, shots, B))) return rc;
+    if (niter < 0 || !d_z || !d_x || !img_pp || !img_ps) return fail(DS2D_EINVAL, "bad CG arguments");
+    const long long A = (long long)h->g.allnx * h->g.allnz;
+    hipStream_t st = h->st;
+    EbWS w;
+    float *x, *p, *s, *d, *r;
+    if ((rc = h->fbuf("ecg_x", 2 * A, &x)) || (rc = h->fbuf("ecg_p", 2 * A, &p)) ||
+        (rc = h->fbuf("ecg_s", 2 * A, &s)))
+        return rc;
+    const long long R = std::max<long long>(B.total_rec, 1);
+    if ((rc = h->fbuf("ecg_d", 2 * R, &d)) || (rc = h->fbuf("ecg_r", 2 * R, &r))) return rc;
+    HIP_TRY(hipMemcpyAsync(x, img_pp, A * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(x + A, img_ps, A * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(d, 0, 2 * R * 4, st));
+    HIP_TRY(hipMemcpyAsync(d, d_z, B.total_rec * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d + R, d_x, B.total_rec * 4, hipMemcpyHostToDevice, st));
+    if ((rc = eb_prepare(h, B, x, x + A, true, w))) return rc;
+    long long props = 0;
+    stats_begin(h);
+    stats_loop_begin(h);
+    // r = d - L x (this pass also fills the history in store mode)
+    eb_fwd(h, B, w, true);
+    props += 2;
+    HIP_TRY(hipMemcpyAsync(r, d, 2 * R * 4, hipMemcpyDeviceToDevice, st));
+    launch_axpby(st, r, w.rec, 2 * R, -1.f, 1.f);
+    const double dd = host_dot(h, d, d, 2 * R, &rc);
+    if (rc) return rc;
+    if (dd <= 0) return fail(DS2D_EINVAL, "observed data are all zero");
+    double rr = host_dot(h, r, r, 2 * R, &rc);
+    if (rc) return rc;
+    if (resid) resid[0] = std::sqrt(rr / dd);
+    // s = L^T r
+    auto adjoint = [&]() -> int {
+        HIP_TRY(hipMemcpyAsync(w.res, r, 2 * R * 4, hipMemcpyDeviceToDevice, st));
+        eb_bwd(h, B, w);
+        props += w.store ? 1 : 2;
+        return eb_grad_to_global(h, B, w, s);
+    };
+    if ((rc = adjoint())) return rc;
+    HIP_TRY(hipMemcpyAsync(p, s, 2 * A * 4, hipMemcpyDeviceToDevice, st));
+    double gamma = host_dot(h, s, s, 2 * A, &rc);
+    if (rc) return rc;
+    for (int k = 0; k < niter; ++k) {
+        if (gamma <= 0) {
+            if (resid)
+                for (int j = k; j < niter; ++j) resid[j + 1] = std::sqrt(rr / dd);
+            break;
+        }
+        // q = L p (w.rec)
+        if ((rc = eb_set_image(h, B, p, p + A, w))) return rc;
+        eb_fwd(h, B, w, !w.store);
+        props += w.store ? 1 : 2;
+        const double qq = host_dot(h, w.rec, w.rec, 2 * R, &rc);
+        if (rc) return rc;
+        const double alpha = qq > 0 ? gamma / qq : 0.0;
+        launch_axpby(st, x, p, 2 * A, (float)alpha, 1.f);
+        launch_axpby(st, r, w.rec, 2 * R, (float)-alpha, 1.f);
+        rr = host_dot(h, r, r, 2 * R, &rc);
+        if (rc) return rc;
+        if (resid) resid[k + 1] = std::sqrt(rr / dd);
+        if ((rc = adjoint())) return rc;
+        const double gnew = host_dot(h, s, s, 2 * A, &rc);
+        if (rc) return rc;
+        const double beta = gamma > 0 ? gnew / gamma : 0.0;
+        launch_axpby(st, p, s, 2 * A, 1.f, (float)beta);  // p = s + beta p
+        gamma = gnew;
+    }
+    stats_loop_end(h);
+    if ((rc = stats_end(h, stats, eb_cells(B, h->g.nt) * props, 0, 0))) return rc;
+    HIP_TRY(hipMemcpyAsync(img_pp, x, A * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(img_ps, x + A, A * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // extern "C"
+
 namespace {
 
 // Learned preconditioner z = rms(g) * U(g / rms(g)): positively homogeneous,

@@ -126,6 +126,32 @@ void launch_fast_params(hipStream_t st, const float* rho, const float* lam, cons
                         float* mdt, long long n, float dt);
 void launch_fast_rmap(hipStream_t st, const BatchShape& b, int* rmap);
 
+// --- plain-L2 elastic Born pair (kernels_eborn.hip) --------------------------
+// Unsplit elastic wavefield (random boundary, no damping).
+struct EbField {
+    float *vx, *vz, *txx, *tzz, *txz;
+};
+struct EbMed {
+    const float *rho, *lam, *mu, *dmp, *dms;  // dmp/dms: padded PP/PS image windows
+};
+// Forward step: k_eb_v (background with source sample w and/or scattered
+// velocity), then k_eb_s (stresses, Born source, recorder).  sign = -1 undoes a
+// background step.  hist_mode: 1 store the background strains (3 arrays of
+// hstride floats), 2 read them (no background).
+void launch_eb_v(hipStream_t st, const BatchShape& b, EbField bg, EbField born, const float* rho, float w,
+                 float sign, int do_bg, int do_born);
+void launch_eb_s(hipStream_t st, const BatchShape& b, EbField bg, EbField born, EbMed m, float* hist,
+                 long long hstride, int it, float sign, int do_bg, int do_born, int hist_mode, const int* rmap,
+                 float* rec_z, float* rec_x);
+// Adjoint step: k_eb_adj_v (receiver injection of step it_inj, -1 none, and the
+// transpose of the stress update), then k_eb_adj_s (transpose of the velocity
+// update and the gradient of step it; strains from hist, or from bg when
+// hist is null).
+void launch_eb_adj_v(hipStream_t st, const BatchShape& b, EbField lm, EbMed m, const int* rmap, const float* res_z,
+                     const float* res_x, int it_inj);
+void launch_eb_adj_s(hipStream_t st, const BatchShape& b, EbField lm, EbField bg, EbMed m, const float* hist,
+                     long long hstride, float* gpp, float* gps, int it);
+
 // --- fused single-kernel elastic step (kernels_fused.hip) -------------------
 struct ChunkDesc {
     int slot, x0, x1, z0;  // columns [x0, x1) of slot, rows from z0 (block of threads - 2*FUSED_HALO rows)

@@ -79,6 +79,9 @@ SIGNATURES = {
     "ds2d_born_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, _F, C.POINTER(Stats)]),
     "ds2d_born_adjoint": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.POINTER(Stats)]),
     "ds2d_lsrtm_cg": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, C.c_int, _F, _D, C.POINTER(Stats)]),
+    "ds2d_eborn_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, _F, _F, C.POINTER(Stats)]),
+    "ds2d_eborn_adjoint": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, _F, _F, C.POINTER(Stats)]),
+    "ds2d_elsrtm_cg": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.c_int, _F, _F, _D, C.POINTER(Stats)]),
     "ds2d_lsrtm_learned": (C.c_int, [_H, _H, C.c_int, C.POINTER(Shot), _F, C.c_int, _F, _D, C.POINTER(C.c_int),
                                      C.POINTER(Stats)]),
     "ds2d_set_batch": (C.c_int, [_H, C.c_int]),
@@ -281,6 +284,40 @@ class Solver:
         _check(self.lib.ds2d_born_adjoint(self.h, len(shots), _shots_array(shots), fptr(res), fptr(g),
                                           C.byref(st)))
         return g.reshape(self.allnx, self.allnz), st.as_dict()
+
+    def eborn_forward(self, shots, img_pp, img_ps):
+        """Plain-L2 elastic Born map: (PP, PS) images -> scattered (vz, vx) records."""
+        pp = np.ascontiguousarray(img_pp, np.float32)
+        ps = np.ascontiguousarray(img_ps, np.float32)
+        rz, rx = self._rec(shots), self._rec(shots)
+        st = Stats()
+        _check(self.lib.ds2d_eborn_forward(self.h, len(shots), _shots_array(shots), fptr(pp), fptr(ps), fptr(rz),
+                                           fptr(rx), C.byref(st)))
+        return rz, rx, st.as_dict()
+
+    def eborn_adjoint(self, shots, res_z, res_x):
+        """Exact adjoint: residual records -> (PP, PS) gradients (allnx x allnz)."""
+        rz = np.ascontiguousarray(res_z, np.float32)
+        rx = np.ascontiguousarray(res_x, np.float32)
+        gpp = np.zeros(self.allnx * self.allnz, np.float32)
+        gps = np.zeros_like(gpp)
+        st = Stats()
+        _check(self.lib.ds2d_eborn_adjoint(self.h, len(shots), _shots_array(shots), fptr(rz), fptr(rx), fptr(gpp),
+                                           fptr(gps), C.byref(st)))
+        return gpp.reshape(self.allnx, self.allnz), gps.reshape(self.allnx, self.allnz), st.as_dict()
+
+    def elsrtm_cg(self, shots, d_z, d_x, niter, pp0=None, ps0=None):
+        """CGLS PP/PS LSRTM on the elastic Born pair: (pp, ps, residual history, stats)."""
+        A = self.allnx * self.allnz
+        pp = np.zeros(A, np.float32) if pp0 is None else np.ascontiguousarray(pp0, np.float32).ravel().copy()
+        ps = np.zeros(A, np.float32) if ps0 is None else np.ascontiguousarray(ps0, np.float32).ravel().copy()
+        res = np.zeros(niter + 1, np.float64)
+        st = Stats()
+        dz = np.ascontiguousarray(d_z, np.float32)
+        dx = np.ascontiguousarray(d_x, np.float32)
+        _check(self.lib.ds2d_elsrtm_cg(self.h, len(shots), _shots_array(shots), fptr(dz), fptr(dx), niter,
+                                       fptr(pp), fptr(ps), res.ctypes.data_as(_D), C.byref(st)))
+        return pp.reshape(self.allnx, self.allnz), ps.reshape(self.allnx, self.allnz), res, st.as_dict()
 
     def set_fused(self, mode):
         """Fused Born/adjoint steps of the resident-history passes: -1 auto, 0 never, 1 always."""

@@ -157,6 +157,23 @@ int ds2d_born_adjoint(ds2d_t* h, int nshots, const ds2d_shot* shots, const float
 int ds2d_lsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_obs, int niter,
                   float* image, double* resid, ds2d_stats* stats);
 
+/* Plain-L2 elastic Born operator L: (PP, PS) images -> scattered (vz, vx)
+ * records, and its exact adjoint (records -> PP, PS gradients, overwritten),
+ * batched over shots; physics 0, random boundary, no damping.  The Born
+ * source is add_born_source (born.cu:351-405, direction 1) on the physical
+ * window and the imaging products are image_gradient's (FD2DGPU.cu:3336-3339);
+ * the propagation replaces the reference's born_modeling / GPUcalculate_*_back
+ * schedule (FD2DGPU.cu:1817-1897, 591-895).  Images allnx*allnz; records
+ * [shot][trace][it]. */
+int ds2d_eborn_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* img_pp, const float* img_ps,
+                       float* rec_z, float* rec_x, ds2d_stats* stats);
+int ds2d_eborn_adjoint(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* res_z, const float* res_x,
+                       float* g_pp, float* g_ps, ds2d_stats* stats);
+/* CGLS PP/PS LSRTM on the elastic Born pair; images in/out, resid[niter+1] as
+ * for ds2d_lsrtm_cg. */
+int ds2d_elsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_z, const float* d_x, int niter,
+                   float* img_pp, float* img_ps, double* resid, ds2d_stats* stats);
+
 /* ---------------------------------------------------------------------------
  * Learned LSRTM preconditioner (no reference counterpart: new capability).
  * A U-Net (levels, base channels; 3x3 convs + ReLU, 2x2 average pooling,

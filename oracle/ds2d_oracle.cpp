@@ -511,6 +511,215 @@ void acou_adjoint(const orc_shot* s, const R* vp, const R* rho, const R* wav, co
     if (grad_pad_out) std::copy(grad.begin(), grad.end(), grad_pad_out);
 }
 
+// ---------------------------------------------------------------------------
+// Plain-L2 elastic Born pair (random boundary, no damping, unsplit fields).
+//
+// Forward, per step: background velocity with the transient source on txx,
+// tzz at the source cell (FD2DGPU.cu:2039-2049; update_v, born.cu:215-279),
+// background stress (update_txx_tzz/update_txz, born.cu:282-349), scattered
+// velocity, scattered stress plus the Born source of add_born_source
+// (born.cu:351-405, direction 1) on [pml, n-pml)^2 from the new background
+// velocity, then the scattered vz, vx at the receivers (FD2DGPU.cu:3093-3101).
+// Without damping the split fields sum to the unsplit update.  The adjoint is
+// the exact transpose of this linear map: its imaging products are those of
+// image_gradient (FD2DGPU.cu:3336-3339, -2 vp^2 rho and -2 vs^2 rho forms).
+template <typename R>
+struct EState {
+    std::vector<R> vx, vz, txx, tzz, txz;
+    void init(size_t n) {
+        vx.assign(n, 0); vz.assign(n, 0); txx.assign(n, 0); tzz.assign(n, 0); txz.assign(n, 0);
+    }
+};
+
+template <typename R>
+struct EBorn {
+    Setup<R> S;
+    std::vector<R> dmp, dms;  // padded image windows, zero outside the physical window
+};
+
+template <typename R>
+EBorn<R> eborn_setup(const orc_shot* s, const R* vp, const R* vs, const R* rho, const R* ipp, const R* ips) {
+    EBorn<R> E;
+    E.S = make_setup<R>(s, vp, vs, rho);
+    const Grid<R>& g = E.S.g;
+    E.dmp.assign(g.n(), 0);
+    E.dms.assign(g.n(), 0);
+    for (int ix = 0; ix < s->nx; ++ix)
+        for (int iz = 0; iz < s->allnz; ++iz) {
+            const size_t c = g.at(ix + s->pml, iz + s->pml), q = (size_t)(s->ngx_min + ix) * s->allnz + iz;
+            if (ipp) E.dmp[c] = ipp[q];
+            if (ips) E.dms[c] = ips[q];
+        }
+    return E;
+}
+
+// v += (dt/d) (1/rho) D sigma, the source w added to txx and tzz at src for
+// this update only (src = n: no source).
+template <typename R>
+void eb_vel(const Grid<R>& g, const Medium<R>& m, EState<R>& f, size_t src, R w) {
+    const int nzp = g.nzp;
+    const R ax = g.dt / g.dx, az = g.dt / g.dz;
+    const bool has = src < g.n();
+    R k1 = 0, k2 = 0;
+    if (has) { k1 = f.txx[src]; k2 = f.tzz[src]; f.txx[src] = k1 + w; f.tzz[src] = k2 + w; }
+#pragma omp parallel for schedule(static)
+    for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+        for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+            const size_t c = g.at(ix, iz);
+            const R br = (R)1 / m.rho[c];
+            f.vx[c] += br * ax * dplus_x(f.txx.data(), c, nzp) + br * az * dminus_z(f.txz.data(), c);
+            f.vz[c] += br * ax * dminus_x(f.txz.data(), c, nzp) + br * az * dplus_z(f.tzz.data(), c);
+        }
+    if (has) { f.txx[src] = k1; f.tzz[src] = k2; }
+}
+
+template <typename R>
+void eb_str(const Grid<R>& g, const Medium<R>& m, EState<R>& f) {
+    const int nzp = g.nzp;
+    const R ax = g.dt / g.dx, az = g.dt / g.dz;
+#pragma omp parallel for schedule(static)
+    for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+        for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+            const size_t c = g.at(ix, iz);
+            const R dvx = dminus_x(f.vx.data(), c, nzp), dvz = dminus_z(f.vz.data(), c);
+            const R dvxz = dplus_z(f.vx.data(), c), dvzx = dplus_x(f.vz.data(), c, nzp);
+            const R l = m.lam[c], mu = m.mu[c], l2m = l + 2 * mu;
+            f.txx[c] += ax * l2m * dvx + az * l * dvz;
+            f.tzz[c] += ax * l * dvx + az * l2m * dvz;
+            f.txz[c] += mu * (ax * dvzx + az * dvxz);
+        }
+}
+
+// Background strains of the Born source: exx = D-x vx / dx, ezz = D-z vz / dz,
+// exz = D+x vz / dx + D+z vx / dz (add_born_source's vxx/dx, vzz/dz, vzx/dx + vxz/dz).
+template <typename R>
+void eb_strain(const Grid<R>& g, const EState<R>& f, size_t c, R& exx, R& ezz, R& exz) {
+    exx = dminus_x(f.vx.data(), c, g.nzp) / g.dx;
+    ezz = dminus_z(f.vz.data(), c) / g.dz;
+    exz = dplus_x(f.vz.data(), c, g.nzp) / g.dx + dplus_z(f.vx.data(), c) / g.dz;
+}
+
+template <typename R>
+void eborn_forward(const orc_shot* s, const R* vp, const R* vs, const R* rho, const R* ipp, const R* ips,
+                   const R* wav, R* rec_z, R* rec_x) {
+    EBorn<R> E = eborn_setup<R>(s, vp, vs, rho, ipp, ips);
+    const Grid<R>& g = E.S.g;
+    const Medium<R>& m = E.S.m;
+    const size_t n = g.n(), src = g.at(s->isx, s->isz);
+    EState<R> bg, b;
+    bg.init(n);
+    b.init(n);
+    const int p = g.pml;
+    for (int it = 0; it < s->nt; ++it) {
+        eb_vel(g, m, bg, src, wav[it]);
+        eb_str(g, m, bg);
+        eb_vel(g, m, b, n, (R)0);
+        eb_str(g, m, b);
+#pragma omp parallel for schedule(static)
+        for (int ix = p; ix < g.nxp - p; ++ix)
+            for (int iz = p; iz < g.nzp - p; ++iz) {
+                const size_t c = g.at(ix, iz);
+                R exx, ezz, exz;
+                eb_strain(g, bg, c, exx, ezz, exz);
+                const R l2m = m.lam[c] + 2 * m.mu[c], mu = m.mu[c];
+                const R mp = -E.dmp[c], ms = -E.dms[c];
+                b.txx[c] += 2 * l2m * mp * exx + 2 * (l2m * mp - 2 * mu * ms) * ezz;
+                b.tzz[c] += 2 * (l2m * mp - 2 * mu * ms) * exx + 2 * l2m * mp * ezz;
+                b.txz[c] += 2 * mu * ms * exz;
+            }
+        record_pair(g, s, b.vz, b.vx, it, rec_z, rec_x);
+    }
+}
+
+// Exact transpose of eborn_forward: residual (res_z, res_x) -> (g_pp, g_ps),
+// added into the global images.  The background strains of every step are
+// kept from one forward sweep.
+template <typename R>
+void eborn_adjoint(const orc_shot* s, const R* vp, const R* vs, const R* rho, const R* wav, const R* res_z,
+                   const R* res_x, R* gpp_glob, R* gps_glob) {
+    EBorn<R> E = eborn_setup<R>(s, vp, vs, rho, nullptr, nullptr);
+    const Grid<R>& g = E.S.g;
+    const Medium<R>& m = E.S.m;
+    const size_t n = g.n(), src = g.at(s->isx, s->isz);
+    const int p = g.pml, nx = s->nx, nz = s->allnz, nzp = g.nzp, nt = s->nt;
+    const size_t nI = (size_t)nx * nz;
+    std::vector<R> hist((size_t)nt * nI * 3);
+    EState<R> bg;
+    bg.init(n);
+    for (int it = 0; it < nt; ++it) {
+        eb_vel(g, m, bg, src, wav[it]);
+        eb_str(g, m, bg);
+        for (int ix = 0; ix < nx; ++ix)
+            for (int iz = 0; iz < nz; ++iz) {
+                R* h = &hist[((size_t)it * nI + (size_t)ix * nz + iz) * 3];
+                eb_strain(g, bg, g.at(ix + p, iz + p), h[0], h[1], h[2]);
+            }
+    }
+    auto inU = [&](int ix, int iz) { return ix >= NOP && ix < g.nxp - NOP && iz >= NOP && iz < g.nzp - NOP; };
+    const R ax = g.dt / g.dx, az = g.dt / g.dz;
+    EState<R> lm;
+    lm.init(n);
+    std::vector<R> wa(n), wb(n), wc(n), ux(n), uz(n), gpp(n, 0), gps(n, 0);
+    for (int it = nt - 2; it >= 0; --it) {
+        // receiver injection of step it+1 (the transpose of its recorder)
+        for (int r = 0; r < s->ntr; ++r) {
+            const size_t c = g.at(s->rec_ix[r], s->igz);
+            lm.vx[c] += res_x[(size_t)r * nt + it + 1];
+            lm.vz[c] += res_z[(size_t)r * nt + it + 1];
+        }
+        // transpose of the stress update: v* += -(dt/d) D (C sigma*), C sigma* masked to U
+        for (int ix = 0; ix < g.nxp; ++ix)
+            for (int iz = 0; iz < g.nzp; ++iz) {
+                const size_t c = g.at(ix, iz);
+                const bool u = inU(ix, iz);
+                const R l = m.lam[c], mu = m.mu[c], l2m = l + 2 * mu;
+                wa[c] = u ? l2m * lm.txx[c] + l * lm.tzz[c] : 0;
+                wb[c] = u ? l * lm.txx[c] + l2m * lm.tzz[c] : 0;
+                wc[c] = u ? mu * lm.txz[c] : 0;
+            }
+#pragma omp parallel for schedule(static)
+        for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+            for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+                const size_t c = g.at(ix, iz);
+                lm.vx[c] += -(ax * dplus_x(wa.data(), c, nzp) + az * dminus_z(wc.data(), c));
+                lm.vz[c] += -(az * dplus_z(wb.data(), c) + ax * dminus_x(wc.data(), c, nzp));
+            }
+        // transpose of the velocity update: sigma* += -(dt/d) D ((1/rho) v*), masked to U
+        for (int ix = 0; ix < g.nxp; ++ix)
+            for (int iz = 0; iz < g.nzp; ++iz) {
+                const size_t c = g.at(ix, iz);
+                const bool u = inU(ix, iz);
+                const R br = (R)1 / m.rho[c];
+                ux[c] = u ? br * lm.vx[c] : 0;
+                uz[c] = u ? br * lm.vz[c] : 0;
+            }
+#pragma omp parallel for schedule(static)
+        for (int ix = NOP; ix < g.nxp - NOP; ++ix)
+            for (int iz = NOP; iz < g.nzp - NOP; ++iz) {
+                const size_t c = g.at(ix, iz);
+                lm.txx[c] += -(ax * dminus_x(ux.data(), c, nzp));
+                lm.tzz[c] += -(az * dminus_z(uz.data(), c));
+                lm.txz[c] += -(az * dplus_z(ux.data(), c) + ax * dplus_x(uz.data(), c, nzp));
+            }
+        // transpose of the Born source of step it
+#pragma omp parallel for schedule(static)
+        for (int ix = 0; ix < nx; ++ix)
+            for (int iz = 0; iz < nz; ++iz) {
+                const size_t c = g.at(ix + p, iz + p);
+                const R* h = &hist[((size_t)it * nI + (size_t)ix * nz + iz) * 3];
+                const R l2m = m.lam[c] + 2 * m.mu[c], mu = m.mu[c];
+                gpp[c] += -2 * l2m * (h[0] + h[1]) * (lm.txx[c] + lm.tzz[c]);
+                gps[c] += 4 * mu * (h[1] * lm.txx[c] + h[0] * lm.tzz[c]) - 2 * mu * h[2] * lm.txz[c];
+            }
+    }
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iz = 0; iz < nz; ++iz) {
+            const size_t c = g.at(ix + p, iz + p), q = (size_t)(s->ngx_min + ix) * s->allnz + iz;
+            if (gpp_glob) gpp_glob[q] += gpp[c];
+            if (gps_glob) gps_glob[q] += gps[c];
+        }
+}
+
 // laplace.cpp:82-106: 5-point Laplacian on [2, n-2) with a zero frame.
 template <typename R>
 void laplace_order(int nz, int nx, R dx, R* dat) {
@@ -630,6 +839,12 @@ int orc_threads(int n) {
     int orc_acou_adjoint_##SUF(const orc_shot* s, const R* vp, const R* rho, const R* wav, const R* res,      \
                                R* grad, R* gpad) {                                                             \
         acou_adjoint<R>(s, vp, rho, wav, res, grad, gpad); return 0; }                                         \
+    int orc_eborn_forward_##SUF(const orc_shot* s, const R* vp, const R* vs, const R* rho, const R* ipp,      \
+                                const R* ips, const R* wav, R* rz, R* rx) {                                     \
+        eborn_forward<R>(s, vp, vs, rho, ipp, ips, wav, rz, rx); return 0; }                                   \
+    int orc_eborn_adjoint_##SUF(const orc_shot* s, const R* vp, const R* vs, const R* rho, const R* wav,      \
+                                const R* res_z, const R* res_x, R* gpp, R* gps) {                             \
+        eborn_adjoint<R>(s, vp, vs, rho, wav, res_z, res_x, gpp, gps); return 0; }                            \
     int orc_laplace_order_##SUF(int nz, int nx, double dx, R* dat) { laplace_order<R>(nz, nx, (R)dx, dat); return 0; } \
     int orc_laplace_band_##SUF(int nz, int nx, R* dat) { laplace_band<R>(nz, nx, dat); return 0; }            \
     int orc_elastic_run_##SUF(int nxp, int nzp, double dx, double dz, double dt, const R* lam, const R* mu,   \

@@ -213,3 +213,24 @@ def rel_l2(a, b):
     b = np.asarray(b, np.float64).ravel()
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def eborn_forward(shot, vp, vs, rho, img_pp, img_ps, wav, prec="d"):
+    """Plain-L2 elastic Born map: (img_pp, img_ps) -> scattered (vz, vx) records."""
+    t = dtype_of(prec)
+    rz = np.zeros(shot.ntr * shot.nt, t)
+    rx = np.zeros_like(rz)
+    args = [_c(a, prec) for a in (vp, vs, rho, img_pp, img_ps, wav)]
+    getattr(load(), f"orc_eborn_forward_{prec}")(C.byref(shot), *[P(a) for a in args], P(rz), P(rx))
+    return rz, rx
+
+
+def eborn_adjoint(shot, vp, vs, rho, wav, res_z, res_x, prec="d"):
+    """Its exact transpose: residual records -> (g_pp, g_ps) global images."""
+    t = dtype_of(prec)
+    A = shot.allnx * shot.allnz
+    gpp = np.zeros(A, t)
+    gps = np.zeros(A, t)
+    args = [_c(a, prec) for a in (vp, vs, rho, wav, res_z, res_x)]
+    getattr(load(), f"orc_eborn_adjoint_{prec}")(C.byref(shot), *[P(a) for a in args], P(gpp), P(gps))
+    return gpp, gps

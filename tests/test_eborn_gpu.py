@@ -1,0 +1,127 @@
+"""Plain-L2 elastic Born pair (PP/PS) on the GPU vs the FP64 CPU oracle, and
+PP/PS CG LSRTM vs CGLS driven by the oracle's pair.
+
+Tolerance: relative L2 <= 1e-5 (FP32 GPU vs FP64 oracle) on records and
+gradients with the resident background history; the reversal mode rebuilds
+the background in FP32 through the random boundary and is held to 1e-4.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from cases import DT, DX, DZ, elastic_model, wavelet
+from ds2d_amd import Solver
+from ds2d_amd.lib import ShotSpec
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _case(nt=300):
+    """Two uneven windows on a 64 x 40 layered elastic model, random boundary 20."""
+    allnx, allnz, pml = 64, 40, 20
+    vp, vs, rho = elastic_model(allnx, allnz, 23)
+    wav = wavelet(nt)
+    shots = []
+    for i, (x0, w) in enumerate(((0, 48), (22, 42))):
+        shots.append(ShotSpec(x0, w, pml + w // 2, pml + 2, pml + 1, pml + np.arange(0, w, 2, dtype=np.int32),
+                              shot_id=i, boundary=1))
+    rng = np.random.default_rng(31)
+    pp = (0.05 * rng.standard_normal((allnx, allnz))).astype(np.float32)
+    ps = (0.03 * rng.standard_normal((allnx, allnz))).astype(np.float32)
+    return allnx, allnz, pml, nt, vp, vs, rho, wav, shots, pp, ps
+
+
+def _solver(allnx, allnz, pml, nt, wav, **kw):
+    return Solver(dx=DX, dz=DZ, dt=DT, nt=nt, allnx=allnx, allnz=allnz, pml=pml, nx_max=allnx, wavelet=wav,
+                  physics=0, **kw)
+
+
+def _oshots(shots, allnx, allnz, pml, nt):
+    return [orc.make_shot(sh, allnx, allnz, pml, nt, DX, DZ, DT) for sh in shots]
+
+
+@pytest.mark.parametrize("bg_mode", [1, 2])
+def test_eborn_forward_and_adjoint_match_oracle(bg_mode):
+    allnx, allnz, pml, nt, vp, vs, rho, wav, shots, pp, ps = _case()
+    s = _solver(allnx, allnz, pml, nt, wav, bg_mode=bg_mode)
+    s.set_model(vp, vs, rho)
+    rz, rx, _ = s.eborn_forward(shots, pp, ps)
+    rng = np.random.default_rng(5)
+    res_z = rng.standard_normal(rz.size).astype(np.float32)
+    res_x = rng.standard_normal(rx.size).astype(np.float32)
+    gpp, gps, _ = s.eborn_adjoint(shots, res_z, res_x)
+    ogpp, ogps = np.zeros(allnx * allnz), np.zeros(allnx * allnz)
+    off = 0
+    for sh, o in zip(shots, _oshots(shots, allnx, allnz, pml, nt)):
+        n = sh.ntr * nt
+        oz, ox = orc.eborn_forward(o, vp, vs, rho, pp, ps, wav)
+        ez, ex = orc.rel_l2(rz[off:off + n], oz), orc.rel_l2(rx[off:off + n], ox)
+        print(f"shot {sh.shot_id}: records rel z {ez:.2e} x {ex:.2e}")
+        assert np.abs(oz).max() > 0 and ez <= TOL and ex <= TOL
+        a, b = orc.eborn_adjoint(o, vp, vs, rho, wav, res_z[off:off + n], res_x[off:off + n])
+        ogpp += a
+        ogps += b
+        off += n
+    epp, eps = orc.rel_l2(gpp.ravel(), ogpp), orc.rel_l2(gps.ravel(), ogps)
+    print(f"bg_mode={bg_mode} gradient rel pp {epp:.2e} ps {eps:.2e}")
+    tol = TOL if bg_mode == 1 else 1e-4
+    assert epp <= tol and eps <= tol
+    # adjoint consistency of the GPU pair itself
+    lhs = float(rz.astype(np.float64) @ res_z + rx.astype(np.float64) @ res_x)
+    rhs = float(pp.ravel().astype(np.float64) @ gpp.ravel() + ps.ravel().astype(np.float64) @ gps.ravel())
+    assert abs(lhs - rhs) <= 1e-4 * abs(lhs)
+
+
+def test_elsrtm_cg_matches_oracle_cg():
+    """Three PP/PS CG iterations on the GPU vs CGLS on the FP64 oracle pair."""
+    allnx, allnz, pml, nt, vp, vs, rho, wav, shots, pp, ps = _case(nt=260)
+    s = _solver(allnx, allnz, pml, nt, wav, bg_mode=1)
+    s.set_model(vp, vs, rho)
+    dz, dx, _ = s.eborn_forward(shots, pp, ps)
+    ipp, ips, res, _ = s.elsrtm_cg(shots, dz, dx, 3)
+    oshots = _oshots(shots, allnx, allnz, pml, nt)
+    n = [sh.ntr * nt for sh in shots]
+    A = allnx * allnz
+
+    def L(m):
+        zs, xs = [], []
+        for o in oshots:
+            a, b = orc.eborn_forward(o, vp, vs, rho, m[:A].reshape(allnx, allnz), m[A:].reshape(allnx, allnz), wav)
+            zs.append(a)
+            xs.append(b)
+        return np.concatenate(zs + xs)
+
+    def LT(r):
+        R = sum(n)
+        out, off = np.zeros(2 * A), 0
+        for o, k in zip(oshots, n):
+            a, b = orc.eborn_adjoint(o, vp, vs, rho, wav, r[off:off + k], r[R + off:R + off + k])
+            out[:A] += a
+            out[A:] += b
+            off += k
+        return out
+
+    d = np.concatenate([dz, dx]).astype(np.float64)
+    x = np.zeros(2 * A)
+    r = d.copy()
+    sv = LT(r)
+    p = sv.copy()
+    gamma = float(sv @ sv)
+    hist = [1.0]
+    for _ in range(3):
+        q = L(p)
+        alpha = gamma / float(q @ q)
+        x += alpha * p
+        r -= alpha * q
+        hist.append(float(np.linalg.norm(r) / np.linalg.norm(d)))
+        sv = LT(r)
+        gnew = float(sv @ sv)
+        p = sv + (gnew / gamma) * p
+        gamma = gnew
+    print("GPU residuals", res, "oracle CGLS", hist)
+    assert res[0] == pytest.approx(1.0, abs=1e-6)
+    assert all(res[k + 1] < res[k] for k in range(3))
+    np.testing.assert_allclose(res, hist, rtol=1e-4)
+    img = np.concatenate([ipp.ravel(), ips.ravel()])
+    assert orc.rel_l2(img, x) <= 1e-4
