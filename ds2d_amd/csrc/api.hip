@@ -1651,15 +1651,112 @@ struct Precond {
         launch_axpby(st, z, z, A, 0.f, (float)rms);
         return 0;
     }
+    // nch images of allnx x allnz one after another, each preconditioned alone
+    int apply_n(ds2d_ctx* h, int nch, const float* g, float* z) {
+        const long long A = (long long)h->g.allnx * h->g.allnz;
+        for (int k = 0; k < nch; ++k) {
+            const int rc = apply(h, g + k * A, z + k * A);
+            if (rc) return rc;
+        }
+        return 0;
+    }
 };
+
+// Flexible preconditioned CG (Polak-Ribiere+, exact line search on the
+// quadratic) with the learned direction, falling back to plain CGLS from the
+// current image when a learned step would lower |r| by less than 1%.
+// On entry: x the image (nch * A), r = d - L x (Rn samples), rr = |r|^2,
+// dd = |d|^2, g = L^T r.  born(m, q): q = L m; adjoint(r, g): g = L^T r.
+template <class BornF, class AdjF>
+int learned_cg(ds2d_ctx* h, Precond& P, int nch, long long A, long long Rn, float* x, float* r, float* q, float* p,
+               float* g, float* z, float* gn, float* zn, double rr, double dd, BornF born, AdjF adjoint, int niter,
+               double* resid, int* accepted_out) {
+    hipStream_t st = h->st;
+    int rc = 0;
+    const long long N = nch * A;
+    auto dot = [&](const float* a, const float* b, long long n) { return host_dot(h, a, b, n, &rc); };
+    bool learned = true;
+    int accepted = 0;
+    double gz = 0, gamma = 0;
+    if ((rc = P.apply_n(h, nch, g, z))) return rc;
+    HIP_TRY(hipMemcpyAsync(p, z, N * 4, hipMemcpyDeviceToDevice, st));
+    gz = dot(g, z, N);
+    if (rc) return rc;
+    for (int k = 0; k < niter; ++k) {
+        if (learned) {
+            if ((rc = born(p, q))) return rc;
+            const double qr = dot(q, r, Rn), qq = dot(q, q, Rn);
+            if (rc) return rc;
+            const double rr_new = qq > 0 ? rr - qr * qr / qq : rr;
+            if (qq > 0 && qr > 0 && rr_new < 0.99 * 0.99 * rr) {
+                const double alpha = qr / qq;
+                launch_axpby(st, x, p, N, (float)alpha, 1.f);
+                launch_axpby(st, r, q, Rn, (float)-alpha, 1.f);
+                rr = dot(r, r, Rn);
+                if (rc) return rc;
+                if (resid) resid[k + 1] = std::sqrt(rr / dd);
+                ++accepted;
+                if ((rc = adjoint(r, gn))) return rc;
+                if ((rc = P.apply_n(h, nch, gn, zn))) return rc;
+                const double zg_new = dot(zn, gn, N), zg_old = dot(zn, g, N);
+                if (rc) return rc;
+                const double beta = gz > 0 ? std::max(0.0, (zg_new - zg_old) / gz) : 0.0;
+                launch_axpby(st, p, zn, N, 1.f, (float)beta);  // p = zn + beta p
+                std::swap(g, gn);
+                std::swap(z, zn);
+                gz = zg_new;
+                continue;
+            }
+            // fall back: plain CGLS from the current image, s = g
+            learned = false;
+            HIP_TRY(hipMemcpyAsync(p, g, N * 4, hipMemcpyDeviceToDevice, st));
+            gamma = dot(g, g, N);
+            if (rc) return rc;
+        }
+        if (!(gamma > 0)) {
+            if (resid)
+                for (int j = k; j < niter; ++j) resid[j + 1] = std::sqrt(rr / dd);
+            break;
+        }
+        if ((rc = born(p, q))) return rc;
+        const double qq = dot(q, q, Rn);
+        if (rc) return rc;
+        const double alpha = qq > 0 ? gamma / qq : 0.0;
+        launch_axpby(st, x, p, N, (float)alpha, 1.f);
+        launch_axpby(st, r, q, Rn, (float)-alpha, 1.f);
+        rr = dot(r, r, Rn);
+        if (rc) return rc;
+        if (resid) resid[k + 1] = std::sqrt(rr / dd);
+        if ((rc = adjoint(r, g))) return rc;
+        const double gnew = dot(g, g, N);
+        if (rc) return rc;
+        launch_axpby(st, p, g, N, 1.f, (float)(gnew / gamma));  // p = s + beta p
+        gamma = gnew;
+    }
+    if (accepted_out) *accepted_out = accepted;
+    return 0;
+}
+
+// The U-Net's padded grid for this model, checked against its limits.
+int precond_for(ds2d_ctx* h, ds2d_unet* u, Precond& P) {
+    int L, mn, mh, mw;
+    unet_limits(u, &L, &mn, &mh, &mw);
+    const int qd = 1 << (L - 1);
+    P = Precond{u, (h->g.allnx + qd - 1) / qd * qd, (h->g.allnz + qd - 1) / qd * qd, nullptr, nullptr};
+    if (P.H > mh || P.W > mw)
+        return fail(DS2D_EINVAL, "model %dx%d (padded %dx%d) exceeds the U-Net's %dx%d", h->g.allnx, h->g.allnz, P.H,
+                    P.W, mh, mw);
+    int rc;
+    if ((rc = h->fbuf("lr_in", (size_t)P.H * P.W, &P.in)) || (rc = h->fbuf("lr_out", (size_t)P.H * P.W, &P.out)))
+        return rc;
+    return 0;
+}
 
 }  // namespace
 
 extern "C" {
 
-// Flexible preconditioned CG (Polak-Ribiere+, exact line search on the
-// quadratic) with the learned direction, falling back to plain CGLS from the
-// current image when a learned step would lower |r| by less than 1%.
+// Learned LSRTM on the plain acoustic Born pair (learned_cg above).
 int ds2d_lsrtm_learned(ds2d_t* h, ds2d_unet_t* u, int nshots, const ds2d_shot* shots, const float* d_obs,
                        int niter, float* image, double* resid, int* learned_steps, ds2d_stats* stats) {
     int rc;
@@ -1667,19 +1764,13 @@ int ds2d_lsrtm_learned(ds2d_t* h, ds2d_unet_t* u, int nshots, const ds2d_shot* s
     if ((rc = acou_begin(h, nshots, shots, B))) return rc;
     if (!u) return fail(DS2D_EINVAL, "null U-Net handle");
     if (niter < 0 || !d_obs || !image) return fail(DS2D_EINVAL, "bad LSRTM arguments");
-    int L, mn, mh, mw;
-    unet_limits(u, &L, &mn, &mh, &mw);
-    const int qd = 1 << (L - 1);
-    Precond P{u, (h->g.allnx + qd - 1) / qd * qd, (h->g.allnz + qd - 1) / qd * qd, nullptr, nullptr};
-    if (P.H > mh || P.W > mw)
-        return fail(DS2D_EINVAL, "model %dx%d (padded %dx%d) exceeds the U-Net's %dx%d", h->g.allnx, h->g.allnz, P.H,
-                    P.W, mh, mw);
+    Precond P{};
+    if ((rc = precond_for(h, u, P))) return rc;
     const long long A = (long long)h->g.allnx * h->g.allnz;
     hipStream_t st = h->st;
     float *x, *p, *g, *z, *gn, *zn;
     if ((rc = h->fbuf("cg_x", A, &x)) || (rc = h->fbuf("cg_p", A, &p)) || (rc = h->fbuf("cg_s", A, &g)) ||
-        (rc = h->fbuf("lr_z", A, &z)) || (rc = h->fbuf("lr_gn", A, &gn)) || (rc = h->fbuf("lr_zn", A, &zn)) ||
-        (rc = h->fbuf("lr_in", (size_t)P.H * P.W, &P.in)) || (rc = h->fbuf("lr_out", (size_t)P.H * P.W, &P.out)))
+        (rc = h->fbuf("lr_z", A, &z)) || (rc = h->fbuf("lr_gn", A, &gn)) || (rc = h->fbuf("lr_zn", A, &zn)))
         return rc;
     float *d, *r, *q;
     const long long R = std::max<long long>(B.total_rec, 1);
@@ -1704,7 +1795,6 @@ int ds2d_lsrtm_learned(ds2d_t* h, ds2d_unet_t* u, int nshots, const ds2d_shot* s
         props += w.store ? 1 : 2;
         return acou_grad_to_global(h, B, w, out);
     };
-    auto dot = [&](const float* a, const float* b, long long n) { return host_dot(h, a, b, n, &rc); };
     stats_begin(h);
     stats_loop_begin(h);
     // r = d - L x (fills the background history in store mode)
@@ -1712,75 +1802,89 @@ int ds2d_lsrtm_learned(ds2d_t* h, ds2d_unet_t* u, int nshots, const ds2d_shot* s
     props += 2;
     HIP_TRY(hipMemcpyAsync(r, d, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
     launch_axpby(st, r, w.rec2, B.total_rec, -1.f, 1.f);
-    const double dd = dot(d, d, B.total_rec);
+    const double dd = host_dot(h, d, d, B.total_rec, &rc);
     if (rc) return rc;
     if (!(dd > 0)) return fail(DS2D_EINVAL, "observed data are all zero");
-    double rr = dot(r, r, B.total_rec);
+    const double rr = host_dot(h, r, r, B.total_rec, &rc);
     if (rc) return rc;
     if (resid) resid[0] = std::sqrt(rr / dd);
     if ((rc = adjoint(r, g))) return rc;
-    bool learned = true;
-    int accepted = 0;
-    double gz = 0, gamma = 0;
-    if ((rc = P.apply(h, g, z))) return rc;
-    HIP_TRY(hipMemcpyAsync(p, z, A * 4, hipMemcpyDeviceToDevice, st));
-    gz = dot(g, z, A);
-    if (rc) return rc;
-    for (int k = 0; k < niter; ++k) {
-        if (learned) {
-            if ((rc = born(p, q))) return rc;
-            const double qr = dot(q, r, B.total_rec), qq = dot(q, q, B.total_rec);
-            if (rc) return rc;
-            const double rr_new = qq > 0 ? rr - qr * qr / qq : rr;
-            if (qq > 0 && qr > 0 && rr_new < 0.99 * 0.99 * rr) {
-                const double alpha = qr / qq;
-                launch_axpby(st, x, p, A, (float)alpha, 1.f);
-                launch_axpby(st, r, q, B.total_rec, (float)-alpha, 1.f);
-                rr = dot(r, r, B.total_rec);
-                if (rc) return rc;
-                if (resid) resid[k + 1] = std::sqrt(rr / dd);
-                ++accepted;
-                if ((rc = adjoint(r, gn))) return rc;
-                if ((rc = P.apply(h, gn, zn))) return rc;
-                const double zg_new = dot(zn, gn, A), zg_old = dot(zn, g, A);
-                if (rc) return rc;
-                const double beta = gz > 0 ? std::max(0.0, (zg_new - zg_old) / gz) : 0.0;
-                launch_axpby(st, p, zn, A, 1.f, (float)beta);  // p = zn + beta p
-                std::swap(g, gn);
-                std::swap(z, zn);
-                gz = zg_new;
-                continue;
-            }
-            // fall back: plain CGLS from the current image, s = g
-            learned = false;
-            HIP_TRY(hipMemcpyAsync(p, g, A * 4, hipMemcpyDeviceToDevice, st));
-            gamma = dot(g, g, A);
-            if (rc) return rc;
-        }
-        if (!(gamma > 0)) {
-            if (resid)
-                for (int j = k; j < niter; ++j) resid[j + 1] = std::sqrt(rr / dd);
-            break;
-        }
-        if ((rc = born(p, q))) return rc;
-        const double qq = dot(q, q, B.total_rec);
-        if (rc) return rc;
-        const double alpha = qq > 0 ? gamma / qq : 0.0;
-        launch_axpby(st, x, p, A, (float)alpha, 1.f);
-        launch_axpby(st, r, q, B.total_rec, (float)-alpha, 1.f);
-        rr = dot(r, r, B.total_rec);
-        if (rc) return rc;
-        if (resid) resid[k + 1] = std::sqrt(rr / dd);
-        if ((rc = adjoint(r, g))) return rc;
-        const double gnew = dot(g, g, A);
-        if (rc) return rc;
-        launch_axpby(st, p, g, A, 1.f, (float)(gnew / gamma));  // p = s + beta p
-        gamma = gnew;
-    }
+    if ((rc = learned_cg(h, P, 1, A, B.total_rec, x, r, q, p, g, z, gn, zn, rr, dd, born, adjoint, niter, resid,
+                         learned_steps)))
+        return rc;
     stats_loop_end(h);
     if ((rc = stats_end(h, stats, cells * h->g.nt * props, 0, 0))) return rc;
-    if (learned_steps) *learned_steps = accepted;
     HIP_TRY(hipMemcpyAsync(image, x, A * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Learned PP/PS LSRTM on the plain elastic Born pair: the U-Net preconditions
+// the PP and the PS gradient images one at a time.
+int ds2d_elsrtm_learned(ds2d_t* h, ds2d_unet_t* u, int nshots, const ds2d_shot* shots, const float* d_z,
+                        const float* d_x, int niter, float* img_pp, float* img_ps, double* resid, int* learned_steps,
+                        ds2d_stats* stats) {
+    int rc;
+    Batch B;
+    if ((rc = eb_begin(h, nshots, shots, B))) return rc;
+    if (!u) return fail(DS2D_EINVAL, "null U-Net handle");
+    if (niter < 0 || !d_z || !d_x || !img_pp || !img_ps) return fail(DS2D_EINVAL, "bad LSRTM arguments");
+    Precond P{};
+    if ((rc = precond_for(h, u, P))) return rc;
+    const long long A = (long long)h->g.allnx * h->g.allnz;
+    hipStream_t st = h->st;
+    float *x, *p, *g, *z, *gn, *zn;
+    if ((rc = h->fbuf("ecg_x", 2 * A, &x)) || (rc = h->fbuf("ecg_p", 2 * A, &p)) ||
+        (rc = h->fbuf("ecg_s", 2 * A, &g)) || (rc = h->fbuf("elr_z", 2 * A, &z)) ||
+        (rc = h->fbuf("elr_gn", 2 * A, &gn)) || (rc = h->fbuf("elr_zn", 2 * A, &zn)))
+        return rc;
+    const long long R = std::max<long long>(B.total_rec, 1);
+    float *d, *r, *q;
+    if ((rc = h->fbuf("ecg_d", 2 * R, &d)) || (rc = h->fbuf("ecg_r", 2 * R, &r)) || (rc = h->fbuf("ecg_q", 2 * R, &q)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(x, img_pp, A * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(x + A, img_ps, A * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(d, 0, 2 * R * 4, st));
+    HIP_TRY(hipMemcpyAsync(d, d_z, B.total_rec * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d + R, d_x, B.total_rec * 4, hipMemcpyHostToDevice, st));
+    EbWS w;
+    if ((rc = eb_prepare(h, B, x, x + A, true, w))) return rc;
+    long long props = 0;
+    auto born = [&](const float* m, float* out) -> int {
+        int e;
+        if ((e = eb_set_image(h, B, m, m + A, w))) return e;
+        eb_fwd(h, B, w, !w.store);
+        props += w.store ? 1 : 2;
+        HIP_TRY(hipMemcpyAsync(out, w.rec, 2 * R * 4, hipMemcpyDeviceToDevice, st));
+        return 0;
+    };
+    auto adjoint = [&](const float* res, float* out) -> int {
+        HIP_TRY(hipMemcpyAsync(w.res, res, 2 * R * 4, hipMemcpyDeviceToDevice, st));
+        eb_bwd(h, B, w);
+        props += w.store ? 1 : 2;
+        return eb_grad_to_global(h, B, w, out);
+    };
+    stats_begin(h);
+    stats_loop_begin(h);
+    eb_fwd(h, B, w, true);
+    props += 2;
+    HIP_TRY(hipMemcpyAsync(r, d, 2 * R * 4, hipMemcpyDeviceToDevice, st));
+    launch_axpby(st, r, w.rec, 2 * R, -1.f, 1.f);
+    const double dd = host_dot(h, d, d, 2 * R, &rc);
+    if (rc) return rc;
+    if (!(dd > 0)) return fail(DS2D_EINVAL, "observed data are all zero");
+    const double rr = host_dot(h, r, r, 2 * R, &rc);
+    if (rc) return rc;
+    if (resid) resid[0] = std::sqrt(rr / dd);
+    if ((rc = adjoint(r, g))) return rc;
+    if ((rc = learned_cg(h, P, 2, A, 2 * R, x, r, q, p, g, z, gn, zn, rr, dd, born, adjoint, niter, resid,
+                         learned_steps)))
+        return rc;
+    stats_loop_end(h);
+    if ((rc = stats_end(h, stats, eb_cells(B, h->g.nt) * props, 0, 0))) return rc;
+    HIP_TRY(hipMemcpyAsync(img_pp, x, A * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(img_ps, x + A, A * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipGetLastError());
     return 0;

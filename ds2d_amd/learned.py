@@ -6,6 +6,8 @@
 2. The U-Net (ds2d_amd.lib.Unet) is trained on those pairs with Adam.
 3. LSRTM on held-out models runs ds2d_lsrtm_learned (U-Net direction,
    plain CGLS fallback) beside ds2d_lsrtm_cg.
+The elastic variant does the same with PP and PS images of the plain elastic
+Born pair (ds2d_elsrtm_learned beside ds2d_elsrtm_cg).
 """
 import time
 
@@ -96,6 +98,82 @@ def compare(solver, unet, prob, niter, target=1e-2, log=None):
         k = first_hit(res, target)
         if log:
             log(f"seed {prob['seed']} {name}: residual {res[-1]:.4f} after {niter} iterations, {wall:.1f} s")
+        out[name] = dict(resid=[float(r) for r in res], wall_s=wall, iters_to_target=k,
+                         time_to_target_s=None if k is None else wall * k / max(niter, 1), image_rel_err=err,
+                         learned_steps=acc)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Elastic PP/PS variant (plain-L2 elastic Born pair)
+
+def elastic_problem(seed, nx=512, nz=256, nshots=16, nt=1500, dt=0.0008, dx=10.0, pml=40, f0=15.0, lenses=3):
+    """Seeded elastic model: vp as acoustic_problem, vs = vp/1.73 scaled by
+    +-15 % from a second seeded layering (so PS differs from PP), Gardner
+    density of the migration vp; 9x9-smoothed migration vp and vs; PP and PS
+    reflectivities (v - v_mig)/v_mig; surface shots, random boundary."""
+    vp_true = layered_model(nx, nz, seed, lenses=lenses)
+    r2 = layered_model(nx, nz, seed + 500).astype(np.float64)
+    r2 = 2 * (r2 - r2.min()) / max(r2.max() - r2.min(), 1.0) - 1
+    vs_true = (vp_true / 1.73 * (1 + 0.15 * r2)).astype(np.float32)
+    vp_mig, vs_mig = gaussian_smooth(vp_true), gaussian_smooth(vs_true)
+    _, rho = elastic_from_vp(vp_mig)
+    refl_pp = ((vp_true - vp_mig) / vp_mig).astype(np.float32)
+    refl_ps = ((vs_true - vs_mig) / vs_mig).astype(np.float32)
+    xs = [int((i + 0.5) * nx / nshots) for i in range(nshots)]
+    shots = [surface_shot(nx, 0, pml, x, 2, 1, shot_id=i, boundary=1) for i, x in enumerate(xs)]
+    return dict(seed=seed, nx=nx, nz=nz, dx=dx, dz=dx, dt=dt, nt=nt, pml=pml, f0=f0, vp=vp_mig, vs=vs_mig, rho=rho,
+                refl_pp=refl_pp, refl_ps=refl_ps, shots=shots)
+
+
+def make_elastic_solver(prob, bg_mode=0):
+    wav = ricker(prob["nt"], prob["f0"], prob["dt"])
+    return Solver(dx=prob["dx"], dz=prob["dz"], dt=prob["dt"], nt=prob["nt"], allnx=prob["nx"], allnz=prob["nz"],
+                  pml=prob["pml"], nx_max=prob["nx"], wavelet=wav, physics=0, bg_mode=bg_mode)
+
+
+def generate_elastic_pairs(seeds, niter=20, solver=None, log=None, **kw):
+    """(X, T, info, solver): PP and PS gradient images (RTM) and their
+    `niter`-iteration CG images, RMS-normalised, two samples per model."""
+    X, T, info = [], [], []
+    s = solver
+    for i, seed in enumerate(list(seeds)):
+        prob = elastic_problem(seed, **kw)
+        if s is None:
+            s = make_elastic_solver(prob)
+        s.set_model(prob["vp"], prob["vs"], prob["rho"])
+        dz, dx, _ = s.eborn_forward(prob["shots"], prob["refl_pp"], prob["refl_ps"])
+        gpp, gps, _ = s.eborn_adjoint(prob["shots"], dz, dx)
+        ipp, ips, res, _ = s.elsrtm_cg(prob["shots"], dz, dx, niter)
+        for g, img in ((gpp, ipp), (gps, ips)):
+            X.append(g / _rms(g))
+            T.append(img / _rms(img))
+        info.append(dict(seed=seed, resid=res.tolist()))
+        if log:
+            log(f"elastic pair {i + 1}: {niter}-iteration residual {res[-1]:.4f}")
+    return np.asarray(X, np.float32), np.asarray(T, np.float32), info, s
+
+
+def compare_elastic(solver, unet, prob, niter, target=1e-2, log=None):
+    """Plain PP/PS CG vs learned elastic LSRTM on one problem."""
+    solver.set_model(prob["vp"], prob["vs"], prob["rho"])
+    dz, dx, _ = solver.eborn_forward(prob["shots"], prob["refl_pp"], prob["refl_ps"])
+    ref = np.concatenate([prob["refl_pp"].ravel(), prob["refl_ps"].ravel()])
+    out = {}
+    for name in ("plain", "learned"):
+        t0 = time.perf_counter()
+        if name == "plain":
+            ipp, ips, res, st = solver.elsrtm_cg(prob["shots"], dz, dx, niter)
+            acc = 0
+        else:
+            ipp, ips, res, st = solver.elsrtm_learned(unet, prob["shots"], dz, dx, niter)
+            acc = st["learned_steps"]
+        wall = time.perf_counter() - t0
+        img = np.concatenate([ipp.ravel(), ips.ravel()])
+        err = float(np.linalg.norm(img - ref) / np.linalg.norm(ref))
+        k = first_hit(res, target)
+        if log:
+            log(f"seed {prob['seed']} elastic {name}: residual {res[-1]:.4f} after {niter} iterations, {wall:.1f} s")
         out[name] = dict(resid=[float(r) for r in res], wall_s=wall, iters_to_target=k,
                          time_to_target_s=None if k is None else wall * k / max(niter, 1), image_rel_err=err,
                          learned_steps=acc)

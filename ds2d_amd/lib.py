@@ -84,6 +84,8 @@ SIGNATURES = {
     "ds2d_elsrtm_cg": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.c_int, _F, _F, _D, C.POINTER(Stats)]),
     "ds2d_lsrtm_learned": (C.c_int, [_H, _H, C.c_int, C.POINTER(Shot), _F, C.c_int, _F, _D, C.POINTER(C.c_int),
                                      C.POINTER(Stats)]),
+    "ds2d_elsrtm_learned": (C.c_int, [_H, _H, C.c_int, C.POINTER(Shot), _F, _F, C.c_int, _F, _F, _D,
+                                      C.POINTER(C.c_int), C.POINTER(Stats)]),
     "ds2d_set_batch": (C.c_int, [_H, C.c_int]),
     "ds2d_set_fused": (C.c_int, [_H, C.c_int]),
     "ds2d_unet_create": (C.c_int, [C.POINTER(UnetCfg), C.POINTER(_H)]),
@@ -340,6 +342,23 @@ class Solver:
         out = st.as_dict()
         out["learned_steps"] = acc.value
         return img.reshape(self.allnx, self.allnz), res, out
+
+    def elsrtm_learned(self, unet, shots, d_z, d_x, niter, pp0=None, ps0=None):
+        """PP/PS LSRTM with the U-Net preconditioner (plain CGLS fallback)."""
+        A = self.allnx * self.allnz
+        pp = np.zeros(A, np.float32) if pp0 is None else np.ascontiguousarray(pp0, np.float32).ravel().copy()
+        ps = np.zeros(A, np.float32) if ps0 is None else np.ascontiguousarray(ps0, np.float32).ravel().copy()
+        res = np.zeros(niter + 1, np.float64)
+        st = Stats()
+        acc = C.c_int(0)
+        dz = np.ascontiguousarray(d_z, np.float32)
+        dx = np.ascontiguousarray(d_x, np.float32)
+        _check(self.lib.ds2d_elsrtm_learned(self.h, unet.h, len(shots), _shots_array(shots), fptr(dz), fptr(dx),
+                                            niter, fptr(pp), fptr(ps), res.ctypes.data_as(_D), C.byref(acc),
+                                            C.byref(st)))
+        out = st.as_dict()
+        out["learned_steps"] = acc.value
+        return pp.reshape(self.allnx, self.allnz), ps.reshape(self.allnx, self.allnz), res, out
 
     def lsrtm_cg(self, shots, d_obs, niter, image0=None):
         img = np.zeros(self.allnx * self.allnz, np.float32) if image0 is None else \

@@ -232,6 +232,11 @@ int ds2d_conv3x3_bf16(int op, int n, int h, int w, int c1, int c2, int up, int c
  * receives the number of accepted learned steps. */
 int ds2d_lsrtm_learned(ds2d_t* h, ds2d_unet_t* u, int nshots, const ds2d_shot* shots, const float* d_obs,
                        int niter, float* image, double* resid, int* learned_steps, ds2d_stats* stats);
+/* The same on the plain elastic PP/PS Born pair (ds2d_eborn_*): the U-Net
+ * preconditions the PP and the PS gradient images one at a time. */
+int ds2d_elsrtm_learned(ds2d_t* h, ds2d_unet_t* u, int nshots, const ds2d_shot* shots, const float* d_z,
+                        const float* d_x, int niter, float* img_pp, float* img_ps, double* resid, int* learned_steps,
+                        ds2d_stats* stats);
 /* Launch shots in groups of at most n per time loop (0 = all shots in one
  * batched launch, the default; 1 = one shot after another as the reference's
  * per-rank shot loop does).  Applies to the plain Born/adjoint/CG entries. */

@@ -125,3 +125,30 @@ def test_elsrtm_cg_matches_oracle_cg():
     np.testing.assert_allclose(res, hist, rtol=1e-4)
     img = np.concatenate([ipp.ravel(), ips.ravel()])
     assert orc.rel_l2(img, x) <= 1e-4
+
+
+def test_elsrtm_learned_trained_network_gain():
+    """Pairs from the plain elastic method (PP and PS gradient -> 20-iteration
+    CG image) on 12 small models train the U-Net; on 2 held-out models the
+    learned PP/PS solver is compared with plain CG after the same number of
+    iterations.  The CGLS fallback bounds the loss; the gain is printed."""
+    from ds2d_amd import learned as LR
+    kw = dict(nx=96, nz=48, nshots=6, nt=700, pml=30, dt=0.0008)
+    X, T, info, s = LR.generate_elastic_pairs(range(100, 112), niter=20, **kw)
+    assert X.shape == (24, 96, 48)
+    u, loss, st = LR.train_unet(X, T, steps=400, batch=8, lr=1e-3, seed=0)
+    print(f"elastic train loss {loss[:10].mean():.4f} -> {loss[-10:].mean():.4f} ({st['wall_s']:.1f} s)")
+    assert loss[-10:].mean() < loss[:10].mean()
+    niter = 6
+    gains = []
+    for seed in (950, 951):
+        prob = LR.elastic_problem(seed, **kw)
+        out = LR.compare_elastic(s, u, prob, niter)
+        p, l = out["plain"]["resid"][-1], out["learned"]["resid"][-1]
+        print(f"seed {seed}: plain {p:.4f} learned {l:.4f} (learned steps {out['learned']['learned_steps']}), "
+              f"image err plain {out['plain']['image_rel_err']:.3f} learned {out['learned']['image_rel_err']:.3f}")
+        r = out["learned"]["resid"]
+        assert all(r[k + 1] <= r[k] * (1 + 1e-6) for k in range(niter))
+        assert l <= 1.05 * p
+        gains.append(p / l)
+    print(f"elastic learned gain (plain / learned residual after {niter} iterations): {np.mean(gains):.3f}")
