@@ -13,7 +13,10 @@ Other workloads (--workload): c1 = single-shot elastic forward on 251x101
 (2500 steps); c2 = acoustic CG LSRTM of 26 random-boundary shots on 251x101
 (time to a relative data residual, residual history); c4 = the learned
 preconditioner: training pairs from the plain method, U-Net training, and
-plain vs learned LSRTM time to the residual target on held-out models.
+plain vs learned LSRTM time to the residual target on held-out models;
+c5 = PP/PS elastic Born CG LSRTM of 256 uneven windows on 4000x1000 (a step
+= one CG iteration), with the FP64-oracle comparison on a 500x250 window;
+unet = U-Net training throughput.
 
 Prints one JSON line: value = Gcell-updates/s (one cell-update = one padded
 cell advanced one step for one wavefield); roofline of the dominant kernel
@@ -40,7 +43,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "unet"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "unet"])
+    ap.add_argument("--c5-iter", type=int, default=2, help="c5: CG iterations on the full configuration")
+    ap.add_argument("--no-window-parity", action="store_true", help="c5: skip the FP64-oracle window comparison")
     ap.add_argument("--train-models", type=int, default=64, help="c4: training models (seeds 1000..)")
     ap.add_argument("--eval-models", type=int, default=4, help="c4: held-out models (seeds 2000..)")
     ap.add_argument("--train-steps", type=int, default=1500, help="c4: Adam steps")
@@ -127,7 +132,7 @@ def roofline(stats):
     x_ms = float(np.mean([st.get("aux_kernel_avg_ms", 0.0) for st in stats]))
     xb = stats[0].get("aux_alg_bytes", 0.0)
     if x_ms > 0 and xb > 0:
-        # two kernels per step (fused step: interior chunks + damping-strip chunks)
+        # two kernels per step (mode 1: velocity kernel + stress kernel)
         out["second_kernel"] = {"kernel_avg_ms": round(x_ms, 4), "alg_bytes_per_launch": xb,
                                 "achieved": round(xb / (x_ms * 1e-3) / 1e9, 1)}
         out["step_achieved"] = round((b + xb) / ((k_ms + x_ms) * 1e-3) / 1e9, 1)
@@ -343,6 +348,126 @@ def run_learned(a):
     return out
 
 
+def run_c5(a):
+    """C5, the full combined path: PP/PS elastic Born LSRTM of 256 uneven
+    windows on the 4000x1000 model, all shots batched.  Plain CG for
+    --c5-iter iterations (and the learned solver with --load-unet, the U-Net
+    applied to the PP and PS gradients): residual history, seconds per
+    iteration, Gcell-updates/s.  Then the same solver on a 500x250 window of
+    the model (4 shots, 2 iterations) against CGLS driven by the FP64 oracle:
+    image error vs the oracle."""
+    from ds2d_amd import Solver, Unet, models
+    from ds2d_amd.lib import ricker
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    cfg = models.c5_combined(nshots=a.nshots or 256, nt=a.nt or 3000)
+    wav = ricker(cfg["nt"], cfg["f0"], cfg["dt"])
+    shots = cfg["shots"]
+    s = Solver(dx=cfg["dx"], dz=cfg["dz"], dt=cfg["dt"], nt=cfg["nt"], allnx=cfg["nx"], allnz=cfg["nz"],
+               pml=cfg["pml"], nx_max=max(x.nx for x in shots), wavelet=wav, physics=0)
+    s.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
+    t0 = time.perf_counter()
+    dz, dx, stf = s.eborn_forward(shots, cfg["refl_pp"], cfg["refl_ps"])
+    t_fwd = time.perf_counter() - t0
+    log(f"c5 Born data: {t_fwd:.1f} s, {stf['cell_updates'] / t_fwd / 1e9:.2f} Gcell/s")
+    ref = np.concatenate([cfg["refl_pp"].ravel(), cfg["refl_ps"].ravel()])
+    runs = {}
+    solvers = [("plain", None)]
+    if a.load_unet:
+        u = Unet(levels=4, base=32, max_batch=1, max_h=cfg["nx"], max_w=cfg["nz"], seed=0)
+        u.set_params(np.load(a.load_unet))
+        solvers.append(("learned", u))
+    for name, u in solvers:
+        t0 = time.perf_counter()
+        if u is None:
+            pp, ps, res, st = s.elsrtm_cg(shots, dz, dx, a.c5_iter)
+            acc = 0
+        else:
+            pp, ps, res, st = s.elsrtm_learned(u, shots, dz, dx, a.c5_iter)
+            acc = st["learned_steps"]
+        el = time.perf_counter() - t0
+        img = np.concatenate([pp.ravel(), ps.ravel()])
+        runs[name] = {"wall_s": round(el, 2), "gcell_per_s": round(st["cell_updates"] / el / 1e9, 3),
+                      "resid": [round(float(r), 6) for r in res], "learned_steps": acc,
+                      "image_rel_err_vs_true_reflectivity": round(float(np.linalg.norm(img - ref) /
+                                                                        np.linalg.norm(ref)), 4)}
+        log(f"c5 {name}: residual {res[-1]:.4f} after {a.c5_iter} iterations, {el:.1f} s")
+    window = None if a.no_window_parity else c5_window_parity(cfg, log)
+    p = runs["plain"]
+    return {"metric": "Gcell-updates/s/GPU", "value": p["gcell_per_s"], "unit": "Gcell-updates/s", "n_gpus": 1,
+            "steps": a.c5_iter, "warmup": 0, "ms_per_step": round(p["wall_s"] * 1e3 / max(a.c5_iter, 1), 1),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic: seeded 4000x1000 layered+faulted elastic model, Born data of the true PP/PS "
+                    "reflectivities",
+            "config": {"workload": f"c5_combined: 4000x1000, dx=dz=10 m, {len(shots)} uneven windows (400-1200 "
+                                   f"cols), random boundary {cfg['pml']}, dt {cfg['dt'] * 1e3:.1f} ms, "
+                                   f"{cfg['nt']} steps, PP/PS elastic Born CG LSRTM",
+                       "padded_cells_per_wavefield": int(sum((x.nx + 2 * cfg["pml"]) * (cfg["nz"] + 2 * cfg["pml"])
+                                                             for x in shots)),
+                       "born_data_s": round(t_fwd, 2), "solvers": runs, "window_parity": window,
+                       "note": "a step = one CG iteration over all shots; 1e-2 is not reached in this many "
+                               "iterations (see resid)"}}
+
+
+def c5_window_parity(cfg, log):
+    """GPU PP/PS CG vs CGLS on the FP64 oracle's Born pair, 500x250 window of
+    the C5 model, 4 full-window shots, 800 steps, 2 iterations."""
+    from ds2d_amd import Solver
+    from ds2d_amd.lib import ricker
+    from ds2d_amd.models import surface_shot
+    orc = _orc()
+    x0, z0, nx, nz, nt, pml, niter = 1750, 0, 500, 250, 800, 40, 2
+    sl = (slice(x0, x0 + nx), slice(z0, z0 + nz))
+    vp, vs, rho = (np.ascontiguousarray(cfg[k][sl]) for k in ("vp", "vs", "rho"))
+    rpp, rps = (np.ascontiguousarray(cfg[k][sl]) for k in ("refl_pp", "refl_ps"))
+    wav = ricker(nt, cfg["f0"], cfg["dt"])
+    shots = [surface_shot(nx, 0, pml, 62 + 125 * i, 2, 1, shot_id=i, boundary=1) for i in range(4)]
+    s = Solver(dx=cfg["dx"], dz=cfg["dz"], dt=cfg["dt"], nt=nt, allnx=nx, allnz=nz, pml=pml, nx_max=nx,
+               wavelet=wav, physics=0)
+    s.set_model(vp, vs, rho)
+    dz, dx, _ = s.eborn_forward(shots, rpp, rps)
+    gpp, gps, res, _ = s.elsrtm_cg(shots, dz, dx, niter)
+    t0 = time.perf_counter()
+    osh = [orc.make_shot(x, nx, nz, pml, nt, cfg["dx"], cfg["dz"], cfg["dt"]) for x in shots]
+    n = [x.ntr * nt for x in shots]
+    A, R = nx * nz, sum(n)
+
+    def L(m):
+        out = [orc.eborn_forward(o, vp, vs, rho, m[:A].reshape(nx, nz), m[A:].reshape(nx, nz), wav) for o in osh]
+        return np.concatenate([z for z, _ in out] + [x for _, x in out])
+
+    def LT(r):
+        g, off = np.zeros(2 * A), 0
+        for o, k in zip(osh, n):
+            a_, b_ = orc.eborn_adjoint(o, vp, vs, rho, wav, r[off:off + k], r[R + off:R + off + k])
+            g[:A] += a_
+            g[A:] += b_
+            off += k
+        return g
+
+    d = np.concatenate([dz, dx]).astype(np.float64)
+    x, r = np.zeros(2 * A), d.copy()
+    sv = LT(r)
+    p, gamma, hist = sv.copy(), float(sv @ sv), [1.0]
+    for _ in range(niter):
+        q = L(p)
+        alpha = gamma / float(q @ q)
+        x += alpha * p
+        r -= alpha * q
+        hist.append(float(np.linalg.norm(r) / np.linalg.norm(d)))
+        sv = LT(r)
+        gnew = float(sv @ sv)
+        p, gamma = sv + (gnew / gamma) * p, gnew
+    img = np.concatenate([gpp.ravel(), gps.ravel()])
+    err = float(np.linalg.norm(img - x) / np.linalg.norm(x))
+    log(f"c5 window parity: image rel err {err:.2e} vs FP64 oracle ({time.perf_counter() - t0:.1f} s CPU)")
+    return {"window": f"x {x0}..{x0 + nx}, z {z0}..{z0 + nz}", "shots": 4, "nt": nt, "iterations": niter,
+            "image_rel_err_vs_fp64_oracle": err, "resid_gpu": [float(v) for v in res], "resid_oracle": hist,
+            "oracle_threads": orc.threads(0)}
+
+
 def run_unet(a):
     """U-Net training throughput: Adam steps (batch 8) of the 4-level, 32-channel
     U-Net on 512x256 images with the BF16 MFMA kernels (synthetic data)."""
@@ -409,6 +534,8 @@ def main():
         out = run_forward(a, cfg)
     elif a.workload == "c4":
         out = run_learned(a)
+    elif a.workload == "c5":
+        out = run_c5(a)
     elif a.workload == "unet":
         out = run_unet(a)
     else:

@@ -109,3 +109,29 @@ def c3_batched_elastic(seed=3, nshots=128, nt=3000, wmin=400, wmax=1200):
         shots.append(surface_shot(w, x0, pml, w // 2, 2, 0, shot_id=i, boundary=0))
     return dict(name="c3_batched_elastic", nx=nx, nz=nz, dx=12.5, dz=12.5, dt=0.001, nt=nt, pml=pml, f0=15.0,
                 vp=vp, vs=vs, rho=rho, shots=shots)
+
+
+def c5_combined(seed=7, nshots=256, nt=3000, dt=0.0008, wmin=400, wmax=1200, pml=40):
+    """Full combined path: 4000x1000 seeded elastic model, dx=dz=10 m, 256
+    uneven-aperture shots (window width U{400..1200}, start uniform), random
+    boundary `pml` cells; vs from vp/1.73 scaled +-15 % by a second layering;
+    9x9-smoothed migration vp/vs; PP and PS reflectivities (v - v_mig)/v_mig.
+    dt 0.8 ms (CFL 0.80 at 5500 m/s), `nt` steps; source at the window
+    centre at depth 2, receivers at depth 1 on every window node."""
+    nx, nz = 4000, 1000
+    vp_true = layered_model(nx, nz, seed)
+    r2 = layered_model(nx, nz, seed + 500).astype(np.float64)
+    r2 = 2 * (r2 - r2.min()) / max(r2.max() - r2.min(), 1.0) - 1
+    vs_true = (vp_true / 1.73 * (1 + 0.15 * r2)).astype(np.float32)
+    vp_mig, vs_mig = gaussian_smooth(vp_true), gaussian_smooth(vs_true)
+    _, rho = elastic_from_vp(vp_mig)
+    rng = np.random.default_rng(seed + 1000)
+    shots = []
+    for i in range(nshots):
+        w = int(rng.integers(wmin, wmax + 1))
+        x0 = int(rng.integers(0, nx - w + 1))
+        shots.append(surface_shot(w, x0, pml, w // 2, 2, 1, shot_id=i, boundary=1))
+    return dict(name="c5_combined", nx=nx, nz=nz, dx=10.0, dz=10.0, dt=dt, nt=nt, pml=pml, f0=15.0, vp=vp_mig,
+                vs=vs_mig, rho=rho, refl_pp=((vp_true - vp_mig) / vp_mig).astype(np.float32),
+                refl_ps=((vs_true - vs_mig) / vs_mig).astype(np.float32), shots=shots)
+
