@@ -1343,6 +1343,7 @@ namespace {
 
 struct EbWS {
     EbField bg{}, bn{}, lm{};
+    EbAux aux{};
     EbMed m{};
     float *gpp = nullptr, *gps = nullptr;  // per-slot padded gradients (n each)
     float* win = nullptr;                  // per-slot window images, PP then PS (2 W)
@@ -1393,12 +1394,13 @@ int eb_prepare(ds2d_ctx* h, const Batch& B, const float* pp_dev, const float* ps
     w.W = std::max<long long>(B.total_win, 1);
     w.H = std::max<long long>(B.total_hist, 1);
     float* base;
-    if ((rc = h->fbuf("eborn", (size_t)n * 17, &base))) return rc;
+    if ((rc = h->fbuf("eborn", (size_t)n * 22, &base))) return rc;
     w.bg = eb_field(base, n);
     w.bn = eb_field(base + 5 * n, n);
     w.lm = eb_field(base + 10 * n, n);
     w.gpp = base + 15 * n;
     w.gps = base + 16 * n;
+    w.aux = EbAux{base + 17 * n, base + 18 * n, base + 19 * n, base + 20 * n, base + 21 * n};
     if ((rc = eb_set_image(h, B, pp_dev, ps_dev, w))) return rc;
     if ((rc = h->fbuf("ebrec", (size_t)w.R * 4, &w.rec))) return rc;
     w.res = w.rec + 2 * w.R;
@@ -1444,7 +1446,7 @@ void eb_bg(ds2d_ctx* h, const Batch& B, EbWS& w) {
 // history the background, left at t = nt by the forward pass, is reversed.
 void eb_bwd(ds2d_ctx* h, const Batch& B, EbWS& w) {
     hipStream_t st = h->st;
-    (void)hipMemsetAsync(w.lm.vx, 0, (size_t)w.n * 7 * 4, st);  // lm + gpp + gps
+    (void)hipMemsetAsync(w.lm.vx, 0, (size_t)w.n * 12 * 4, st);  // lm + gpp + gps + adjoint products
     const int nt = h->g.nt;
     for (const BatchShape& sh : launch_groups(h, B))
         for (int it = nt - 2; it >= 0; --it) {
@@ -1452,8 +1454,8 @@ void eb_bwd(ds2d_ctx* h, const Batch& B, EbWS& w) {
                 launch_eb_s(st, sh, w.bg, w.bn, w.m, nullptr, w.H, it + 1, -1.f, 1, 0, 0, w.rmap, nullptr, nullptr);
                 launch_eb_v(st, sh, w.bg, w.bn, w.m.rho, h->wav[it + 1], -1.f, 1, 0);
             }
-            launch_eb_adj_v(st, sh, w.lm, w.m, w.rmap, w.res, w.res + w.R, it + 1);
-            launch_eb_adj_s(st, sh, w.lm, w.bg, w.m, w.store ? w.hist : nullptr, w.H, w.gpp, w.gps, it);
+            launch_eb_adj_v(st, sh, w.lm, w.aux, w.m, w.rmap, w.res, w.res + w.R, it + 1);
+            launch_eb_adj_s(st, sh, w.lm, w.aux, w.bg, w.m, w.store ? w.hist : nullptr, w.H, w.gpp, w.gps, it);
         }
 }
 

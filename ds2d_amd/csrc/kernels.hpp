@@ -134,6 +134,12 @@ struct EbField {
 struct EbMed {
     const float *rho, *lam, *mu, *dmp, *dms;  // dmp/dms: padded PP/PS image windows
 };
+// Adjoint products handed between the two adjoint kernels (zero outside the
+// update region): C s* as wa = (l+2m) sxx* + l szz*, wb = l sxx* + (l+2m) szz*,
+// wc = m sxz*; and u = (1/rho) v*.
+struct EbAux {
+    float *wa, *wb, *wc, *ux, *uz;
+};
 // Forward step: k_eb_v (background with source sample w and/or scattered
 // velocity), then k_eb_s (stresses, Born source, recorder).  sign = -1 undoes a
 // background step.  hist_mode: 1 store the background strains (3 arrays of
@@ -147,10 +153,10 @@ void launch_eb_s(hipStream_t st, const BatchShape& b, EbField bg, EbField born, 
 // transpose of the stress update), then k_eb_adj_s (transpose of the velocity
 // update and the gradient of step it; strains from hist, or from bg when
 // hist is null).
-void launch_eb_adj_v(hipStream_t st, const BatchShape& b, EbField lm, EbMed m, const int* rmap, const float* res_z,
-                     const float* res_x, int it_inj);
-void launch_eb_adj_s(hipStream_t st, const BatchShape& b, EbField lm, EbField bg, EbMed m, const float* hist,
-                     long long hstride, float* gpp, float* gps, int it);
+void launch_eb_adj_v(hipStream_t st, const BatchShape& b, EbField lm, EbAux x, EbMed m, const int* rmap,
+                     const float* res_z, const float* res_x, int it_inj);
+void launch_eb_adj_s(hipStream_t st, const BatchShape& b, EbField lm, EbAux x, EbField bg, EbMed m,
+                     const float* hist, long long hstride, float* gpp, float* gps, int it);
 
 // --- fused single-kernel elastic step (kernels_fused.hip) -------------------
 struct ChunkDesc {

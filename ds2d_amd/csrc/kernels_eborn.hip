@@ -10,10 +10,12 @@
 // fields, update region [nop, n-nop)^2.
 //
 // The adjoint runs the transposed steps backward: v* += -(dt/d) D (C s*) and
-// s* += -(dt/d) D ((1/rho) v*), each with its operand masked to the update
-// region, then the transpose of the Born source (the image_gradient products
-// of FD2DGPU.cu:3336-3339) from the background strains of that step, read
-// from the resident history or rebuilt by reversing the background.
+// s* += -(dt/d) D ((1/rho) v*), each operand masked to the update region (the
+// products C s* and (1/rho) v* are written by the kernel that updates s* or
+// v*, so cells outside the region keep their zero), then the transpose of the
+// Born source (the image_gradient products of FD2DGPU.cu:3336-3339) from the
+// background strains of that step, read from the resident history or rebuilt
+// by reversing the background.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -161,30 +163,16 @@ __global__ __launch_bounds__(256) void k_eb_s(EbField bg, EbField b, EbMed m, fl
     }
 }
 
-// Adjoint, part 1: v* += inj(step it_inj) - (dt/d) D (C s*), C s* masked to the
-// update region.
-__global__ __launch_bounds__(256) void k_eb_adj_v(EbField lm, EbMed m, const SlotDesc* __restrict__ slots, float ax,
-                                                  float az, const int* __restrict__ rmap, const float* res_z,
-                                                  const float* res_x, int nt, int it_inj) {
+// Adjoint, part 1: v* += inj(step it_inj) - (dt/d) D (C s*), reading the
+// products C s* that part 2 left (zero outside the update region); leaves
+// u = (1/rho) v* for part 2.
+__global__ __launch_bounds__(256) void k_eb_adj_v(EbField lm, EbAux x, EbMed m, const SlotDesc* __restrict__ slots,
+                                                  float ax, float az, const int* __restrict__ rmap,
+                                                  const float* res_z, const float* res_x, int nt, int it_inj) {
     const SlotDesc S = slots[blockIdx.z];
     Cell t;
     if (!cell_u(S, t)) return;
     const long long c = (long long)t.c, s = S.nzp;
-    // operands masked to the update region: neighbours k cells away along x or z
-    auto ux_ = [&](int k) { const int x = t.ix + k; return x >= NOP && x < S.nxp - NOP; };
-    auto uz_ = [&](int k) { const int z = t.iz + k; return z >= NOP && z < S.nzp - NOP; };
-    auto wa_x = [&](int k) {
-        const long long j = c + (long long)k * s;
-        const float l = m.lam[j], mu = m.mu[j];
-        return ux_(k) ? (l + 2 * mu) * lm.txx[j] + l * lm.tzz[j] : 0.f;
-    };
-    auto wb_z = [&](int k) {
-        const long long j = c + k;
-        const float l = m.lam[j], mu = m.mu[j];
-        return uz_(k) ? l * lm.txx[j] + (l + 2 * mu) * lm.tzz[j] : 0.f;
-    };
-    auto wc_x = [&](int k) { const long long j = c + (long long)k * s; return ux_(k) ? m.mu[j] * lm.txz[j] : 0.f; };
-    auto wc_z = [&](int k) { const long long j = c + k; return uz_(k) ? m.mu[j] * lm.txz[j] : 0.f; };
     float ix_ = 0.f, iz_ = 0.f;
     if (it_inj >= 0 && t.iz == S.igz) {
         const int r = rmap[S.rmap_off + t.ix];
@@ -194,33 +182,37 @@ __global__ __launch_bounds__(256) void k_eb_adj_v(EbField lm, EbMed m, const Slo
             iz_ = res_z[o];
         }
     }
-    lm.vx[c] += ix_ - (ax * dplus(wa_x) + az * dminus(wc_z));
-    lm.vz[c] += iz_ - (az * dplus(wb_z) + ax * dminus(wc_x));
+    const float vx = lm.vx[c] + ix_ - (ax * dplus(EB_X(x.wa)) + az * dminus(EB_Z(x.wc)));
+    const float vz = lm.vz[c] + iz_ - (az * dplus(EB_Z(x.wb)) + ax * dminus(EB_X(x.wc)));
+    lm.vx[c] = vx;
+    lm.vz[c] = vz;
+    const float br = 1.f / m.rho[c];
+    x.ux[c] = br * vx;
+    x.uz[c] = br * vz;
 }
 
-// Adjoint, part 2: s* += -(dt/d) D ((1/rho) v*) masked to the update region,
-// then the gradient of step it at the physical window: the strains come from
-// the history (hist != null) or from the background velocity.
-__global__ __launch_bounds__(256) void k_eb_adj_s(EbField lm, EbField bg, EbMed m, const float* __restrict__ hist,
-                                                  long long hstride, float* gpp, float* gps,
-                                                  const SlotDesc* __restrict__ slots, float ax, float az, float idx,
-                                                  float idz, int it) {
+// Adjoint, part 2: s* += -(dt/d) D u (u = (1/rho) v* from part 1, zero outside
+// the update region); leaves C s* for the next part 1; then the gradient of
+// step it at the physical window: the strains come from the history
+// (hist != null) or from the background velocity.
+__global__ __launch_bounds__(256) void k_eb_adj_s(EbField lm, EbAux x, EbField bg, EbMed m,
+                                                  const float* __restrict__ hist, long long hstride, float* gpp,
+                                                  float* gps, const SlotDesc* __restrict__ slots, float ax, float az,
+                                                  float idx, float idz, int it) {
     const SlotDesc S = slots[blockIdx.z];
     Cell t;
     if (!cell_u(S, t)) return;
     const long long c = (long long)t.c, s = S.nzp;
-    auto in_x = [&](int k) { const int x = t.ix + k; return x >= NOP && x < S.nxp - NOP; };
-    auto in_z = [&](int k) { const int z = t.iz + k; return z >= NOP && z < S.nzp - NOP; };
-    auto ux_x = [&](int k) { const long long j = c + (long long)k * s; return in_x(k) ? lm.vx[j] / m.rho[j] : 0.f; };
-    auto ux_z = [&](int k) { const long long j = c + k; return in_z(k) ? lm.vx[j] / m.rho[j] : 0.f; };
-    auto uz_x = [&](int k) { const long long j = c + (long long)k * s; return in_x(k) ? lm.vz[j] / m.rho[j] : 0.f; };
-    auto uz_z = [&](int k) { const long long j = c + k; return in_z(k) ? lm.vz[j] / m.rho[j] : 0.f; };
-    const float txx = lm.txx[c] - ax * dminus(ux_x);
-    const float tzz = lm.tzz[c] - az * dminus(uz_z);
-    const float txz = lm.txz[c] - (az * dplus(ux_z) + ax * dplus(uz_x));
+    const float txx = lm.txx[c] - ax * dminus(EB_X(x.ux));
+    const float tzz = lm.tzz[c] - az * dminus(EB_Z(x.uz));
+    const float txz = lm.txz[c] - (az * dplus(EB_Z(x.ux)) + ax * dplus(EB_X(x.uz)));
     lm.txx[c] = txx;
     lm.tzz[c] = tzz;
     lm.txz[c] = txz;
+    const float l = m.lam[c], mu = m.mu[c], l2m = l + 2 * mu;
+    x.wa[c] = l2m * txx + l * tzz;
+    x.wb[c] = l * txx + l2m * tzz;
+    x.wc[c] = mu * txz;
     size_t h;
     if (!hist_at(S, t.ix, t.iz, it, h)) return;
     float exx, ezz, exz;
@@ -231,7 +223,6 @@ __global__ __launch_bounds__(256) void k_eb_adj_s(EbField lm, EbField bg, EbMed 
     } else {
         strains(bg, c, s, idx, idz, exx, ezz, exz);
     }
-    const float l2m = m.lam[c] + 2 * m.mu[c], mu = m.mu[c];
     gpp[c] += -2 * l2m * (exx + ezz) * (txx + tzz);
     gps[c] += 4 * mu * (ezz * txx + exx * tzz) - 2 * mu * exz * txz;
 }
@@ -252,14 +243,14 @@ void launch_eb_s(hipStream_t st, const BatchShape& b, EbField bg, EbField born, 
                        b.dt / b.dz, 1.f / b.dx, 1.f / b.dz, it, sign, do_bg, do_born, hist_mode, rmap, rec_z, rec_x,
                        b.nt);
 }
-void launch_eb_adj_v(hipStream_t st, const BatchShape& b, EbField lm, EbMed m, const int* rmap, const float* res_z,
-                     const float* res_x, int it_inj) {
-    hipLaunchKernelGGL(k_eb_adj_v, egrid(b), dim3(BZ, BX), 0, st, lm, m, b.slots, b.dt / b.dx, b.dt / b.dz, rmap,
+void launch_eb_adj_v(hipStream_t st, const BatchShape& b, EbField lm, EbAux x, EbMed m, const int* rmap,
+                     const float* res_z, const float* res_x, int it_inj) {
+    hipLaunchKernelGGL(k_eb_adj_v, egrid(b), dim3(BZ, BX), 0, st, lm, x, m, b.slots, b.dt / b.dx, b.dt / b.dz, rmap,
                        res_z, res_x, b.nt, it_inj);
 }
-void launch_eb_adj_s(hipStream_t st, const BatchShape& b, EbField lm, EbField bg, EbMed m, const float* hist,
-                     long long hstride, float* gpp, float* gps, int it) {
-    hipLaunchKernelGGL(k_eb_adj_s, egrid(b), dim3(BZ, BX), 0, st, lm, bg, m, hist, hstride, gpp, gps, b.slots,
+void launch_eb_adj_s(hipStream_t st, const BatchShape& b, EbField lm, EbAux x, EbField bg, EbMed m,
+                     const float* hist, long long hstride, float* gpp, float* gps, int it) {
+    hipLaunchKernelGGL(k_eb_adj_s, egrid(b), dim3(BZ, BX), 0, st, lm, x, bg, m, hist, hstride, gpp, gps, b.slots,
                        b.dt / b.dx, b.dt / b.dz, 1.f / b.dx, 1.f / b.dz, it);
 }
 
