@@ -550,11 +550,13 @@ def main():
         tr = json.load(open(a.traffic_json))
         cfgd = out.get("config", {})
         tile = cfgd.get("autotuned_tile") or {}
+        key = [tile.get("x_width"), tile.get("threads"), tile.get("prefetch_columns")]
         # per-launch traffic depends on the batch and the tile, not on the number of steps
         if (tr.get("workload_prefix") and str(cfgd.get("workload", "")).startswith(tr["workload_prefix"])
-                and tr.get("shots") == cfgd.get("shots")
-                and tr.get("tile") == [tile.get("x_width"), tile.get("threads"), tile.get("prefetch_columns")]):
-            out["roofline"]["traffic"] = tr.get("traffic_bytes_per_launch")
+                and tr.get("shots") == cfgd.get("shots")):
+            for e in tr.get("entries", [tr]):
+                if e.get("tile") == key:
+                    out["roofline"]["traffic"] = e.get("traffic_bytes_per_launch")
     print(json.dumps(out), flush=True)
 
 
