@@ -712,8 +712,8 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
         stats_loop_end(h);
         launches = (long long)nt;
         // unsplit interior, per padded cell: reads stress 3 + velocity 2 + dt/rho, lambda, mu; writes 5.
-        // Each PML cell also reads dampx, dampz and reads + writes 4 velocity and 6 stress splits.
-        bytes_v = (double)B.total_cells * 13 * 4 + pml_cells * 22 * 4;
+        // Each PML cell also reads dampx, dampz and reads + writes one split of each of the 5 components.
+        bytes_v = (double)B.total_cells * 13 * 4 + pml_cells * 12 * 4;
         bytes_s = 0;
     } else {
         float* par;

@@ -4,7 +4,8 @@
 // Scheme: born.cu:215-349 (update_vx/vz then update_txx_tzz/txz) with the
 // transient explosive source of FD2DGPU.cu:2039-2049 and the surface
 // recorder of FD2DGPU.cu:355-365; split PML memory only in the damping
-// strips (elsewhere the split update equals the unsplit one).
+// strips (elsewhere the split update equals the unsplit one), one split per
+// component (the other is the unsplit value minus it).
 //
 // Layout of the work: one workgroup owns columns [x0, x1) and a block of
 // rows of one shot window, one z row per lane (rows contiguous in memory, so
@@ -42,9 +43,9 @@ struct Col {
     float tzz, vx, vz, bx;       // stress tzz, old velocity, dt/rho of column j
     float lam, mu;               // moduli of column j-4
     float dxv, dzv;              // damping of column j (velocity, PML lanes only)
-    float vxx, vxz, vzx, vzz;    // old velocity splits of column j (PML lanes only)
+    float vxx, vzz;              // old vx_x, vz_z of column j (PML lanes only)
     float dxs, dzs;              // damping of column j-4 (stress, PML lanes only)
-    float sxx, sxz, szx, szz, sex, sez;  // old stress splits of column j-4 (PML lanes only)
+    float sxx, szz, sxz;         // old txx_x, tzz_z, txz_x of column j-4 (PML lanes only)
 };
 }  // namespace
 
@@ -119,27 +120,24 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
         o.lam = ldc(Plam, cs);
         o.mu = ldc(Pmu, cs);
         // damping-strip operands: one branch per side (vupd/supd imply an in-window cell)
-        o.dxv = o.dzv = o.vxx = o.vxz = o.vzx = o.vzz = 0.f;
+        // One split of each component is kept (vx_x, vz_z, txx_x, tzz_z, txz_x);
+        // its partner is the unsplit value minus it.
+        o.dxv = o.dzv = o.vxx = o.vzz = 0.f;
         if (PML && vupd && damp_cell_f(S, j, z)) {
             const uint32_t c = zo4 + (uint32_t)j * colb;
             o.dxv = at(a.dampx, c);
             o.dzv = at(a.dampz, c);
             o.vxx = at(split(a.A, 0), c);
-            o.vxz = at(split(a.A, 1), c);
-            o.vzx = at(split(a.A, 2), c);
             o.vzz = at(split(a.A, 3), c);
         }
-        o.dxs = o.dzs = o.sxx = o.sxz = o.szx = o.szz = o.sex = o.sez = 0.f;
+        o.dxs = o.dzs = o.sxx = o.szz = o.sxz = 0.f;
         if (PML && supd && damp_cell_f(S, cs, z)) {
             const uint32_t c = zo4 + (uint32_t)cs * colb;
             o.dxs = at(a.dampx, c);
             o.dzs = at(a.dampz, c);
             o.sxx = at(split(a.A, 4), c);
-            o.sxz = at(split(a.A, 5), c);
-            o.szx = at(split(a.A, 6), c);
             o.szz = at(split(a.A, 7), c);
-            o.sex = at(split(a.A, 8), c);
-            o.sez = at(split(a.A, 9), c);
+            o.sxz = at(split(a.A, 8), c);
         }
         return o;
     };
@@ -212,14 +210,12 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
                 const float d1 = 1.f - dt * o.dxv / 2, d2 = 1.f + dt * o.dxv / 2;
                 const float d3 = 1.f - dt * o.dzv / 2, d4 = 1.f + dt * o.dzv / 2;
                 const float vxx = (d1 * o.vxx + bx * idx * ax) / d2;
-                const float vxz = (d3 * o.vxz + bx * idz * bz) / d4;
-                const float vzx = (d1 * o.vzx + bx * idx * ex) / d2;
+                const float vxz = (d3 * (o.vx - o.vxx) + bx * idz * bz) / d4;
+                const float vzx = (d1 * (o.vz - o.vzz) + bx * idx * ex) / d2;
                 const float vzz = (d3 * o.vzz + bx * idz * cz) / d4;
                 if (zown && j >= x0 && j < x1) {
                     const uint32_t c = zo4 + (uint32_t)j * colb;
                     st(splitB(0), c, vxx);
-                    st(splitB(1), c, vxz);
-                    st(splitB(2), c, vzx);
                     st(splitB(3), c, vzz);
                 }
                 nvx = vxx + vxz;
@@ -265,17 +261,14 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
                     const float d1 = 1.f - dt * o.dxs / 2, d2 = 1.f + dt * o.dxs / 2;
                     const float d3 = 1.f - dt * o.dzs / 2, d4 = 1.f + dt * o.dzs / 2;
                     const float a1 = (d1 * o.sxx + l2m * dvx) / d2;
-                    const float a2 = (d3 * o.sxz + l * dvz) / d4;
-                    const float b1 = (d1 * o.szx + l * dvx) / d2;
+                    const float a2 = (d3 * (txx - o.sxx) + l * dvz) / d4;
+                    const float b1 = (d1 * (tzz - o.szz) + l * dvx) / d2;
                     const float b2 = (d3 * o.szz + l2m * dvz) / d4;
-                    const float e1 = (d1 * o.sex + mu * dvzx) / d2;
-                    const float e2 = (d3 * o.sez + mu * dvxz) / d4;
+                    const float e1 = (d1 * o.sxz + mu * dvzx) / d2;
+                    const float e2 = (d3 * (txz - o.sxz) + mu * dvxz) / d4;
                     st(splitB(4), c, a1);
-                    st(splitB(5), c, a2);
-                    st(splitB(6), c, b1);
                     st(splitB(7), c, b2);
                     st(splitB(8), c, e1);
-                    st(splitB(9), c, e2);
                     txx = a1 + a2;
                     tzz = b1 + b2;
                     txz = e1 + e2;

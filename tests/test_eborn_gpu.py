@@ -152,3 +152,25 @@ def test_elsrtm_learned_trained_network_gain():
         assert l <= 1.05 * p
         gains.append(p / l)
     print(f"elastic learned gain (plain / learned residual after {niter} iterations): {np.mean(gains):.3f}")
+
+
+def test_eborn_rejects_pml_shots_and_acoustic_handles():
+    """The plain elastic pair needs the random boundary and an elastic handle;
+    both errors come back as status codes with a message."""
+    from ds2d_amd.lib import DS2DError
+    allnx, allnz, pml, nt, vp, vs, rho, wav, shots, pp, ps = _case(nt=50)
+    s = _solver(allnx, allnz, pml, nt, wav)
+    s.set_model(vp, vs, rho)
+    bad = [ShotSpec(0, 48, pml + 24, pml + 2, pml + 1, pml + np.arange(48, dtype=np.int32), boundary=0)]
+    with pytest.raises(DS2DError, match="random boundary"):
+        s.eborn_forward(bad, pp, ps)
+    with pytest.raises(DS2DError, match="random boundary"):
+        s.elsrtm_cg(bad, np.zeros(48 * nt, np.float32), np.zeros(48 * nt, np.float32), 1)
+    a = Solver(dx=DX, dz=DZ, dt=DT, nt=nt, allnx=allnx, allnz=allnz, pml=pml, nx_max=allnx, wavelet=wav, physics=1)
+    a.set_model(vp, None, rho)
+    with pytest.raises(DS2DError, match="elastic"):
+        a.eborn_forward(shots, pp, ps)
+    # zero data: CG refuses instead of dividing by zero
+    with pytest.raises(DS2DError, match="all zero"):
+        s.elsrtm_cg(shots, np.zeros(sum(x.ntr for x in shots) * nt, np.float32),
+                    np.zeros(sum(x.ntr for x in shots) * nt, np.float32), 1)
