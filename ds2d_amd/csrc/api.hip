@@ -76,7 +76,7 @@ struct ds2d_ctx {
     hipEvent_t ev[4]{};
     ds2d_stats last{};
     int chunk_w = 0;    // fused-kernel x-chunk width (0: default)
-    int chunk_t = 0;    // fused-kernel workgroup size (256/512/768; 0: default)
+    int chunk_t = 0;    // fused-kernel workgroup size (256/512/704/768; 0: default)
     int chunk_d = 0;    // fused-kernel prefetch depth in columns (2 or 4; 0: default 2)
     int use_graph = 0;  // capture time loops into one hipGraph per loop
     int per_launch = 0; // plain Born/adjoint: shots per time loop (0: all)
@@ -789,7 +789,8 @@ int ds2d_set_chunk(ds2d_t* h, int width, int threads, int depth) {
     int rc;
     if ((rc = check_h(h))) return rc;
     if (width < 0 || (width > 0 && width < 8)) return fail(DS2D_EINVAL, "chunk width must be 0 or >= 8");
-    if (threads != 0 && !fused_threads_ok(threads)) return fail(DS2D_EINVAL, "threads must be 0, 256, 512 or 768");
+    if (threads != 0 && !fused_threads_ok(threads))
+        return fail(DS2D_EINVAL, "threads must be 0, 256, 512, 704 or 768");
     if (depth != 0 && depth != 2 && depth != 4) return fail(DS2D_EINVAL, "depth must be 0, 2 or 4");
     h->chunk_w = width;
     h->chunk_t = threads;
@@ -808,8 +809,9 @@ int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nst
     struct Cand {
         int w, t, d;
     };
-    static const Cand cand[] = {{0, 0, 0},    {32, 256, 2},  {64, 256, 2}, {128, 256, 2}, {32, 256, 4},
-                                {64, 256, 4}, {128, 256, 4}, {64, 512, 2}, {64, 768, 2},  {64, 512, 4}};
+    static const Cand cand[] = {{0, 0, 0},     {32, 256, 2},  {64, 256, 2},  {64, 256, 4},
+                                {128, 256, 4}, {64, 512, 4},  {64, 768, 2},  {64, 768, 4},
+                                {128, 768, 4}, {64, 704, 4},  {128, 704, 4}, {32, 768, 4}};
     Cand best{64, 256, 2};
     double bt = 1e300;
     const int keep_w = h->chunk_w, keep_t = h->chunk_t, keep_d = h->chunk_d, keep_two = h->fwd_two;

@@ -176,7 +176,7 @@ struct FusedArgs {
 };
 // Fused addressing uses 32-bit byte offsets within each field.
 constexpr long long FUSED_MAX_CELLS = (1LL << 30) - (1LL << 20);
-// A fused workgroup is `threads` lanes (256, 512 or 768), one z row per lane:
+// A fused workgroup is `threads` lanes (256, 512, 704 or 768), one z row per lane:
 // FUSED_HALO rows above and below its block are loaded to feed the z taps,
 // so it owns threads - 2*FUSED_HALO rows.
 constexpr int FUSED_HALO = 8;

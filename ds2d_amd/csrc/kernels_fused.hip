@@ -314,7 +314,9 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
     }
 }
 
-bool fused_threads_ok(int threads) { return threads == 256 || threads == 512 || threads == 768; }
+// 704 lanes own 688 rows: one block covers a 600-row model with a 20-cell
+// frame (640 padded rows) with one wave fewer than 768.
+bool fused_threads_ok(int threads) { return threads == 256 || threads == 512 || threads == 704 || threads == 768; }
 
 void launch_fused_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, const ChunkDesc* chunks,
                        int nchunks, const int* rmap, int it, int threads, int depth, bool pml) {
@@ -329,6 +331,8 @@ void launch_fused_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, 
         DS2D_FUSED_LAUNCH(256, PP, MM);  \
     else if (threads == 512)             \
         DS2D_FUSED_LAUNCH(512, PP, MM);  \
+    else if (threads == 704)             \
+        DS2D_FUSED_LAUNCH(704, PP, MM);  \
     else                                 \
         DS2D_FUSED_LAUNCH(768, PP, MM);
     if (pml) {

@@ -128,7 +128,7 @@ int ds2d_forward(ds2d_t* h, float* rec_z, float* rec_x);
 int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec_z, float* rec_x,
                        int mode, ds2d_stats* stats);
 /* Tile of the fused kernel: x-chunk width (0 = default 64), workgroup size
- * (256, 512 or 768 lanes, one z row each, owning threads-16 rows; 0 =
+ * (256, 512, 704 or 768 lanes, one z row each, owning threads-16 rows; 0 =
  * default 256) and prefetch depth (columns of operands in flight, 2 or 4;
  * 0 = 2); and an autotuner that times nsteps per candidate on this batch --
  * the fused kernel at several tiles and the two-kernel step (reported as

@@ -58,7 +58,7 @@ def test_elastic_forward_batched_uneven(mode):
 
 
 @pytest.mark.parametrize("width,threads,depth", [(16, 256, 2), (128, 256, 2), (64, 512, 2), (40, 768, 2),
-                                                 (64, 256, 4), (24, 512, 4)])
+                                                 (64, 256, 4), (24, 512, 4), (48, 704, 4)])
 def test_fused_tiles_match_oracle(width, threads, depth):
     """Fused kernel with several tiles on a window deeper than one z block:
     the source sits on the first block boundary of the 256-lane tile (row 240)
