@@ -174,3 +174,19 @@ def test_eborn_rejects_pml_shots_and_acoustic_handles():
     with pytest.raises(DS2DError, match="all zero"):
         s.elsrtm_cg(shots, np.zeros(sum(x.ntr for x in shots) * nt, np.float32),
                     np.zeros(sum(x.ntr for x in shots) * nt, np.float32), 1)
+
+
+def test_eborn_one_shot_per_launch_matches_batched():
+    """ds2d_set_batch(1) runs the shots one time loop after another with the
+    same kernels: records and gradients equal the batched run bit for bit."""
+    allnx, allnz, pml, nt, vp, vs, rho, wav, shots, pp, ps = _case(nt=200)
+    out = []
+    for per_launch in (0, 1):
+        s = _solver(allnx, allnz, pml, nt, wav, bg_mode=1)
+        s.set_model(vp, vs, rho)
+        s.set_batch(per_launch)
+        rz, rx, _ = s.eborn_forward(shots, pp, ps)
+        gpp, gps, _ = s.eborn_adjoint(shots, rz, rx)
+        out.append((rz, rx, gpp, gps))
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
