@@ -1540,6 +1540,43 @@ int ds2d_eborn_adjoint(ds2d_t* h, int nshots, const ds2d_shot* shots, const floa
     return 0;
 }
 
+// ELSRTM_CALCULATE_ALPHA (FD2DGPU.cu:4885-5035) on the plain elastic pair:
+// r = L m - d, q = L g; num = <q, r>, den = <q, q> summed in FP64 over all
+// shots (the step length of m - alpha g is num / den).
+int ds2d_eborn_line_search(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* img_pp, const float* img_ps,
+                           const float* g_pp, const float* g_ps, const float* d_z, const float* d_x, double* num,
+                           double* den) {
+    int rc;
+    Batch B;
+    if ((rc = eb_begin(h, nshots, shots, B))) return rc;
+    if (!img_pp || !img_ps || !g_pp || !g_ps || !d_z || !d_x || !num || !den)
+        return fail(DS2D_EINVAL, "null line-search argument");
+    const long long A = (long long)h->g.allnx * h->g.allnz;
+    const long long R = std::max<long long>(B.total_rec, 1);
+    hipStream_t st = h->st;
+    float *m, *r;
+    if ((rc = h->fbuf("els_m", 4 * A, &m)) || (rc = h->fbuf("els_r", 2 * R, &r))) return rc;
+    HIP_TRY(hipMemcpyAsync(m, img_pp, A * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(m + A, img_ps, A * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(m + 2 * A, g_pp, A * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(m + 3 * A, g_ps, A * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(r, 0, 2 * R * 4, st));
+    HIP_TRY(hipMemcpyAsync(r, d_z, B.total_rec * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(r + R, d_x, B.total_rec * 4, hipMemcpyHostToDevice, st));
+    EbWS w;
+    if ((rc = eb_prepare(h, B, m, m + A, false, w))) return rc;
+    eb_fwd(h, B, w, true);
+    launch_axpby(st, r, w.rec, 2 * R, 1.f, -1.f);  // r = L m - d
+    if ((rc = eb_set_image(h, B, m + 2 * A, m + 3 * A, w))) return rc;
+    eb_fwd(h, B, w, true);  // q = L g in w.rec
+    *num = host_dot(h, w.rec, r, 2 * R, &rc);
+    if (rc) return rc;
+    *den = host_dot(h, w.rec, w.rec, 2 * R, &rc);
+    if (rc) return rc;
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 // CGLS on ||L [pp; ps] - [d_z; d_x]||^2 with the plain elastic Born pair.
 // Store mode: one background pass fills the strain history, then every
 // iteration is one scattered-field pass and one adjoint pass; otherwise each

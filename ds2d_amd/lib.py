@@ -82,6 +82,7 @@ SIGNATURES = {
     "ds2d_eborn_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, _F, _F, C.POINTER(Stats)]),
     "ds2d_eborn_adjoint": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, _F, _F, C.POINTER(Stats)]),
     "ds2d_elsrtm_cg": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.c_int, _F, _F, _D, C.POINTER(Stats)]),
+    "ds2d_eborn_line_search": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, _F, _F, _F, _F, _D, _D]),
     "ds2d_lsrtm_learned": (C.c_int, [_H, _H, C.c_int, C.POINTER(Shot), _F, C.c_int, _F, _D, C.POINTER(C.c_int),
                                      C.POINTER(Stats)]),
     "ds2d_elsrtm_learned": (C.c_int, [_H, _H, C.c_int, C.POINTER(Shot), _F, _F, C.c_int, _F, _F, _D,
@@ -307,6 +308,14 @@ class Solver:
         _check(self.lib.ds2d_eborn_adjoint(self.h, len(shots), _shots_array(shots), fptr(rz), fptr(rx), fptr(gpp),
                                            fptr(gps), C.byref(st)))
         return gpp.reshape(self.allnx, self.allnz), gps.reshape(self.allnx, self.allnz), st.as_dict()
+
+    def eborn_line_search(self, shots, img_pp, img_ps, g_pp, g_ps, d_z, d_x):
+        """(num, den): <L g, L m - d> and <L g, L g> (the reference's line-search sums)."""
+        a = [np.ascontiguousarray(x, np.float32) for x in (img_pp, img_ps, g_pp, g_ps, d_z, d_x)]
+        num, den = C.c_double(0), C.c_double(0)
+        _check(self.lib.ds2d_eborn_line_search(self.h, len(shots), _shots_array(shots), *[fptr(x) for x in a],
+                                               C.byref(num), C.byref(den)))
+        return num.value, den.value
 
     def elsrtm_cg(self, shots, d_z, d_x, niter, pp0=None, ps0=None):
         """CGLS PP/PS LSRTM on the elastic Born pair: (pp, ps, residual history, stats)."""

@@ -169,6 +169,12 @@ int ds2d_eborn_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, const floa
                        float* rec_z, float* rec_x, ds2d_stats* stats);
 int ds2d_eborn_adjoint(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* res_z, const float* res_x,
                        float* g_pp, float* g_ps, ds2d_stats* stats);
+/* ELSRTM_CALCULATE_ALPHA (FD2DGPU.cu:4885-5035, uncalled in the reference) on
+ * the plain elastic pair: r = L m - d, q = L g, *num = <q, r>, *den = <q, q>
+ * (FP64 over all shots; the step of m - alpha g is alpha = num / den). */
+int ds2d_eborn_line_search(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* img_pp, const float* img_ps,
+                           const float* g_pp, const float* g_ps, const float* d_z, const float* d_x, double* num,
+                           double* den);
 /* CGLS PP/PS LSRTM on the elastic Born pair; images in/out, resid[niter+1] as
  * for ds2d_lsrtm_cg. */
 int ds2d_elsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_z, const float* d_x, int niter,

@@ -190,3 +190,20 @@ def test_eborn_one_shot_per_launch_matches_batched():
         out.append((rz, rx, gpp, gps))
     for a, b in zip(out[0], out[1]):
         np.testing.assert_array_equal(a, b)
+
+
+def test_eborn_line_search_recovers_known_step():
+    """ELSRTM_CALCULATE_ALPHA's sums: with d = L m - c L g, the ratio
+    <L g, L m - d> / <L g, L g> is c."""
+    allnx, allnz, pml, nt, vp, vs, rho, wav, shots, pp, ps = _case(nt=200)
+    s = _solver(allnx, allnz, pml, nt, wav)
+    s.set_model(vp, vs, rho)
+    rng = np.random.default_rng(3)
+    gpp = (0.02 * rng.standard_normal((allnx, allnz))).astype(np.float32)
+    gps = (0.02 * rng.standard_normal((allnx, allnz))).astype(np.float32)
+    mz, mx, _ = s.eborn_forward(shots, pp, ps)
+    gz, gx, _ = s.eborn_forward(shots, gpp, gps)
+    c = 0.37
+    num, den = s.eborn_line_search(shots, pp, ps, gpp, gps, mz - c * gz, mx - c * gx)
+    assert den > 0
+    assert abs(num / den - c) <= 1e-4 * c
