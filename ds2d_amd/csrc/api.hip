@@ -811,7 +811,7 @@ int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nst
     };
     static const Cand cand[] = {{0, 0, 0},     {32, 256, 2},  {64, 256, 2},  {64, 256, 4},
                                 {128, 256, 4}, {64, 512, 4},  {64, 768, 2},  {64, 768, 4},
-                                {128, 768, 4}, {64, 704, 4},  {128, 704, 4}, {32, 768, 4}};
+                                {128, 768, 4}, {64, 704, 4},  {112, 704, 4}, {160, 704, 4}};
     Cand best{64, 256, 2};
     double bt = 1e300;
     const int keep_w = h->chunk_w, keep_t = h->chunk_t, keep_d = h->chunk_d, keep_two = h->fwd_two;
