@@ -1186,6 +1186,42 @@ long long padded_cells(const Batch& B) {
     return cu;
 }
 
+// CGLS iterations on ||L x - d||^2 (image of N floats, records of Rn).  On
+// entry: x, r = d - L x, rr = |r|^2, dd = |d|^2 and s = L^T r.  born(p)
+// leaves L p in q; adjoint(r) leaves L^T r in s.  resid[k] = |r_k| / |d|.
+template <class BornF, class AdjF>
+int cgls_loop(ds2d_ctx* h, long long N, long long Rn, float* x, float* r, float* p, float* s, const float* q,
+              double rr, double dd, BornF born, AdjF adjoint, int niter, double* resid) {
+    hipStream_t st = h->st;
+    int rc = 0;
+    HIP_TRY(hipMemcpyAsync(p, s, N * 4, hipMemcpyDeviceToDevice, st));
+    double gamma = host_dot(h, s, s, N, &rc);
+    if (rc) return rc;
+    for (int k = 0; k < niter; ++k) {
+        if (gamma <= 0) {
+            if (resid)
+                for (int j = k; j < niter; ++j) resid[j + 1] = std::sqrt(rr / dd);
+            break;
+        }
+        if ((rc = born(p))) return rc;  // q = L p
+        const double qq = host_dot(h, q, q, Rn, &rc);
+        if (rc) return rc;
+        const double alpha = qq > 0 ? gamma / qq : 0.0;
+        launch_axpby(st, x, p, N, (float)alpha, 1.f);
+        launch_axpby(st, r, q, Rn, (float)-alpha, 1.f);
+        rr = host_dot(h, r, r, Rn, &rc);
+        if (rc) return rc;
+        if (resid) resid[k + 1] = std::sqrt(rr / dd);
+        if ((rc = adjoint(r))) return rc;  // s = L^T r
+        const double gnew = host_dot(h, s, s, N, &rc);
+        if (rc) return rc;
+        const double beta = gamma > 0 ? gnew / gamma : 0.0;
+        launch_axpby(st, p, s, N, 1.f, (float)beta);  // p = s + beta p
+        gamma = gnew;
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1273,47 +1309,25 @@ int ds2d_lsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_
     const double dd = host_dot(h, d, d, B.total_rec, &rc);
     if (rc) return rc;
     if (dd <= 0) return fail(DS2D_EINVAL, "observed data are all zero");
-    double rr = host_dot(h, r, r, B.total_rec, &rc);
+    const double rr = host_dot(h, r, r, B.total_rec, &rc);
     if (rc) return rc;
     if (resid) resid[0] = std::sqrt(rr / dd);
-    // s = L^T r
-    HIP_TRY(hipMemcpyAsync(w.rec2, r, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
-    acou_bwd(h, B, w);
-    props += w.store ? 1 : 2;
-    if ((rc = acou_grad_to_global(h, B, w, s))) return rc;
-    HIP_TRY(hipMemcpyAsync(p, s, A * 4, hipMemcpyDeviceToDevice, st));
-    double gamma = host_dot(h, s, s, A, &rc);
-    if (rc) return rc;
-    for (int k = 0; k < niter; ++k) {
-        if (gamma <= 0) {
-            if (resid)
-                for (int j = k; j < niter; ++j) resid[j + 1] = std::sqrt(rr / dd);
-            break;
-        }
-        // q = L p
-        if ((rc = acou_set_refl(h, B, p, w))) return rc;
+    auto born = [&](const float* m) -> int {  // q = L m
+        int e;
+        if ((e = acou_set_refl(h, B, m, w))) return e;
         acou_fwd(h, B, w, !w.store, true, false);
         props += w.store ? 1 : 2;
         HIP_TRY(hipMemcpyAsync(q, w.rec2, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
-        const double qq = host_dot(h, q, q, B.total_rec, &rc);
-        if (rc) return rc;
-        const double alpha = qq > 0 ? gamma / qq : 0.0;
-        launch_axpby(st, x, p, A, (float)alpha, 1.f);
-        launch_axpby(st, r, q, B.total_rec, (float)-alpha, 1.f);
-        rr = host_dot(h, r, r, B.total_rec, &rc);
-        if (rc) return rc;
-        if (resid) resid[k + 1] = std::sqrt(rr / dd);
-        // s = L^T r
-        HIP_TRY(hipMemcpyAsync(w.rec2, r, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
+        return 0;
+    };
+    auto adjoint = [&](const float* res) -> int {  // s = L^T res
+        HIP_TRY(hipMemcpyAsync(w.rec2, res, B.total_rec * 4, hipMemcpyDeviceToDevice, st));
         acou_bwd(h, B, w);
         props += w.store ? 1 : 2;
-        if ((rc = acou_grad_to_global(h, B, w, s))) return rc;
-        const double gnew = host_dot(h, s, s, A, &rc);
-        if (rc) return rc;
-        const double beta = gamma > 0 ? gnew / gamma : 0.0;
-        launch_axpby(st, p, s, A, 1.f, (float)beta);  // p = s + beta p
-        gamma = gnew;
-    }
+        return acou_grad_to_global(h, B, w, s);
+    };
+    if ((rc = adjoint(r))) return rc;
+    if ((rc = cgls_loop(h, A, B.total_rec, x, r, p, s, q, rr, dd, born, adjoint, niter, resid))) return rc;
     stats_loop_end(h);
     if ((rc = stats_end(h, stats, cells * h->g.nt * props, 0, 0))) return rc;
     HIP_TRY(hipMemcpyAsync(image, x, A * 4, hipMemcpyDeviceToHost, st));
@@ -1613,45 +1627,24 @@ int ds2d_elsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d
     const double dd = host_dot(h, d, d, 2 * R, &rc);
     if (rc) return rc;
     if (dd <= 0) return fail(DS2D_EINVAL, "observed data are all zero");
-    double rr = host_dot(h, r, r, 2 * R, &rc);
+    const double rr = host_dot(h, r, r, 2 * R, &rc);
     if (rc) return rc;
     if (resid) resid[0] = std::sqrt(rr / dd);
-    // s = L^T r
-    auto adjoint = [&]() -> int {
-        HIP_TRY(hipMemcpyAsync(w.res, r, 2 * R * 4, hipMemcpyDeviceToDevice, st));
+    auto born = [&](const float* m) -> int {  // L m in w.rec
+        int e;
+        if ((e = eb_set_image(h, B, m, m + A, w))) return e;
+        eb_fwd(h, B, w, !w.store);
+        props += w.store ? 1 : 2;
+        return 0;
+    };
+    auto adjoint = [&](const float* res) -> int {  // s = L^T res
+        HIP_TRY(hipMemcpyAsync(w.res, res, 2 * R * 4, hipMemcpyDeviceToDevice, st));
         eb_bwd(h, B, w);
         props += w.store ? 1 : 2;
         return eb_grad_to_global(h, B, w, s);
     };
-    if ((rc = adjoint())) return rc;
-    HIP_TRY(hipMemcpyAsync(p, s, 2 * A * 4, hipMemcpyDeviceToDevice, st));
-    double gamma = host_dot(h, s, s, 2 * A, &rc);
-    if (rc) return rc;
-    for (int k = 0; k < niter; ++k) {
-        if (gamma <= 0) {
-            if (resid)
-                for (int j = k; j < niter; ++j) resid[j + 1] = std::sqrt(rr / dd);
-            break;
-        }
-        // q = L p (w.rec)
-        if ((rc = eb_set_image(h, B, p, p + A, w))) return rc;
-        eb_fwd(h, B, w, !w.store);
-        props += w.store ? 1 : 2;
-        const double qq = host_dot(h, w.rec, w.rec, 2 * R, &rc);
-        if (rc) return rc;
-        const double alpha = qq > 0 ? gamma / qq : 0.0;
-        launch_axpby(st, x, p, 2 * A, (float)alpha, 1.f);
-        launch_axpby(st, r, w.rec, 2 * R, (float)-alpha, 1.f);
-        rr = host_dot(h, r, r, 2 * R, &rc);
-        if (rc) return rc;
-        if (resid) resid[k + 1] = std::sqrt(rr / dd);
-        if ((rc = adjoint())) return rc;
-        const double gnew = host_dot(h, s, s, 2 * A, &rc);
-        if (rc) return rc;
-        const double beta = gamma > 0 ? gnew / gamma : 0.0;
-        launch_axpby(st, p, s, 2 * A, 1.f, (float)beta);  // p = s + beta p
-        gamma = gnew;
-    }
+    if ((rc = adjoint(r))) return rc;
+    if ((rc = cgls_loop(h, 2 * A, 2 * R, x, r, p, s, w.rec, rr, dd, born, adjoint, niter, resid))) return rc;
     stats_loop_end(h);
     if ((rc = stats_end(h, stats, eb_cells(B, h->g.nt) * props, 0, 0))) return rc;
     HIP_TRY(hipMemcpyAsync(img_pp, x, A * 4, hipMemcpyDeviceToHost, st));
