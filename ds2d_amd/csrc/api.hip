@@ -1463,16 +1463,19 @@ void eb_bg(ds2d_ctx* h, const Batch& B, EbWS& w) {
 void eb_bwd(ds2d_ctx* h, const Batch& B, EbWS& w) {
     hipStream_t st = h->st;
     (void)hipMemsetAsync(w.lm.vx, 0, (size_t)w.n * 12 * 4, st);  // lm + gpp + gps + adjoint products
-    const int nt = h->g.nt;
-    for (const BatchShape& sh : launch_groups(h, B))
+    const int nt = h->g.nt, undo = w.store ? 0 : 1;
+    for (const BatchShape& sh : launch_groups(h, B)) {
+        // Without the history the background is rebuilt alongside: the stress of
+        // step nt-1 comes off here, then each iteration takes the velocity of
+        // step it+1 (with k_eb_adj_v) and the stress of step it (with k_eb_adj_s).
+        if (undo && nt >= 2)
+            launch_eb_s(st, sh, w.bg, w.bn, w.m, nullptr, w.H, nt - 1, -1.f, 1, 0, 0, w.rmap, nullptr, nullptr);
         for (int it = nt - 2; it >= 0; --it) {
-            if (!w.store) {  // background of step it from that of step it + 1
-                launch_eb_s(st, sh, w.bg, w.bn, w.m, nullptr, w.H, it + 1, -1.f, 1, 0, 0, w.rmap, nullptr, nullptr);
-                launch_eb_v(st, sh, w.bg, w.bn, w.m.rho, h->wav[it + 1], -1.f, 1, 0);
-            }
-            launch_eb_adj_v(st, sh, w.lm, w.aux, w.m, w.rmap, w.res, w.res + w.R, it + 1);
-            launch_eb_adj_s(st, sh, w.lm, w.aux, w.bg, w.m, w.store ? w.hist : nullptr, w.H, w.gpp, w.gps, it);
+            launch_eb_adj_v(st, sh, w.lm, w.aux, w.bg, w.m, w.rmap, w.res, w.res + w.R, it + 1, undo,
+                            h->wav[it + 1]);
+            launch_eb_adj_s(st, sh, w.lm, w.aux, w.bg, w.m, w.store ? w.hist : nullptr, w.H, w.gpp, w.gps, it, undo);
         }
+    }
 }
 
 // Padded per-slot gradients -> global PP then PS (deterministic slot order), overwrite.

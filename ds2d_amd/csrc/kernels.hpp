@@ -152,11 +152,13 @@ void launch_eb_s(hipStream_t st, const BatchShape& b, EbField bg, EbField born, 
 // Adjoint step: k_eb_adj_v (receiver injection of step it_inj, -1 none, and the
 // transpose of the stress update), then k_eb_adj_s (transpose of the velocity
 // update and the gradient of step it; strains from hist, or from bg when
-// hist is null).
-void launch_eb_adj_v(hipStream_t st, const BatchShape& b, EbField lm, EbAux x, EbMed m, const int* rmap,
-                     const float* res_z, const float* res_x, int it_inj);
+// hist is null).  undo = 1 (no history): k_eb_adj_v also takes the background
+// velocity one step back (source sample w) and k_eb_adj_s the background
+// stress, so the background is rebuilt alongside the adjoint.
+void launch_eb_adj_v(hipStream_t st, const BatchShape& b, EbField lm, EbAux x, EbField bg, EbMed m,
+                     const int* rmap, const float* res_z, const float* res_x, int it_inj, int undo, float w);
 void launch_eb_adj_s(hipStream_t st, const BatchShape& b, EbField lm, EbAux x, EbField bg, EbMed m,
-                     const float* hist, long long hstride, float* gpp, float* gps, int it);
+                     const float* hist, long long hstride, float* gpp, float* gps, int it, int undo);
 
 // --- fused single-kernel elastic step (kernels_fused.hip) -------------------
 struct ChunkDesc {

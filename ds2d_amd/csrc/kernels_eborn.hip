@@ -166,13 +166,30 @@ __global__ __launch_bounds__(256) void k_eb_s(EbField bg, EbField b, EbMed m, fl
 // Adjoint, part 1: v* += inj(step it_inj) - (dt/d) D (C s*), reading the
 // products C s* that part 2 left (zero outside the update region); leaves
 // u = (1/rho) v* for part 2.
-__global__ __launch_bounds__(256) void k_eb_adj_v(EbField lm, EbAux x, EbMed m, const SlotDesc* __restrict__ slots,
-                                                  float ax, float az, const int* __restrict__ rmap,
-                                                  const float* res_z, const float* res_x, int nt, int it_inj) {
+// undo = 1 also takes one step off the background velocity, v -= (dt/d) D sigma
+// with the source sample w (the background stress is already one step back).
+__global__ __launch_bounds__(256) void k_eb_adj_v(EbField lm, EbAux x, EbField bg, EbMed m,
+                                                  const SlotDesc* __restrict__ slots, float ax, float az,
+                                                  const int* __restrict__ rmap, const float* res_z,
+                                                  const float* res_x, int nt, int it_inj, int undo, float w) {
     const SlotDesc S = slots[blockIdx.z];
     Cell t;
     if (!cell_u(S, t)) return;
     const long long c = (long long)t.c, s = S.nzp;
+    const float br = 1.f / m.rho[c];
+    if (undo) {
+        const long long src = S.off + (long long)S.isx * S.nzp + S.isz;
+        auto txx = [&](int k) {
+            const long long j = c + (long long)k * s;
+            return j == src ? bg.txx[j] + w : bg.txx[j];
+        };
+        auto tzz = [&](int k) {
+            const long long j = c + k;
+            return j == src ? bg.tzz[j] + w : bg.tzz[j];
+        };
+        bg.vx[c] += -(br * ax * dplus(txx) + br * az * dminus(EB_Z(bg.txz)));
+        bg.vz[c] += -(br * ax * dminus(EB_X(bg.txz)) + br * az * dplus(tzz));
+    }
     float ix_ = 0.f, iz_ = 0.f;
     if (it_inj >= 0 && t.iz == S.igz) {
         const int r = rmap[S.rmap_off + t.ix];
@@ -186,7 +203,6 @@ __global__ __launch_bounds__(256) void k_eb_adj_v(EbField lm, EbAux x, EbMed m, 
     const float vz = lm.vz[c] + iz_ - (az * dplus(EB_Z(x.wb)) + ax * dminus(EB_X(x.wc)));
     lm.vx[c] = vx;
     lm.vz[c] = vz;
-    const float br = 1.f / m.rho[c];
     x.ux[c] = br * vx;
     x.uz[c] = br * vz;
 }
@@ -195,10 +211,12 @@ __global__ __launch_bounds__(256) void k_eb_adj_v(EbField lm, EbAux x, EbMed m, 
 // the update region); leaves C s* for the next part 1; then the gradient of
 // step it at the physical window: the strains come from the history
 // (hist != null) or from the background velocity.
+// undo = 1 (no history) also takes the background stress one step back,
+// sigma -= (dt/d) C D v, from the same velocity differences as the strains.
 __global__ __launch_bounds__(256) void k_eb_adj_s(EbField lm, EbAux x, EbField bg, EbMed m,
                                                   const float* __restrict__ hist, long long hstride, float* gpp,
                                                   float* gps, const SlotDesc* __restrict__ slots, float ax, float az,
-                                                  float idx, float idz, int it) {
+                                                  float idx, float idz, int it, int undo) {
     const SlotDesc S = slots[blockIdx.z];
     Cell t;
     if (!cell_u(S, t)) return;
@@ -214,14 +232,26 @@ __global__ __launch_bounds__(256) void k_eb_adj_s(EbField lm, EbAux x, EbField b
     x.wb[c] = l * txx + l2m * tzz;
     x.wc[c] = mu * txz;
     size_t h;
-    if (!hist_at(S, t.ix, t.iz, it, h)) return;
-    float exx, ezz, exz;
+    const bool inside = hist_at(S, t.ix, t.iz, it, h);
+    float exx = 0.f, ezz = 0.f, exz = 0.f;
     if (hist) {
+        if (!inside) return;
         exx = hist[h];
         ezz = hist[h + hstride];
         exz = hist[h + 2 * hstride];
     } else {
-        strains(bg, c, s, idx, idz, exx, ezz, exz);
+        if (!inside && !undo) return;
+        const float dvx = dminus(EB_X(bg.vx)), dvz = dminus(EB_Z(bg.vz));
+        const float dvxz = dplus(EB_Z(bg.vx)), dvzx = dplus(EB_X(bg.vz));
+        exx = dvx * idx;
+        ezz = dvz * idz;
+        exz = dvzx * idx + dvxz * idz;
+        if (undo) {
+            bg.txx[c] += -(ax * l2m * dvx + az * l * dvz);
+            bg.tzz[c] += -(ax * l * dvx + az * l2m * dvz);
+            bg.txz[c] += -(mu * (ax * dvzx + az * dvxz));
+        }
+        if (!inside) return;
     }
     gpp[c] += -2 * l2m * (exx + ezz) * (txx + tzz);
     gps[c] += 4 * mu * (ezz * txx + exx * tzz) - 2 * mu * exz * txz;
@@ -243,15 +273,15 @@ void launch_eb_s(hipStream_t st, const BatchShape& b, EbField bg, EbField born, 
                        b.dt / b.dz, 1.f / b.dx, 1.f / b.dz, it, sign, do_bg, do_born, hist_mode, rmap, rec_z, rec_x,
                        b.nt);
 }
-void launch_eb_adj_v(hipStream_t st, const BatchShape& b, EbField lm, EbAux x, EbMed m, const int* rmap,
-                     const float* res_z, const float* res_x, int it_inj) {
-    hipLaunchKernelGGL(k_eb_adj_v, egrid(b), dim3(BZ, BX), 0, st, lm, x, m, b.slots, b.dt / b.dx, b.dt / b.dz, rmap,
-                       res_z, res_x, b.nt, it_inj);
+void launch_eb_adj_v(hipStream_t st, const BatchShape& b, EbField lm, EbAux x, EbField bg, EbMed m,
+                     const int* rmap, const float* res_z, const float* res_x, int it_inj, int undo, float w) {
+    hipLaunchKernelGGL(k_eb_adj_v, egrid(b), dim3(BZ, BX), 0, st, lm, x, bg, m, b.slots, b.dt / b.dx, b.dt / b.dz,
+                       rmap, res_z, res_x, b.nt, it_inj, undo, w);
 }
 void launch_eb_adj_s(hipStream_t st, const BatchShape& b, EbField lm, EbAux x, EbField bg, EbMed m,
-                     const float* hist, long long hstride, float* gpp, float* gps, int it) {
+                     const float* hist, long long hstride, float* gpp, float* gps, int it, int undo) {
     hipLaunchKernelGGL(k_eb_adj_s, egrid(b), dim3(BZ, BX), 0, st, lm, x, bg, m, hist, hstride, gpp, gps, b.slots,
-                       b.dt / b.dx, b.dt / b.dz, 1.f / b.dx, 1.f / b.dz, it);
+                       b.dt / b.dx, b.dt / b.dz, 1.f / b.dx, 1.f / b.dz, it, undo);
 }
 
 }  // namespace ds2d
