@@ -67,7 +67,7 @@ def surface_shot(nx_win, ngx_min, pml, sx_local, sz=2, gz=0, shot_id=0, boundary
 
 
 # ---------------------------------------------------------------------------
-# Workloads
+# Workloads (the benchmark configurations C1-C5)
 # ---------------------------------------------------------------------------
 
 def c1_elastic_forward(seed=1):
