@@ -129,8 +129,115 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
         }
 }
 
+// Same GEMM with the input staged through LDS: a block's 128 * WM pixels are
+// R = 128 * WM / w whole image rows; per K step of 16 channels the block loads
+// those rows plus a one-pixel frame ((R + 2) x (w + 2) pixels, 32 B each, zero
+// outside the image, skip concatenation and upsampling resolved in the
+// address) once, and all nine taps read their A operands from LDS.
+template <int WM, int WN>
+__global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
+    extern __shared__ u32x4 tile[];  // [(R + 2) * (w + 2)][2]: 16 channels of one pixel
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 31, kh = lane >> 5;
+    constexpr int P = 4 * WM * 32;
+    const int w = a.w, h = a.h, R = P / w, wp = w + 2;
+    const long long pb0 = (long long)blockIdx.x * P;
+    const int nimg = (int)(pb0 / ((long long)h * w));
+    const int x0 = (int)((pb0 - (long long)nimg * h * w) / w);
+    const int co0 = blockIdx.y * (WN * 32);
+    const int ctot = a.c1 + a.c2;
+    const int h2 = h >> a.up, w2 = w >> a.up;
+    const int npx = (R + 2) * wp;
+    int lp[WM];  // LDS pixel of each tile row's pixel at tap (0, 0)
+#pragma unroll
+    for (int m = 0; m < WM; ++m) {
+        const int i = wave * WM * 32 + m * 32 + li;
+        lp[m] = (i / w + 1) * wp + (i % w) + 1;
+    }
+    f32x16 acc[WM][WN];
+#pragma unroll
+    for (int m = 0; m < WM; ++m)
+#pragma unroll
+        for (int j = 0; j < WN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[m][j][r] = 0.f;
+    const u32x4 zero = {0u, 0u, 0u, 0u};
+    for (int c = 0; c < ctot; c += 16) {
+        for (int q = tid; q < 2 * npx; q += 256) {
+            const int px = q >> 1, half = q & 1;
+            const int x = x0 - 1 + px / wp, z = px % wp - 1;
+            u32x4 v = zero;
+            if (x >= 0 && x < h && z >= 0 && z < w)
+                v = c < a.c1 ? ld16(a.x1 + (((long long)nimg * h + x) * w + z) * a.c1 + c + 8 * half)
+                             : ld16(a.x2 + (((long long)nimg * h2 + (x >> a.up)) * w2 + (z >> a.up)) * a.c2 +
+                                    (c - a.c1) + 8 * half);
+            tile[q] = v;
+        }
+        __syncthreads();
+#pragma unroll 3
+        for (int tap = 0; tap < 9; ++tap) {
+            const int off = (tap / 3 - 1) * wp + (tap % 3 - 1);
+            const bf16_t* wt = a.wt + ((long long)tap * a.cout + co0 + li) * ctot + c + 8 * kh;
+            u32x4 A[WM], B[WN];
+#pragma unroll
+            for (int j = 0; j < WN; ++j) B[j] = ld16(wt + (long long)j * 32 * ctot);
+#pragma unroll
+            for (int m = 0; m < WM; ++m) A[m] = tile[2 * (lp[m] + off) + kh];
+#pragma unroll
+            for (int m = 0; m < WM; ++m)
+#pragma unroll
+                for (int j = 0; j < WN; ++j) acc[m][j] = mfma(A[m], B[j], acc[m][j]);
+        }
+        __syncthreads();
+    }
+    const long long pbase = pb0 + (long long)wave * WM * 32;
+    const int c2o = a.cout - a.split;
+#pragma unroll
+    for (int m = 0; m < WM; ++m)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) {
+            const int co = co0 + j * 32 + li;
+            const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = (r & 3) + 4 * kh + 8 * (r >> 2);
+                const long long p = pbase + m * 32 + i;
+                float v = acc[m][j][r] + b;
+                if (a.relu) v = fmaxf(v, 0.f);
+                if (a.mask && !(bf2f(a.mask[p * a.cout + co]) > 0.f)) v = 0.f;
+                if (co < a.split) {
+                    if (a.out_bf16)
+                        ((bf16_t*)a.y1)[p * a.split + co] = f2bf(v);
+                    else
+                        ((float*)a.y1)[p * a.split + co] = v;
+                } else {
+                    if (a.out_bf16)
+                        ((bf16_t*)a.y2)[p * c2o + co - a.split] = f2bf(v);
+                    else
+                        ((float*)a.y2)[p * c2o + co - a.split] = v;
+                }
+            }
+        }
+}
+
+// The LDS kernel takes blocks of whole image rows: w divides the block's
+// pixels, the block's pixels divide one image, channels come in 16s.
+bool conv_lds_ok(const ConvArgs& a, int P) {
+    return a.w > 0 && P % a.w == 0 && ((long long)a.h * a.w) % P == 0 && a.c1 % 16 == 0 && a.c2 % 16 == 0;
+}
+
 void launch_conv(hipStream_t st, const ConvArgs& a) {
     const long long npix = (long long)a.n * a.h * a.w;
+    if (a.cout % 64 == 0 && conv_lds_ok(a, 256)) {
+        const size_t lds = (size_t)(256 / a.w + 2) * (a.w + 2) * 32;
+        hipLaunchKernelGGL((k_conv_lds<2, 2>), dim3((unsigned)(npix / 256), a.cout / 64), dim3(256), lds, st, a);
+        return;
+    }
+    if (a.cout % 64 != 0 && conv_lds_ok(a, 512)) {
+        const size_t lds = (size_t)(512 / a.w + 2) * (a.w + 2) * 32;
+        hipLaunchKernelGGL((k_conv_lds<4, 1>), dim3((unsigned)(npix / 512), a.cout / 32), dim3(256), lds, st, a);
+        return;
+    }
     if (a.cout % 64 == 0) {
         dim3 grid((unsigned)((npix + 255) / 256), a.cout / 64);
         hipLaunchKernelGGL((k_conv<2, 2>), grid, dim3(256), 0, st, a);

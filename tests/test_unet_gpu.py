@@ -46,6 +46,28 @@ def test_conv_forward_matches_oracle(c1, c2, up, cout):
     assert e <= 1e-5
 
 
+@pytest.mark.parametrize("c1,c2,up,cout,h,w", [(32, 0, 0, 32, 16, 32), (64, 0, 0, 64, 8, 64), (32, 64, 1, 32, 16, 32),
+                                               (64, 64, 0, 128, 8, 32), (128, 256, 1, 128, 16, 16),
+                                               (32, 32, 0, 64, 4, 256)])
+def test_conv_lds_kernel_matches_oracle(c1, c2, up, cout, h, w):
+    """Shapes whose blocks are whole image rows take the LDS-staged kernel
+    (forward with skip/upsampled input, and the data gradient)."""
+    from ds2d_amd.lib import conv3x3_bf16
+    n = 2
+    x1 = U.bf16(rnd((n, h, w, c1), 21))
+    x2 = U.bf16(rnd((n, h >> up, w >> up, c2), 22)) if c2 else None
+    wt = U.bf16(rnd((9, cout, c1 + c2), 23) * 0.1)
+    b = rnd((cout,), 24)
+    y, _ = conv3x3_bf16(0, x1, wt, x2=x2, up=up, bias=b, relu=True)
+    inp = x1 if not c2 else np.concatenate([x1, U.up2(x2) if up else x2], axis=-1)
+    e = rel(y, np.maximum(U.conv3x3(inp, wt, b), 0))
+    dy = U.bf16(rnd((n, h, w, cout), 25))
+    dx, _ = conv3x3_bf16(1, dy, wt)
+    e2 = rel(dx, U.conv3x3_dgrad(dy, wt))
+    print(f"conv (LDS) c1={c1} c2={c2} up={up} cout={cout} {h}x{w}: forward rel {e:.2e}, dgrad rel {e2:.2e}")
+    assert e <= 1e-5 and e2 <= 1e-5
+
+
 @pytest.mark.parametrize("cin,cout", [(32, 32), (64, 32), (96, 32), (192, 64), (64, 128)])
 def test_conv_data_gradient_matches_oracle(cin, cout):
     from ds2d_amd.lib import conv3x3_bf16
