@@ -512,20 +512,33 @@ void launch_conv1_wgrad(hipStream_t st, const float* x, const bf16_t* dy, float*
     hipLaunchKernelGGL(k_conv1_wgrad, dim3(conv1_wgrad_blocks(npix)), dim3(256), 0, st, x, dy, part, n, h, wd, cout);
 }
 
-// part[blk][co] = sum_{p in blk} dy[p, co]; cout divides 256
+// part[blk][co] = sum_{p in blk} dy[p, co]; cout a multiple of 8 dividing 2048.
+// Each thread reads 8 channels of a pixel (one 16-byte load) per pass; the
+// per-thread sums are combined in a fixed order.
 __global__ __launch_bounds__(256) void k_bias_grad(const bf16_t* dy, long long npix, int cout, float* part) {
-    __shared__ float red[256];
+    __shared__ float red[256 * 8];
     const long long per = (npix + gridDim.x - 1) / gridDim.x;
     const long long p0 = blockIdx.x * per, p1 = std::min(npix, p0 + per);
-    const int c = threadIdx.x % cout, sub = threadIdx.x / cout, nsub = 256 / cout;
-    float acc = 0.f;
-    for (long long p = p0 + sub; p < p1; p += nsub) acc += bf2f(dy[p * cout + c]);
-    red[threadIdx.x] = acc;
+    const int ng = cout / 8, g = threadIdx.x % ng, sub = threadIdx.x / ng, nsub = 256 / ng;
+    float acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+    for (long long p = p0 + sub; p < p1; p += nsub) {
+        const u32x4 v = ld16(dy + p * cout + 8 * g);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            acc[2 * k] += __uint_as_float(v[k] << 16);
+            acc[2 * k + 1] += __uint_as_float(v[k] & 0xFFFF0000u);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) red[threadIdx.x * 8 + k] = acc[k];
     __syncthreads();
-    if (threadIdx.x < cout) {
-        float s = 0.f;
-        for (int k = 0; k < nsub; ++k) s += red[k * cout + c];
-        part[(long long)blockIdx.x * cout + c] = s;
+    for (int c = threadIdx.x; c < cout; c += 256) {
+        const int gc = c / 8, e = c % 8;
+        float sum = 0.f;
+        for (int k = 0; k < nsub; ++k) sum += red[(k * ng + gc) * 8 + e];
+        part[(long long)blockIdx.x * cout + c] = sum;
     }
 }
 
