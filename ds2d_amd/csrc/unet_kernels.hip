@@ -489,11 +489,18 @@ __global__ __launch_bounds__(256) void k_conv1_wgrad(const float* x, const bf16_
     const long long p0 = blockIdx.x * per, p1 = std::min(npix, p0 + per);
     const int c = threadIdx.x % cout, sub = threadIdx.x / cout, nsub = 256 / cout;
     float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (long long p = p0 + sub; p < p1; p += nsub) {
-        const int iz = (int)(p % w), ix = (int)((p / w) % h), in = (int)(p / ((long long)w * h));
+    // pixel coordinates of p, advanced by nsub per pass without divisions
+    const long long q0 = p0 + sub;
+    int iz = (int)(q0 % w), ix = (int)((q0 / w) % h), in = (int)(q0 / ((long long)w * h));
+    for (long long p = q0; p < p1; p += nsub) {
         const float g = bf2f(dy[p * cout + c]);
 #pragma unroll
         for (int t = 0; t < 9; ++t) acc[t] += g * tap_in(x, in, h, w, ix, iz, t);
+        iz += nsub;
+        while (iz >= w) {
+            iz -= w;
+            if (++ix == h) ix = 0, ++in;
+        }
     }
 #pragma unroll
     for (int t = 0; t < 9; ++t) red[t][threadIdx.x] = acc[t];
