@@ -253,14 +253,26 @@ void launch_conv(hipStream_t st, const ConvArgs& a) {
 constexpr int ZS = 64;  // z segment (pixels of K per row)
 constexpr int LP = 72;  // LDS row pitch in elements (144 B: 16-byte aligned rows)
 
-template <int TCO, int TCI>
-__global__ __launch_bounds__(64 * (TCO / 32) * (TCI / 32)) void k_wgrad(WgradArgs a) {
-    constexpr int NT = 64 * (TCO / 32) * (TCI / 32);
-    __shared__ __attribute__((aligned(16))) bf16_t As[TCO][LP];
-    __shared__ __attribute__((aligned(16))) bf16_t Bs[3][TCI][LP];
+// Block = (TCO/32) x (TCI/32) x DS waves; wave (cot, cit, dsel) accumulates the
+// 32x32 (co, ci) tile of the taps dx = d - 1 with d = dsel, dsel + DS, ... < 3.
+// Input rows live in a 4-slot LDS ring (row r in slot r & 3, r >= -1) and dy
+// rows in two buffers, so the global loads of rows x + 2 (input) and x + 1
+// (dy) are issued before the MFMAs of row x and land in LDS after them, with
+// one barrier per row.
+template <int TCO, int TCI, int DS>
+__global__ __launch_bounds__(64 * (TCO / 32) * (TCI / 32) * DS) void k_wgrad(WgradArgs a) {
+    constexpr int NT = 64 * (TCO / 32) * (TCI / 32) * DS;
+    constexpr int GI = TCI / 8, GO = TCO / 8;
+    constexpr int QI = (ZS + 2) * GI, QO = ZS * GO;
+    constexpr int NI = (QI + NT - 1) / NT, NO = (QO + NT - 1) / NT;
+    constexpr int ND = 3 / DS;
+    static_assert(DS == 1 || DS == 3, "taps split over 1 or 3 waves");
+    __shared__ __attribute__((aligned(16))) bf16_t As[2][TCO][LP];
+    __shared__ __attribute__((aligned(16))) bf16_t Bs[4][TCI][LP];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 31, kh = lane >> 5;
-    const int cot = wave % (TCO / 32), cit = wave / (TCO / 32);
+    const int cot = wave % (TCO / 32), cit = (wave / (TCO / 32)) % (TCI / 32);
+    const int dsel = wave / ((TCO / 32) * (TCI / 32));
     const int cin = a.c1 + a.c2;
     const int ntci = cin / TCI;
     const int co0 = (blockIdx.y / ntci) * TCO, ci0 = (blockIdx.y % ntci) * TCI;
@@ -273,80 +285,109 @@ __global__ __launch_bounds__(64 * (TCO / 32) * (TCI / 32)) void k_wgrad(WgradArg
     const int h2 = a.h >> a.up, w2 = a.w >> a.up;
     const int nks = (min(ZS, a.w - zs) + 15) / 16;
 
-    // Bs[slot][ci][zz] = in[n][x][zs + zz - 1][ci0 + ci] for zz in [0, ZS + 2), zero outside the image
-    auto load_in_row = [&](int x, int slot) {
-        constexpr int G = TCI / 8;
-        for (int q = tid; q < (ZS + 2) * G; q += NT) {
-            const int zz = q / G, g = q % G;
-            const int z = zs + zz - 1, c = ci0 + g * 8;
-            u32x4 v = {0u, 0u, 0u, 0u};
-            if (x >= 0 && x < a.h && z >= 0 && z < a.w)
-                v = c < a.c1 ? ld16(a.x1 + (((long long)n * a.h + x) * a.w + z) * a.c1 + c)
-                             : ld16(a.x2 + (((long long)n * h2 + (x >> a.up)) * w2 + (z >> a.up)) * a.c2 + (c - a.c1));
+    // input row x: item q = (zz, g) holds in[n][x][zs + zz - 1][ci0 + 8g .. + 8], zero outside the image
+    u32x4 ri[NI], ro[NO];
+    auto fetch_in = [&](int x) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                Bs[slot][g * 8 + 2 * k][zz] = (bf16_t)(v[k] & 0xFFFFu);
-                Bs[slot][g * 8 + 2 * k + 1][zz] = (bf16_t)(v[k] >> 16);
-            }
+        for (int k = 0; k < NI; ++k) {
+            const int q = tid + k * NT, zz = q / GI, g = q % GI;
+            const int z = zs + zz - 1, c = ci0 + g * 8;
+            ri[k] = u32x4{0u, 0u, 0u, 0u};
+            if (q < QI && x >= 0 && x < a.h && z >= 0 && z < a.w)
+                ri[k] = c < a.c1 ? ld16(a.x1 + (((long long)n * a.h + x) * a.w + z) * a.c1 + c)
+                                 : ld16(a.x2 + (((long long)n * h2 + (x >> a.up)) * w2 + (z >> a.up)) * a.c2 +
+                                        (c - a.c1));
         }
     };
-    // As[co][zz] = dy[n][x][zs + zz][co0 + co], zero beyond the image
-    auto load_dy_row = [&](int x) {
-        constexpr int G = TCO / 8;
-        for (int q = tid; q < ZS * G; q += NT) {
-            const int zz = q / G, g = q % G;
-            const int z = zs + zz;
-            u32x4 v = {0u, 0u, 0u, 0u};
-            if (z < a.w) v = ld16(a.dy + (((long long)n * a.h + x) * a.w + z) * a.cout + co0 + g * 8);
+    auto store_in = [&](int slot) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                As[g * 8 + 2 * k][zz] = (bf16_t)(v[k] & 0xFFFFu);
-                As[g * 8 + 2 * k + 1][zz] = (bf16_t)(v[k] >> 16);
-            }
+        for (int k = 0; k < NI; ++k) {
+            const int q = tid + k * NT, zz = q / GI, g = q % GI;
+            if (q < QI)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    Bs[slot][g * 8 + 2 * e][zz] = (bf16_t)(ri[k][e] & 0xFFFFu);
+                    Bs[slot][g * 8 + 2 * e + 1][zz] = (bf16_t)(ri[k][e] >> 16);
+                }
+        }
+    };
+    // dy row x: item q = (zz, g) holds dy[n][x][zs + zz][co0 + 8g .. + 8], zero beyond the image
+    auto fetch_dy = [&](int x) {
+#pragma unroll
+        for (int k = 0; k < NO; ++k) {
+            const int q = tid + k * NT, zz = q / GO, g = q % GO;
+            ro[k] = u32x4{0u, 0u, 0u, 0u};
+            if (q < QO && zs + zz < a.w) ro[k] = ld16(a.dy + (((long long)n * a.h + x) * a.w + zs + zz) * a.cout + co0 + g * 8);
+        }
+    };
+    auto store_dy = [&](int buf) {
+#pragma unroll
+        for (int k = 0; k < NO; ++k) {
+            const int q = tid + k * NT, zz = q / GO, g = q % GO;
+            if (q < QO)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    As[buf][g * 8 + 2 * e][zz] = (bf16_t)(ro[k][e] & 0xFFFFu);
+                    As[buf][g * 8 + 2 * e + 1][zz] = (bf16_t)(ro[k][e] >> 16);
+                }
         }
     };
 
-    f32x16 acc[9];
+    f32x16 acc[3 * ND];
 #pragma unroll
-    for (int t = 0; t < 9; ++t)
+    for (int t = 0; t < 3 * ND; ++t)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
 
-    // ring slot of input row x (x >= -1): (x + 3) % 3
-    load_in_row(xa - 1, (xa + 2) % 3);
-    load_in_row(xa, xa % 3);
+    for (int r = xa - 1; r <= xa + 1; ++r) {
+        fetch_in(r);
+        store_in((r + 4) & 3);
+    }
+    fetch_dy(xa);
+    store_dy(xa & 1);
+    __syncthreads();
     for (int x = xa; x < xe; ++x) {
-        load_in_row(x + 1, (x + 1) % 3);
-        load_dy_row(x);
-        __syncthreads();
+        const bool more = x + 1 < xe;
+        if (more) {
+            fetch_in(x + 2);
+            fetch_dy(x + 1);
+        }
         for (int s = 0; s < nks; ++s) {
             const int base = 16 * s + 8 * kh;
-            const u32x4 A = *reinterpret_cast<const u32x4*>(&As[cot * 32 + li][base]);
+            const u32x4 A = *reinterpret_cast<const u32x4*>(&As[x & 1][cot * 32 + li][base]);
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {  // dx = d - 1
-                const bf16_t* row = &Bs[(x + d + 2) % 3][cit * 32 + li][base];
+            for (int j = 0; j < ND; ++j) {
+                const int d = dsel + j * DS;  // dx = d - 1: input row x + d - 1
+                const bf16_t* row = &Bs[(x + d + 3) & 3][cit * 32 + li][base];
                 const u32x4 q = *reinterpret_cast<const u32x4*>(row);
                 const unsigned e = *reinterpret_cast<const unsigned*>(row + 8);
                 const u32x4 bm = q;                                                   // dz = -1
                 const u32x4 b0 = {__builtin_amdgcn_alignbyte(q[1], q[0], 2), __builtin_amdgcn_alignbyte(q[2], q[1], 2),
                                   __builtin_amdgcn_alignbyte(q[3], q[2], 2), __builtin_amdgcn_alignbyte(e, q[3], 2)};
                 const u32x4 bp = {q[1], q[2], q[3], e};                               // dz = +1
-                acc[d * 3 + 0] = mfma(A, bm, acc[d * 3 + 0]);
-                acc[d * 3 + 1] = mfma(A, b0, acc[d * 3 + 1]);
-                acc[d * 3 + 2] = mfma(A, bp, acc[d * 3 + 2]);
+                acc[j * 3 + 0] = mfma(A, bm, acc[j * 3 + 0]);
+                acc[j * 3 + 1] = mfma(A, b0, acc[j * 3 + 1]);
+                acc[j * 3 + 2] = mfma(A, bp, acc[j * 3 + 2]);
             }
+        }
+        if (more) {
+            store_in((x + 2) & 3);
+            store_dy((x + 1) & 1);
         }
         __syncthreads();
     }
     float* out = a.part + (long long)chunk * 9 * a.cout * cin;
 #pragma unroll
-    for (int t = 0; t < 9; ++t)
+    for (int j = 0; j < ND; ++j)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int i = (r & 3) + 4 * kh + 8 * (r >> 2);
-            const int co = co0 + cot * 32 + i, ci = ci0 + cit * 32 + li;
-            out[((long long)t * a.cout + co) * cin + ci] = acc[t][r];
-        }
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int t = (dsel + j * DS) * 3 + dz;
+                const int i = (r & 3) + 4 * kh + 8 * (r >> 2);
+                const int co = co0 + cot * 32 + i, ci = ci0 + cit * 32 + li;
+                out[((long long)t * a.cout + co) * cin + ci] = acc[j * 3 + dz][r];
+            }
 }
 
 void wgrad_plan(int n, int h, int w, int cout, int cin, long long max_part, int* rows, int* nchunk) {
@@ -374,13 +415,13 @@ void launch_wgrad(hipStream_t st, const WgradArgs& a) {
     const int tco = a.cout % 64 == 0 ? 64 : 32, tci = cin % 64 == 0 ? 64 : 32;
     dim3 grid((unsigned)(a.n * a.nseg * a.nxblk), (a.cout / tco) * (cin / tci));
     if (tco == 64 && tci == 64)
-        hipLaunchKernelGGL((k_wgrad<64, 64>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_wgrad<64, 64, 3>), grid, dim3(768), 0, st, a);
     else if (tco == 64)
-        hipLaunchKernelGGL((k_wgrad<64, 32>), grid, dim3(128), 0, st, a);
+        hipLaunchKernelGGL((k_wgrad<64, 32, 3>), grid, dim3(384), 0, st, a);
     else if (tci == 64)
-        hipLaunchKernelGGL((k_wgrad<32, 64>), grid, dim3(128), 0, st, a);
+        hipLaunchKernelGGL((k_wgrad<32, 64, 3>), grid, dim3(384), 0, st, a);
     else
-        hipLaunchKernelGGL((k_wgrad<32, 32>), grid, dim3(64), 0, st, a);
+        hipLaunchKernelGGL((k_wgrad<32, 32, 3>), grid, dim3(192), 0, st, a);
 }
 
 __global__ void k_reduce(const float* part, int nchunk, long long len, float* out, int accumulate) {

@@ -82,7 +82,8 @@ def test_conv_data_gradient_matches_oracle(cin, cout):
 
 
 @pytest.mark.parametrize("c1,c2,up,cout,h,w", [(32, 0, 0, 32, 12, 20), (64, 0, 0, 64, 10, 70), (32, 64, 1, 32, 8, 16),
-                                               (64, 64, 0, 128, 6, 8), (128, 256, 1, 128, 8, 8)])
+                                               (64, 64, 0, 128, 6, 8), (128, 256, 1, 128, 8, 8),
+                                               (32, 0, 0, 64, 5, 33), (64, 0, 0, 32, 7, 130), (32, 0, 0, 32, 1, 150)])
 def test_conv_weight_gradient_matches_oracle(c1, c2, up, cout, h, w):
     from ds2d_amd.lib import conv3x3_bf16
     n = 2
