@@ -87,6 +87,14 @@ void launch_combine_up(hipStream_t st, const float* dup, const bf16_t* act, bf16
 void launch_prep_weights(hipStream_t st, const float* w, bf16_t* wf, bf16_t* wb, int cout, int cin);
 void launch_to_bf16(hipStream_t st, const float* x, bf16_t* y, long long n);
 void launch_from_bf16(hipStream_t st, const bf16_t* x, float* y, long long n);
+// Training batch assembly: x[b] = src_x[idx.j[b]], t[b] = src_t[idx.j[b]] for
+// b < batch (images of img floats), in one launch.
+constexpr int GATHER_MAX = 64;
+struct GatherIdx {
+    int j[GATHER_MAX];
+};
+void launch_gather_images(hipStream_t st, const float* src_x, const float* src_t, float* x, float* t,
+                          const GatherIdx& idx, int batch, long long img);
 void launch_adam(hipStream_t st, float* theta, const float* g, float* m, float* v, long long n, float lr, float b1,
                  float b2, float eps, float bc1, float bc2);
 // out[0] = scale * sum_i part[i] (one block, fixed order)

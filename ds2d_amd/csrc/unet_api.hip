@@ -538,6 +538,7 @@ int ds2d_unet_train(ds2d_unet_t* u, int nsteps, int batch, float lr, unsigned lo
     int pos = u->ds_n;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     HIP_TRY(hipEventRecord(u->ev[0], u->st));
+    GatherIdx gi{};
     for (int k = 0; k < nsteps; ++k) {
         for (int b = 0; b < batch; ++b) {
             if (pos >= u->ds_n) {  // new epoch: Fisher-Yates shuffle
@@ -545,9 +546,14 @@ int ds2d_unet_train(ds2d_unet_t* u, int nsteps, int batch, float lr, unsigned lo
                 pos = 0;
             }
             const int j = perm[pos++];
-            HIP_TRY(hipMemcpyAsync(u->x + b * img, u->dset_x + j * img, img * 4, hipMemcpyDeviceToDevice, u->st));
-            HIP_TRY(hipMemcpyAsync(u->t + b * img, u->dset_t + j * img, img * 4, hipMemcpyDeviceToDevice, u->st));
+            if (batch <= GATHER_MAX) {
+                gi.j[b] = j;
+            } else {
+                HIP_TRY(hipMemcpyAsync(u->x + b * img, u->dset_x + j * img, img * 4, hipMemcpyDeviceToDevice, u->st));
+                HIP_TRY(hipMemcpyAsync(u->t + b * img, u->dset_t + j * img, img * 4, hipMemcpyDeviceToDevice, u->st));
+            }
         }
+        if (batch <= GATHER_MAX) launch_gather_images(u->st, u->dset_x, u->dset_t, u->x, u->t, gi, batch, (long long)img);
         forward(u, batch, h, w, u->st);
         backward(u, batch, h, w, u->d_loss + k, u->st);
         ++u->adam_t;

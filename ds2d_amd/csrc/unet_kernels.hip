@@ -133,14 +133,16 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
 // R = 128 * WM / w whole image rows; per K step of 16 channels the block loads
 // those rows plus a one-pixel frame ((R + 2) x (w + 2) pixels, 32 B each, zero
 // outside the image, skip concatenation and upsampling resolved in the
-// address) once, and all nine taps read their A operands from LDS.
+// address) and the step's weights of all nine taps for the block's output
+// channels once; all nine taps then read both MFMA operands from LDS.
 template <int WM, int WN>
-__global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
-    extern __shared__ u32x4 tile[];  // [(R + 2) * (w + 2)][2]: 16 channels of one pixel
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_conv_lds(ConvArgs a) {
+    extern __shared__ u32x4 tile[];  // [(R + 2) * (w + 2)][2]: 16 channels of one pixel; then the weights
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 31, kh = lane >> 5;
-    constexpr int P = 4 * WM * 32;
+    constexpr int P = 4 * WM * 32, NC = WN * 32;
     const int w = a.w, h = a.h, R = P / w, wp = w + 2;
+    u32x4* wl = tile + 2 * (R + 2) * wp;  // [9][NC][2]: 16 input channels of each tap and output channel
     const long long pb0 = (long long)blockIdx.x * P;
     const int nimg = (int)(pb0 / ((long long)h * w));
     const int x0 = (int)((pb0 - (long long)nimg * h * w) / w);
@@ -173,14 +175,18 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
                                     (c - a.c1) + 8 * half);
             tile[q] = v;
         }
+        // wl[(tap * NC + col) * 2 + half] = wt[tap][co0 + col][c + 8 * half .. + 8]
+        for (int q = tid; q < 9 * NC * 2; q += 256) {
+            const int half = q & 1, r = q >> 1, tap = r / NC, col = r % NC;
+            wl[q] = ld16(a.wt + ((long long)tap * a.cout + co0 + col) * ctot + c + 8 * half);
+        }
         __syncthreads();
 #pragma unroll 3
         for (int tap = 0; tap < 9; ++tap) {
             const int off = (tap / 3 - 1) * wp + (tap % 3 - 1);
-            const bf16_t* wt = a.wt + ((long long)tap * a.cout + co0 + li) * ctot + c + 8 * kh;
             u32x4 A[WM], B[WN];
 #pragma unroll
-            for (int j = 0; j < WN; ++j) B[j] = ld16(wt + (long long)j * 32 * ctot);
+            for (int j = 0; j < WN; ++j) B[j] = wl[2 * (tap * NC + j * 32 + li) + kh];
 #pragma unroll
             for (int m = 0; m < WM; ++m) A[m] = tile[2 * (lp[m] + off) + kh];
 #pragma unroll
@@ -229,12 +235,12 @@ bool conv_lds_ok(const ConvArgs& a, int P) {
 void launch_conv(hipStream_t st, const ConvArgs& a) {
     const long long npix = (long long)a.n * a.h * a.w;
     if (a.cout % 64 == 0 && conv_lds_ok(a, 256)) {
-        const size_t lds = (size_t)(256 / a.w + 2) * (a.w + 2) * 32;
+        const size_t lds = (size_t)(256 / a.w + 2) * (a.w + 2) * 32 + 9 * 64 * 32;
         hipLaunchKernelGGL((k_conv_lds<2, 2>), dim3((unsigned)(npix / 256), a.cout / 64), dim3(256), lds, st, a);
         return;
     }
     if (a.cout % 64 != 0 && conv_lds_ok(a, 512)) {
-        const size_t lds = (size_t)(512 / a.w + 2) * (a.w + 2) * 32;
+        const size_t lds = (size_t)(512 / a.w + 2) * (a.w + 2) * 32 + 9 * 32 * 32;
         hipLaunchKernelGGL((k_conv_lds<4, 1>), dim3((unsigned)(npix / 512), a.cout / 32), dim3(256), lds, st, a);
         return;
     }
@@ -483,37 +489,44 @@ __device__ __forceinline__ float tap_in(const float* x, int n, int h, int w, int
     return (xs >= 0 && xs < h && zs >= 0 && zs < w) ? x[((long long)n * h + xs) * w + zs] : 0.f;
 }
 
-__global__ void k_conv1(const float* x, const float* wt, const float* b, bf16_t* y, int n, int h, int w, int cout) {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// y[p][co] = relu(sum_t x[p + off(t)] wt[t][co] + b[co]); one thread per
+// (pixel, 8 output channels), so consecutive lanes store consecutive 16 bytes.
+__global__ __launch_bounds__(256) void k_conv1(const float* x, const float* wt, const float* b, bf16_t* y, int n,
+                                               int h, int w, int cout) {
+    __shared__ float ws[10][256];  // taps 0..8, then the bias
+    for (int i = threadIdx.x; i < 10 * cout; i += 256) ws[i / cout][i % cout] = i < 9 * cout ? wt[i] : b[i - 9 * cout];
+    __syncthreads();
+    const int G = cout / 8;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long npix = (long long)n * h * w;
-    if (p >= npix) return;
+    if (e >= npix * G) return;
+    const long long p = e / G;
+    const int c0 = 8 * (int)(e % G);
     const int iz = (int)(p % w), ix = (int)((p / w) % h), in = (int)(p / ((long long)w * h));
     float v[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) v[t] = tap_in(x, in, h, w, ix, iz, t);
-    for (int c0 = 0; c0 < cout; c0 += 8) {
-        u32x4 o;
+    u32x4 o;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            float s[2];
+    for (int k = 0; k < 4; ++k) {
+        float s2[2];
 #pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                const int co = c0 + 2 * k + e;
-                float acc = 0.f;
+        for (int e2 = 0; e2 < 2; ++e2) {
+            const int co = c0 + 2 * k + e2;
+            float acc = 0.f;
 #pragma unroll
-                for (int t = 0; t < 9; ++t) acc += v[t] * wt[t * cout + co];
-                s[e] = fmaxf(acc + b[co], 0.f);
-            }
-            o[k] = (unsigned)f2bf(s[0]) | ((unsigned)f2bf(s[1]) << 16);
+            for (int t = 0; t < 9; ++t) acc += v[t] * ws[t][co];
+            s2[e2] = fmaxf(acc + ws[9][co], 0.f);
         }
-        *reinterpret_cast<u32x4*>(y + p * cout + c0) = o;
+        o[k] = (unsigned)f2bf(s2[0]) | ((unsigned)f2bf(s2[1]) << 16);
     }
+    *reinterpret_cast<u32x4*>(y + p * cout + c0) = o;
 }
 
 void launch_conv1(hipStream_t st, const float* x, const float* w, const float* b, bf16_t* y, int n, int h, int wd,
                   int cout) {
-    const long long npix = (long long)n * h * wd;
-    hipLaunchKernelGGL(k_conv1, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, x, w, b, y, n, h, wd, cout);
+    const long long m = (long long)n * h * wd * (cout / 8);
+    hipLaunchKernelGGL(k_conv1, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, x, w, b, y, n, h, wd, cout);
 }
 
 static int reduce_blocks(long long npix) { return (int)std::min<long long>(1024, (npix + 255) / 256); }
@@ -521,37 +534,57 @@ int conv1_wgrad_blocks(long long npix) { return reduce_blocks(npix); }
 int bias_blocks(long long npix) { return reduce_blocks(npix); }
 int head_blocks(long long npix) { return reduce_blocks(npix); }
 
-// part[blk][t * cout + co] = sum_{p in blk} dy[p, co] * x[p + off(t)]; cout divides 256
+// part[blk][t * cout + co] = sum_{p in blk} dy[p, co] * x[p + off(t)]; cout a
+// multiple of 8 dividing 2048.  Each thread reads 8 channels of a pixel (one
+// 16-byte load) per pass; the per-thread sums are combined in a fixed order.
 __global__ __launch_bounds__(256) void k_conv1_wgrad(const float* x, const bf16_t* dy, float* part, int n, int h,
                                                      int w, int cout) {
-    __shared__ float red[9][256];
+    __shared__ float red[256 * 8];
     const long long npix = (long long)n * h * w;
     const long long per = (npix + gridDim.x - 1) / gridDim.x;
     const long long p0 = blockIdx.x * per, p1 = std::min(npix, p0 + per);
-    const int c = threadIdx.x % cout, sub = threadIdx.x / cout, nsub = 256 / cout;
-    float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int ng = cout / 8, g = threadIdx.x % ng, sub = threadIdx.x / ng, nsub = 256 / ng;
+    float acc[9][8];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[t][k] = 0.f;
     // pixel coordinates of p, advanced by nsub per pass without divisions
     const long long q0 = p0 + sub;
     int iz = (int)(q0 % w), ix = (int)((q0 / w) % h), in = (int)(q0 / ((long long)w * h));
     for (long long p = q0; p < p1; p += nsub) {
-        const float g = bf2f(dy[p * cout + c]);
+        const u32x4 v = ld16(dy + p * cout + 8 * g);
+        float gv[8];
 #pragma unroll
-        for (int t = 0; t < 9; ++t) acc[t] += g * tap_in(x, in, h, w, ix, iz, t);
+        for (int k = 0; k < 4; ++k) {
+            gv[2 * k] = __uint_as_float(v[k] << 16);
+            gv[2 * k + 1] = __uint_as_float(v[k] & 0xFFFF0000u);
+        }
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const float xt = tap_in(x, in, h, w, ix, iz, t);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc[t][k] += gv[k] * xt;
+        }
         iz += nsub;
         while (iz >= w) {
             iz -= w;
             if (++ix == h) ix = 0, ++in;
         }
     }
+    float* out = part + (long long)blockIdx.x * 9 * cout;
 #pragma unroll
-    for (int t = 0; t < 9; ++t) red[t][threadIdx.x] = acc[t];
-    __syncthreads();
-    if (threadIdx.x < cout)
-        for (int t = 0; t < 9; ++t) {
+    for (int t = 0; t < 9; ++t) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) red[threadIdx.x * 8 + k] = acc[t][k];
+        __syncthreads();
+        for (int c = threadIdx.x; c < cout; c += 256) {
             float s = 0.f;
-            for (int k = 0; k < nsub; ++k) s += red[t][k * cout + c];
-            part[(long long)blockIdx.x * 9 * cout + t * cout + c] = s;
+            for (int j = 0; j < nsub; ++j) s += red[(j * ng + c / 8) * 8 + c % 8];
+            out[t * cout + c] = s;
         }
+        __syncthreads();
+    }
 }
 
 void launch_conv1_wgrad(hipStream_t st, const float* x, const bf16_t* dy, float* part, int n, int h, int wd,
@@ -780,6 +813,21 @@ void launch_to_bf16(hipStream_t st, const float* x, bf16_t* y, long long n) {
 }
 void launch_from_bf16(hipStream_t st, const bf16_t* x, float* y, long long n) {
     hipLaunchKernelGGL(k_from_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, y, n);
+}
+
+__global__ void k_gather_images(const float* sx, const float* stg, float* x, float* t, GatherIdx idx, long long img) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (e >= img) return;
+    const long long src = (long long)idx.j[b] * img + e, dst = (long long)b * img + e;
+    x[dst] = sx[src];
+    t[dst] = stg[src];
+}
+
+void launch_gather_images(hipStream_t st, const float* src_x, const float* src_t, float* x, float* t,
+                          const GatherIdx& idx, int batch, long long img) {
+    hipLaunchKernelGGL(k_gather_images, dim3((unsigned)((img + 255) / 256), batch), dim3(256), 0, st, src_x, src_t, x,
+                       t, idx, img);
 }
 
 __global__ void k_adam(float* th, const float* g, float* m, float* v, long long n, float lr, float b1, float b2,
