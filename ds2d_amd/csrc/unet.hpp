@@ -37,15 +37,19 @@ struct ConvArgs {
 
 // Weight gradient: part[chunk][tap][co][ci] = sum over the chunk's pixels of
 // dy[p, co] * in[p + off(tap), ci]  (in as in ConvArgs).  Chunks are
-// (image, 64-wide z segment, block of `rows` x rows).
+// (image, 64-wide z segment, block of `rows` x rows).  With bias set, each
+// chunk's record is followed by the bias partials part[chunk][9 cout cin + co]
+// = sum over the chunk's pixels of dy[p, co] (the layout of a layer's weights
+// followed by its biases, so one reduction yields both gradients).
 struct WgradArgs {
     const bf16_t* x1;
     const bf16_t* x2;
     int c1, c2, up;
     const bf16_t* dy;   // [npix][cout]
-    float* part;        // [nchunk][9][cout][c1 + c2]
+    float* part;        // [nchunk][9 cout (c1 + c2) (+ cout with bias)]
     int n, h, w, cout;
     int rows, nseg, nxblk;
+    int bias;
 };
 
 void launch_conv(hipStream_t st, const ConvArgs& a);
@@ -61,13 +65,10 @@ void launch_reduce(hipStream_t st, float* part, int nchunk, long long len, float
 // First layer (one input channel, fp32 image): y = relu(conv(x) + b) -> bf16 [npix][cout]
 void launch_conv1(hipStream_t st, const float* x, const float* w, const float* b, bf16_t* y, int n, int h, int wd,
                   int cout);
-// First-layer weight gradient partials: part[blk][9 * cout]
+// First-layer weight and bias gradient partials: part[blk][9 * cout + cout]
 int conv1_wgrad_blocks(long long npix);
 void launch_conv1_wgrad(hipStream_t st, const float* x, const bf16_t* dy, float* part, int n, int h, int wd,
                         int cout);
-// Bias gradient partials: part[blk][cout] = sum over the block's pixels of dy
-int bias_blocks(long long npix);
-void launch_bias_grad(hipStream_t st, const bf16_t* dy, long long npix, int cout, float* part);
 
 // Head: y[p] = sum_c a[p, c] * wh[c] + bh.  With a target t: per-block partials
 // of the loss sum (double, part_loss[blk]) and of the head gradients
@@ -85,6 +86,18 @@ void launch_combine_pool(hipStream_t st, const float* dp, const float* dskip, co
 void launch_combine_up(hipStream_t st, const float* dup, const bf16_t* act, bf16_t* dz, int n, int h, int w, int c);
 // bf16 forward weights [9][co][ci] and flipped-transposed data-gradient weights [9][ci][co]
 void launch_prep_weights(hipStream_t st, const float* w, bf16_t* wf, bf16_t* wb, int cout, int cin);
+// The same for up to PREP_MAX layers in one launch (grid row = layer).
+constexpr int PREP_MAX = 24;
+struct PrepList {
+    struct Job {
+        const float* w;
+        bf16_t* wf;
+        bf16_t* wb;
+        int cout, cin;
+    } job[PREP_MAX];
+    int n;
+};
+void launch_prep_weights_list(hipStream_t st, const PrepList& pl);
 void launch_to_bf16(hipStream_t st, const float* x, bf16_t* y, long long n);
 void launch_from_bf16(hipStream_t st, const bf16_t* x, float* y, long long n);
 // Training batch assembly: x[b] = src_x[idx.j[b]], t[b] = src_t[idx.j[b]] for

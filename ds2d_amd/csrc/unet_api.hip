@@ -130,9 +130,16 @@ int check_dims(const ds2d_unet* u, int n, int h, int w) {
 
 void prep_weights(ds2d_unet* u, hipStream_t st) {
     if (!u->wdirty) return;
-    for (const Layer& q : u->layers)
-        if (q.k == 3 && q.cin > 1) launch_prep_weights(st, u->theta + q.w_off, u->wbf + q.wf_off, u->wbf + q.wb_off,
-                                                       q.cout, q.cin);
+    PrepList pl{};
+    for (const Layer& q : u->layers) {
+        if (q.k != 3 || q.cin <= 1) continue;
+        pl.job[pl.n++] = {u->theta + q.w_off, u->wbf + q.wf_off, u->wbf + q.wb_off, q.cout, q.cin};
+        if (pl.n == PREP_MAX) {
+            launch_prep_weights_list(st, pl);
+            pl.n = 0;
+        }
+    }
+    launch_prep_weights_list(st, pl);
     u->wdirty = false;
 }
 
@@ -219,11 +226,10 @@ void layer_grads(ds2d_unet* u, const Layer& q, const bf16_t* x1, int c1, const b
     a.rows = rows;
     a.nseg = (w + 63) / 64;
     a.nxblk = (h + rows - 1) / rows;
+    a.bias = 1;
     launch_wgrad(st, a);
-    launch_reduce(st, u->part, nchunk, 9LL * q.cout * q.cin, u->grad + q.w_off, 0);
-    const long long npix = (long long)n * h * w;
-    launch_bias_grad(st, dz, npix, q.cout, u->part);
-    launch_reduce(st, u->part, bias_blocks(npix), q.cout, u->grad + q.b_off, 0);
+    // weights then biases: b_off follows w_off in the parameter vector
+    launch_reduce(st, u->part, nchunk, 9LL * q.cout * q.cin + q.cout, u->grad + q.w_off, 0);
 }
 
 // Data gradient of one 3x3 layer: conv of dz with the flipped weights.
@@ -287,9 +293,7 @@ void backward(ds2d_unet* u, int n, int h, int w, double* loss_dev, hipStream_t s
         const Layer& qa = u->layers[u->enc(l, 0)];
         if (l == 0) {
             launch_conv1_wgrad(st, u->x, cur, u->part, n, hl, wl, qa.cout);
-            launch_reduce(st, u->part, conv1_wgrad_blocks(npix), 9LL * qa.cout, u->grad + qa.w_off, 0);
-            launch_bias_grad(st, cur, npix, qa.cout, u->part);
-            launch_reduce(st, u->part, bias_blocks(npix), qa.cout, u->grad + qa.b_off, 0);
+            launch_reduce(st, u->part, conv1_wgrad_blocks(npix), 10LL * qa.cout, u->grad + qa.w_off, 0);
             break;
         }
         layer_grads(u, qa, u->pool[l - 1], qa.cin, nullptr, 0, cur, n, hl, wl, st);
@@ -409,15 +413,14 @@ int ds2d_unet_create(const ds2d_unet_cfg* cfg, ds2d_unet_t** out) {
         return bail(rc);
     // split-K partial buffer: the largest of any weight-gradient plan and the small reductions
     long long part = (long long)head_blocks(npix(0)) * (C0 + 1);
-    part = std::max(part, (long long)conv1_wgrad_blocks(npix(0)) * 9 * C0);
+    part = std::max(part, (long long)conv1_wgrad_blocks(npix(0)) * 10 * C0);
     // a weight-gradient plan stays within MAX_PART unless it is down to one
     // chunk per (image, z segment); bound that case over every layer and level
     part = std::max(part, MAX_PART);
     for (int l = 0; l < L; ++l) {
-        part = std::max(part, (long long)bias_blocks(npix(l)) * u->ch[l]);
         const long long nseg = ((cfg->max_w >> l) + 63) / 64;
         for (const Layer& q : u->layers)
-            if (q.k == 3 && q.cin > 1) part = std::max(part, N * nseg * 9 * q.cout * q.cin);
+            if (q.k == 3 && q.cin > 1) part = std::max(part, N * nseg * (9 * q.cout * q.cin + q.cout));
     }
     if ((rc = u->alloc(&u->part, part)) || (rc = u->alloc(&u->part_loss, head_blocks(npix(0)))) ||
         (rc = u->alloc(&u->d_loss, 1)))

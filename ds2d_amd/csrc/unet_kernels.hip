@@ -344,6 +344,12 @@ __global__ __launch_bounds__(64 * (TCO / 32) * (TCI / 32) * DS) void k_wgrad(Wgr
     for (int t = 0; t < 3 * ND; ++t)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
+    // bias partials (blocks of the first cin tile): NB threads per output
+    // channel, each summing ZS / NB pixels of every dy row
+    constexpr int NB = NT / TCO >= 8 ? 8 : NT / TCO >= 4 ? 4 : NT / TCO >= 2 ? 2 : 1;
+    const bool do_bias = a.bias && blockIdx.y % ntci == 0;
+    const int bco = tid % TCO, bpart = tid / TCO;
+    float bacc = 0.f;
 
     for (int r = xa - 1; r <= xa + 1; ++r) {
         fetch_in(r);
@@ -376,13 +382,36 @@ __global__ __launch_bounds__(64 * (TCO / 32) * (TCI / 32) * DS) void k_wgrad(Wgr
                 acc[j * 3 + 2] = mfma(A, bp, acc[j * 3 + 2]);
             }
         }
+        if (do_bias && bpart < NB) {
+            const u32x4* rp = reinterpret_cast<const u32x4*>(&As[x & 1][bco][bpart * (ZS / NB)]);
+#pragma unroll
+            for (int k = 0; k < ZS / NB / 8; ++k) {
+                const u32x4 v = rp[k];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    bacc += __uint_as_float(v[e] << 16);
+                    bacc += __uint_as_float(v[e] & 0xFFFF0000u);
+                }
+            }
+        }
         if (more) {
             store_in((x + 2) & 3);
             store_dy((x + 1) & 1);
         }
         __syncthreads();
     }
-    float* out = a.part + (long long)chunk * 9 * a.cout * cin;
+    const long long rec = 9LL * a.cout * cin;
+    float* out = a.part + (long long)chunk * (rec + (a.bias ? a.cout : 0));
+    if (do_bias) {  // the ring is free after the last barrier
+        float* red = reinterpret_cast<float*>(&Bs[0][0][0]);
+        if (bpart < NB) red[bpart * TCO + bco] = bacc;
+        __syncthreads();
+        if (tid < TCO) {
+            float sb = 0.f;
+            for (int k = 0; k < NB; ++k) sb += red[k * TCO + tid];
+            out[rec + co0 + tid] = sb;
+        }
+    }
 #pragma unroll
     for (int j = 0; j < ND; ++j)
 #pragma unroll
@@ -404,7 +433,7 @@ void wgrad_plan(int n, int h, int w, int cout, int cin, long long max_part, int*
     long long per = std::max<long long>(1, (want + (long long)n * nseg - 1) / ((long long)n * nseg));
     int r = (int)std::max<long long>(1, (h + per - 1) / per);
     r = std::max(r, std::min(4, h));
-    const long long per_chunk = 9LL * cout * cin;
+    const long long per_chunk = 9LL * cout * cin + cout;
     for (;;) {
         const long long nc = (long long)n * nseg * ((h + r - 1) / r);
         if (nc * per_chunk <= max_part || r >= h) {
@@ -531,11 +560,11 @@ void launch_conv1(hipStream_t st, const float* x, const float* w, const float* b
 
 static int reduce_blocks(long long npix) { return (int)std::min<long long>(1024, (npix + 255) / 256); }
 int conv1_wgrad_blocks(long long npix) { return reduce_blocks(npix); }
-int bias_blocks(long long npix) { return reduce_blocks(npix); }
 int head_blocks(long long npix) { return reduce_blocks(npix); }
 
-// part[blk][t * cout + co] = sum_{p in blk} dy[p, co] * x[p + off(t)]; cout a
-// multiple of 8 dividing 2048.  Each thread reads 8 channels of a pixel (one
+// part[blk][t * cout + co] = sum_{p in blk} dy[p, co] * x[p + off(t)] for the
+// taps t < 9 and sum_{p in blk} dy[p, co] at t = 9; cout a multiple of 8
+// dividing 2048.  Each thread reads 8 channels of a pixel (one
 // 16-byte load) per pass; the per-thread sums are combined in a fixed order.
 __global__ __launch_bounds__(256) void k_conv1_wgrad(const float* x, const bf16_t* dy, float* part, int n, int h,
                                                      int w, int cout) {
@@ -544,9 +573,9 @@ __global__ __launch_bounds__(256) void k_conv1_wgrad(const float* x, const bf16_
     const long long per = (npix + gridDim.x - 1) / gridDim.x;
     const long long p0 = blockIdx.x * per, p1 = std::min(npix, p0 + per);
     const int ng = cout / 8, g = threadIdx.x % ng, sub = threadIdx.x / ng, nsub = 256 / ng;
-    float acc[9][8];
+    float acc[10][8];  // taps 0..8, then the bias
 #pragma unroll
-    for (int t = 0; t < 9; ++t)
+    for (int t = 0; t < 10; ++t)
 #pragma unroll
         for (int k = 0; k < 8; ++k) acc[t][k] = 0.f;
     // pixel coordinates of p, advanced by nsub per pass without divisions
@@ -566,15 +595,17 @@ __global__ __launch_bounds__(256) void k_conv1_wgrad(const float* x, const bf16_
 #pragma unroll
             for (int k = 0; k < 8; ++k) acc[t][k] += gv[k] * xt;
         }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[9][k] += gv[k];
         iz += nsub;
         while (iz >= w) {
             iz -= w;
             if (++ix == h) ix = 0, ++in;
         }
     }
-    float* out = part + (long long)blockIdx.x * 9 * cout;
+    float* out = part + (long long)blockIdx.x * 10 * cout;
 #pragma unroll
-    for (int t = 0; t < 9; ++t) {
+    for (int t = 0; t < 10; ++t) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) red[threadIdx.x * 8 + k] = acc[t][k];
         __syncthreads();
@@ -591,40 +622,6 @@ void launch_conv1_wgrad(hipStream_t st, const float* x, const bf16_t* dy, float*
                         int cout) {
     const long long npix = (long long)n * h * wd;
     hipLaunchKernelGGL(k_conv1_wgrad, dim3(conv1_wgrad_blocks(npix)), dim3(256), 0, st, x, dy, part, n, h, wd, cout);
-}
-
-// part[blk][co] = sum_{p in blk} dy[p, co]; cout a multiple of 8 dividing 2048.
-// Each thread reads 8 channels of a pixel (one 16-byte load) per pass; the
-// per-thread sums are combined in a fixed order.
-__global__ __launch_bounds__(256) void k_bias_grad(const bf16_t* dy, long long npix, int cout, float* part) {
-    __shared__ float red[256 * 8];
-    const long long per = (npix + gridDim.x - 1) / gridDim.x;
-    const long long p0 = blockIdx.x * per, p1 = std::min(npix, p0 + per);
-    const int ng = cout / 8, g = threadIdx.x % ng, sub = threadIdx.x / ng, nsub = 256 / ng;
-    float acc[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) acc[k] = 0.f;
-    for (long long p = p0 + sub; p < p1; p += nsub) {
-        const u32x4 v = ld16(dy + p * cout + 8 * g);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            acc[2 * k] += __uint_as_float(v[k] << 16);
-            acc[2 * k + 1] += __uint_as_float(v[k] & 0xFFFF0000u);
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) red[threadIdx.x * 8 + k] = acc[k];
-    __syncthreads();
-    for (int c = threadIdx.x; c < cout; c += 256) {
-        const int gc = c / 8, e = c % 8;
-        float sum = 0.f;
-        for (int k = 0; k < nsub; ++k) sum += red[(k * ng + gc) * 8 + e];
-        part[(long long)blockIdx.x * cout + c] = sum;
-    }
-}
-
-void launch_bias_grad(hipStream_t st, const bf16_t* dy, long long npix, int cout, float* part) {
-    hipLaunchKernelGGL(k_bias_grad, dim3(bias_blocks(npix)), dim3(256), 0, st, dy, npix, cout, part);
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -798,6 +795,25 @@ __global__ void k_prep_weights(const float* w, bf16_t* wf, bf16_t* wb, int cout,
 void launch_prep_weights(hipStream_t st, const float* w, bf16_t* wf, bf16_t* wb, int cout, int cin) {
     const long long m = 9LL * cout * cin;
     hipLaunchKernelGGL(k_prep_weights, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, w, wf, wb, cout, cin);
+}
+
+__global__ void k_prep_weights_list(PrepList pl) {
+    const PrepList::Job& J = pl.job[blockIdx.y];
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per = (long long)J.cout * J.cin;
+    if (e >= 9 * per) return;
+    const int t = (int)(e / per);
+    const int co = (int)((e % per) / J.cin), ci = (int)(e % J.cin);
+    const bf16_t v = f2bf(J.w[e]);
+    J.wf[e] = v;
+    J.wb[((long long)(8 - t) * J.cin + ci) * J.cout + co] = v;
+}
+
+void launch_prep_weights_list(hipStream_t st, const PrepList& pl) {
+    long long m = 0;
+    for (int i = 0; i < pl.n; ++i) m = std::max(m, 9LL * pl.job[i].cout * pl.job[i].cin);
+    if (pl.n > 0)
+        hipLaunchKernelGGL(k_prep_weights_list, dim3((unsigned)((m + 255) / 256), pl.n), dim3(256), 0, st, pl);
 }
 
 __global__ void k_to_bf16(const float* x, bf16_t* y, long long n) {
