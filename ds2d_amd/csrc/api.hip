@@ -410,10 +410,12 @@ int ds2d_create(const ds2d_geom* g, const float* wavelet, ds2d_t** out) {
         delete h;
         return fail(DS2D_ENOMEM, "model allocation failed");
     }
-    (void)hipMemcpy(h->d_wav, wavelet, g->nt * sizeof(float), hipMemcpyHostToDevice);
-    (void)hipMemset(h->d_grad7, 0, 7 * A * 4);
-    (void)hipMemset(h->d_ipp, 0, A * 4);
-    (void)hipMemset(h->d_ips, 0, A * 4);
+    if (hipMemcpy(h->d_wav, wavelet, g->nt * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(h->d_grad7, 0, 7 * A * 4) != hipSuccess || hipMemset(h->d_ipp, 0, A * 4) != hipSuccess ||
+        hipMemset(h->d_ips, 0, A * 4) != hipSuccess) {
+        delete h;
+        return fail(DS2D_EDEVICE, "initialising the model buffers failed");
+    }
     *out = h;
     return 0;
 }
