@@ -207,12 +207,14 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
             if (j == S.isx) cz += w * tap(S.isz - z);
             const float bx = o.bx;
             if (PML && damp_cell_f(S, j, z)) {
-                const float d1 = 1.f - dt * o.dxv / 2, d2 = 1.f + dt * o.dxv / 2;
-                const float d3 = 1.f - dt * o.dzv / 2, d4 = 1.f + dt * o.dzv / 2;
-                const float vxx = (d1 * o.vxx + bx * idx * ax) / d2;
-                const float vxz = (d3 * (o.vx - o.vxx) + bx * idz * bz) / d4;
-                const float vzx = (d1 * (o.vz - o.vzz) + bx * idx * ex) / d2;
-                const float vzz = (d3 * o.vzz + bx * idz * cz) / d4;
+                // one hardware reciprocal (1 ulp) per direction instead of four IEEE divisions:
+                // the damping waves sit on every column's critical path
+                const float d1 = 1.f - dt * o.dxv / 2, r2 = __builtin_amdgcn_rcpf(1.f + dt * o.dxv / 2);
+                const float d3 = 1.f - dt * o.dzv / 2, r4 = __builtin_amdgcn_rcpf(1.f + dt * o.dzv / 2);
+                const float vxx = (d1 * o.vxx + bx * idx * ax) * r2;
+                const float vxz = (d3 * (o.vx - o.vxx) + bx * idz * bz) * r4;
+                const float vzx = (d1 * (o.vz - o.vzz) + bx * idx * ex) * r2;
+                const float vzz = (d3 * o.vzz + bx * idz * cz) * r4;
                 if (zown && j >= x0 && j < x1) {
                     const uint32_t c = zo4 + (uint32_t)j * colb;
                     st(splitB(0), c, vxx);
@@ -258,14 +260,14 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
                                     c3 * (L2[L + 3] - L2[L - 2]) + c4 * (L2[L + 4] - L2[L - 3])) * idz;
                 const float l = o.lam, mu = o.mu, l2m = l + 2 * mu;
                 if (PML && damp_cell_f(S, cs, z)) {
-                    const float d1 = 1.f - dt * o.dxs / 2, d2 = 1.f + dt * o.dxs / 2;
-                    const float d3 = 1.f - dt * o.dzs / 2, d4 = 1.f + dt * o.dzs / 2;
-                    const float a1 = (d1 * o.sxx + l2m * dvx) / d2;
-                    const float a2 = (d3 * (txx - o.sxx) + l * dvz) / d4;
-                    const float b1 = (d1 * (tzz - o.szz) + l * dvx) / d2;
-                    const float b2 = (d3 * o.szz + l2m * dvz) / d4;
-                    const float e1 = (d1 * o.sxz + mu * dvzx) / d2;
-                    const float e2 = (d3 * (txz - o.sxz) + mu * dvxz) / d4;
+                    const float d1 = 1.f - dt * o.dxs / 2, r2 = __builtin_amdgcn_rcpf(1.f + dt * o.dxs / 2);
+                    const float d3 = 1.f - dt * o.dzs / 2, r4 = __builtin_amdgcn_rcpf(1.f + dt * o.dzs / 2);
+                    const float a1 = (d1 * o.sxx + l2m * dvx) * r2;
+                    const float a2 = (d3 * (txx - o.sxx) + l * dvz) * r4;
+                    const float b1 = (d1 * (tzz - o.szz) + l * dvx) * r2;
+                    const float b2 = (d3 * o.szz + l2m * dvz) * r4;
+                    const float e1 = (d1 * o.sxz + mu * dvzx) * r2;
+                    const float e2 = (d3 * (txz - o.sxz) + mu * dvxz) * r4;
                     st(splitB(4), c, a1);
                     st(splitB(7), c, b2);
                     st(splitB(8), c, e1);
