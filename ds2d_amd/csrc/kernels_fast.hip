@@ -59,12 +59,13 @@ __global__ __launch_bounds__(256) void k_fast_v(FastArgs a, const int* __restric
     float nvx, nvz;
     if (damp_cell(S, ix, iz)) {
         const float dpx_ = a.dampx[c], dpz_ = a.dampz[c];
-        const float d1 = 1.f - dt * dpx_ / 2, d2 = 1.f + dt * dpx_ / 2;
-        const float d3 = 1.f - dt * dpz_ / 2, d4 = 1.f + dt * dpz_ / 2;
-        const float vxx = (d1 * a.vx_x[c] + bx * idx * ax) / d2;
-        const float vxz = (d3 * a.vx_z[c] + bx * idz * bz) / d4;
-        const float vzx = (d1 * a.vz_x[c] + bx * idx * ex) / d2;
-        const float vzz = (d3 * a.vz_z[c] + bx * idz * cz) / d4;
+        // one hardware reciprocal (1 ulp) per direction instead of an IEEE division per component
+        const float d1 = 1.f - dt * dpx_ / 2, r2 = __builtin_amdgcn_rcpf(1.f + dt * dpx_ / 2);
+        const float d3 = 1.f - dt * dpz_ / 2, r4 = __builtin_amdgcn_rcpf(1.f + dt * dpz_ / 2);
+        const float vxx = (d1 * a.vx_x[c] + bx * idx * ax) * r2;
+        const float vxz = (d3 * a.vx_z[c] + bx * idz * bz) * r4;
+        const float vzx = (d1 * a.vz_x[c] + bx * idx * ex) * r2;
+        const float vzz = (d3 * a.vz_z[c] + bx * idz * cz) * r4;
         a.vx_x[c] = vxx; a.vx_z[c] = vxz; a.vz_x[c] = vzx; a.vz_z[c] = vzz;
         nvx = vxx + vxz;
         nvz = vzx + vzz;
@@ -99,14 +100,14 @@ __global__ __launch_bounds__(256) void k_fast_s(FastArgs a, const SlotDesc* __re
     const float l = a.lam[c], mu = a.mu[c], l2m = l + 2 * mu;
     if (damp_cell(S, ix, iz)) {
         const float dpx_ = a.dampx[c], dpz_ = a.dampz[c];
-        const float d1 = 1.f - dt * dpx_ / 2, d2 = 1.f + dt * dpx_ / 2;
-        const float d3 = 1.f - dt * dpz_ / 2, d4 = 1.f + dt * dpz_ / 2;
-        const float a1 = (d1 * a.txx_x[c] + l2m * dvx) / d2;
-        const float a2 = (d3 * a.txx_z[c] + l * dvz) / d4;
-        const float b1 = (d1 * a.tzz_x[c] + l * dvx) / d2;
-        const float b2 = (d3 * a.tzz_z[c] + l2m * dvz) / d4;
-        const float c1 = (d1 * a.txz_x[c] + mu * dvzx) / d2;
-        const float c2 = (d3 * a.txz_z[c] + mu * dvxz) / d4;
+        const float d1 = 1.f - dt * dpx_ / 2, r2 = __builtin_amdgcn_rcpf(1.f + dt * dpx_ / 2);
+        const float d3 = 1.f - dt * dpz_ / 2, r4 = __builtin_amdgcn_rcpf(1.f + dt * dpz_ / 2);
+        const float a1 = (d1 * a.txx_x[c] + l2m * dvx) * r2;
+        const float a2 = (d3 * a.txx_z[c] + l * dvz) * r4;
+        const float b1 = (d1 * a.tzz_x[c] + l * dvx) * r2;
+        const float b2 = (d3 * a.tzz_z[c] + l2m * dvz) * r4;
+        const float c1 = (d1 * a.txz_x[c] + mu * dvzx) * r2;
+        const float c2 = (d3 * a.txz_z[c] + mu * dvxz) * r4;
         a.txx_x[c] = a1; a.txx_z[c] = a2; a.tzz_x[c] = b1; a.tzz_z[c] = b2; a.txz_x[c] = c1; a.txz_z[c] = c2;
         a.txx[c] = a1 + a2;
         a.tzz[c] = b1 + b2;
