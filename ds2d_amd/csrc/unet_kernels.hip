@@ -129,23 +129,28 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
         }
 }
 
-// Same GEMM with the input staged through LDS: a block's 128 * WM pixels are
-// R = 128 * WM / w whole image rows; per K step of 16 channels the block loads
-// those rows plus a one-pixel frame ((R + 2) x (w + 2) pixels, 32 B each, zero
+// Columns of a block of the LDS kernel: 128 when that divides a wider image
+// (row blocks x 128-column blocks: a smaller frame per pixel), else whole rows.
+__host__ __device__ inline int conv_lds_cols(int w) { return w > 128 && w % 128 == 0 ? 128 : w; }
+
+// Same GEMM with the input staged through LDS: a block's P = 128 * WM pixels
+// are R = P / CB image rows of CB = conv_lds_cols(w) columns; per K step of 16
+// channels the block loads them plus a one-pixel frame ((R + 2) x (CB + 2)
+// pixels, 32 B each, zero
 // outside the image, skip concatenation and upsampling resolved in the
 // address) and the step's weights of all nine taps for the block's output
 // channels once; all nine taps then read both MFMA operands from LDS.
 template <int WM, int WN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_conv_lds(ConvArgs a) {
-    extern __shared__ u32x4 tile[];  // [(R + 2) * (w + 2)][2]: 16 channels of one pixel; then the weights
+    extern __shared__ u32x4 tile[];  // [(R + 2) * (CB + 2)][2]: 16 channels of one pixel; then the weights
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 31, kh = lane >> 5;
     constexpr int P = 4 * WM * 32, NC = WN * 32;
-    const int w = a.w, h = a.h, R = P / w, wp = w + 2;
+    const int w = a.w, h = a.h, CB = conv_lds_cols(w), R = P / CB, wp = CB + 2;
     u32x4* wl = tile + 2 * (R + 2) * wp;  // [9][NC][2]: 16 input channels of each tap and output channel
-    const long long pb0 = (long long)blockIdx.x * P;
-    const int nimg = (int)(pb0 / ((long long)h * w));
-    const int x0 = (int)((pb0 - (long long)nimg * h * w) / w);
+    const int ncb = w / CB, bpi = (h / R) * ncb;  // column blocks per row block, blocks per image
+    const int nimg = blockIdx.x / bpi, rem = blockIdx.x % bpi;
+    const int x0 = (rem / ncb) * R, zb0 = (rem % ncb) * CB;
     const int co0 = blockIdx.y * (WN * 32);
     const int ctot = a.c1 + a.c2;
     const int h2 = h >> a.up, w2 = w >> a.up;
@@ -154,7 +159,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
     for (int m = 0; m < WM; ++m) {
         const int i = wave * WM * 32 + m * 32 + li;
-        lp[m] = (i / w + 1) * wp + (i % w) + 1;
+        lp[m] = (i / CB + 1) * wp + (i % CB) + 1;
     }
     f32x16 acc[WM][WN];
 #pragma unroll
@@ -167,7 +172,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int c = 0; c < ctot; c += 16) {
         for (int q = tid; q < 2 * npx; q += 256) {
             const int px = q >> 1, half = q & 1;
-            const int x = x0 - 1 + px / wp, z = px % wp - 1;
+            const int x = x0 - 1 + px / wp, z = zb0 + px % wp - 1;
             u32x4 v = zero;
             if (x >= 0 && x < h && z >= 0 && z < w)
                 v = c < a.c1 ? ld16(a.x1 + (((long long)nimg * h + x) * w + z) * a.c1 + c + 8 * half)
@@ -196,7 +201,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
         __syncthreads();
     }
-    const long long pbase = pb0 + (long long)wave * WM * 32;
+    const long long img0 = (long long)nimg * h * w;
     const int c2o = a.cout - a.split;
 #pragma unroll
     for (int m = 0; m < WM; ++m)
@@ -206,8 +211,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             const float b = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int i = (r & 3) + 4 * kh + 8 * (r >> 2);
-                const long long p = pbase + m * 32 + i;
+                const int i = wave * WM * 32 + m * 32 + (r & 3) + 4 * kh + 8 * (r >> 2);  // pixel of the block
+                const long long p = img0 + (long long)(x0 + i / CB) * w + zb0 + i % CB;
                 float v = acc[m][j][r] + b;
                 if (a.relu) v = fmaxf(v, 0.f);
                 if (a.mask && !(bf2f(a.mask[p * a.cout + co]) > 0.f)) v = 0.f;
@@ -226,21 +231,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
 }
 
-// The LDS kernel takes blocks of whole image rows: w divides the block's
-// pixels, the block's pixels divide one image, channels come in 16s.
+// The LDS kernel takes blocks of R = P / CB rows of CB = conv_lds_cols(w)
+// columns: CB divides P, R divides h, channels come in 16s.
 bool conv_lds_ok(const ConvArgs& a, int P) {
-    return a.w > 0 && P % a.w == 0 && ((long long)a.h * a.w) % P == 0 && a.c1 % 16 == 0 && a.c2 % 16 == 0;
+    if (a.w <= 0 || a.h <= 0) return false;
+    const int cb = conv_lds_cols(a.w);
+    return P % cb == 0 && a.h % (P / cb) == 0 && a.c1 % 16 == 0 && a.c2 % 16 == 0;
+}
+
+// Dynamic LDS of the kernel: the framed pixel tile and the weights of nine taps.
+static size_t conv_lds_bytes(int w, int P, int nc) {
+    const int cb = conv_lds_cols(w);
+    return (size_t)(P / cb + 2) * (cb + 2) * 32 + 9 * (size_t)nc * 32;
 }
 
 void launch_conv(hipStream_t st, const ConvArgs& a) {
     const long long npix = (long long)a.n * a.h * a.w;
     if (a.cout % 64 == 0 && conv_lds_ok(a, 256)) {
-        const size_t lds = (size_t)(256 / a.w + 2) * (a.w + 2) * 32 + 9 * 64 * 32;
+        const size_t lds = conv_lds_bytes(a.w, 256, 64);
         hipLaunchKernelGGL((k_conv_lds<2, 2>), dim3((unsigned)(npix / 256), a.cout / 64), dim3(256), lds, st, a);
         return;
     }
     if (a.cout % 64 != 0 && conv_lds_ok(a, 512)) {
-        const size_t lds = (size_t)(512 / a.w + 2) * (a.w + 2) * 32 + 9 * 32 * 32;
+        const size_t lds = conv_lds_bytes(a.w, 512, 32);
         hipLaunchKernelGGL((k_conv_lds<4, 1>), dim3((unsigned)(npix / 512), a.cout / 32), dim3(256), lds, st, a);
         return;
     }
