@@ -757,40 +757,71 @@ void launch_pool(hipStream_t st, const bf16_t* x, bf16_t* y, int n, int h, int w
     hipLaunchKernelGGL(k_pool, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, x, y, n, h, w, c);
 }
 
+// 8 consecutive floats (two 16-byte accesses)
+__device__ __forceinline__ void ld8f(const float* p, float v[8]) {
+    const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
+    v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+}
+// dz = bf16(act > 0 ? v : 0) for 8 channels, one 16-byte load of act and one store of dz
+__device__ __forceinline__ void masked_store8(const bf16_t* act, bf16_t* dz, const float v[8]) {
+    const u32x4 m = ld16(act);
+    u32x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float za = __uint_as_float(m[k] << 16) > 0.f ? v[2 * k] : 0.f;
+        const float zb = __uint_as_float(m[k] & 0xFFFF0000u) > 0.f ? v[2 * k + 1] : 0.f;
+        o[k] = (unsigned)f2bf(za) | ((unsigned)f2bf(zb) << 16);
+    }
+    *reinterpret_cast<u32x4*>(dz) = o;
+}
+
+// dz[p][ch] = bf16(act > 0 ? dp[pool(p)][ch] / 4 + dskip[p][ch] : 0); 8 channels per thread
 __global__ void k_combine_pool(const float* dp, const float* dskip, const bf16_t* act, bf16_t* dz, int n, int h,
                                int w, int c) {
+    const int G = c / 8;
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long long)n * h * w * c) return;
-    const int ch = (int)(e % c);
-    const long long p = e / c;
+    if (e >= (long long)n * h * w * G) return;
+    const int ch = 8 * (int)(e % G);
+    const long long p = e / G;
     const int iz = (int)(p % w), ix = (int)((p / w) % h), in = (int)(p / ((long long)w * h));
     const long long q = ((long long)in * (h / 2) + ix / 2) * (w / 2) + iz / 2;
-    float v = dp[q * c + ch] * 0.25f + (dskip ? dskip[e] : 0.f);
-    dz[e] = f2bf(bf2f(act[e]) > 0.f ? v : 0.f);
+    float v[8], sk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    ld8f(dp + q * c + ch, v);
+    if (dskip) ld8f(dskip + p * c + ch, sk);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = v[k] * 0.25f + sk[k];
+    masked_store8(act + p * c + ch, dz + p * c + ch, v);
 }
 
 void launch_combine_pool(hipStream_t st, const float* dp, const float* dskip, const bf16_t* act, bf16_t* dz, int n,
                          int h, int w, int c) {
-    const long long m = (long long)n * h * w * c;
+    const long long m = (long long)n * h * w * (c / 8);
     hipLaunchKernelGGL(k_combine_pool, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, dp, dskip, act, dz, n, h,
                        w, c);
 }
 
+// dz[p][ch] = bf16(act > 0 ? sum of dup over the 2x2 block above p : 0); 8 channels per thread
 __global__ void k_combine_up(const float* dup, const bf16_t* act, bf16_t* dz, int n, int h, int w, int c) {
+    const int G = c / 8;
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long long)n * h * w * c) return;
-    const int ch = (int)(e % c);
-    const long long p = e / c;
+    if (e >= (long long)n * h * w * G) return;
+    const int ch = 8 * (int)(e % G);
+    const long long p = e / G;
     const int iz = (int)(p % w), ix = (int)((p / w) % h), in = (int)(p / ((long long)w * h));
     const int H = 2 * h, W = 2 * w;
-    float s = 0.f;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int da = 0; da < 2; ++da)
-        for (int db = 0; db < 2; ++db) s += dup[(((long long)in * H + 2 * ix + da) * W + 2 * iz + db) * c + ch];
-    dz[e] = f2bf(bf2f(act[e]) > 0.f ? s : 0.f);
+        for (int db = 0; db < 2; ++db) {
+            float u[8];
+            ld8f(dup + (((long long)in * H + 2 * ix + da) * W + 2 * iz + db) * c + ch, u);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s[k] += u[k];
+        }
+    masked_store8(act + p * c + ch, dz + p * c + ch, s);
 }
 
 void launch_combine_up(hipStream_t st, const float* dup, const bf16_t* act, bf16_t* dz, int n, int h, int w, int c) {
-    const long long m = (long long)n * h * w * c;
+    const long long m = (long long)n * h * w * (c / 8);
     hipLaunchKernelGGL(k_combine_up, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, dup, act, dz, n, h, w, c);
 }
 
