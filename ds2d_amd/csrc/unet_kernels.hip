@@ -442,7 +442,10 @@ void wgrad_plan(int n, int h, int w, int cout, int cin, long long max_part, int*
     const int tco = cout % 64 == 0 ? 64 : 32, tci = cin % 64 == 0 ? 64 : 32;
     const long long tiles = (long long)(cout / tco) * (cin / tci);
     const int nseg = (w + ZS - 1) / ZS;
-    const long long want = std::max<long long>(1, (512 + tiles - 1) / tiles);  // chunks for ~512 blocks
+    // workgroups for one full round of the GPU: 256 CUs x the workgroups of this tile shape that fit on a CU
+    // (64x64: 12 waves, one per CU; 64x32 and 32x64: 6 waves, two; 32x32: 3 waves, five)
+    const long long target = tco == 64 && tci == 64 ? 256 : tco == 32 && tci == 32 ? 1280 : 512;
+    const long long want = std::max<long long>(1, (target + tiles - 1) / tiles);
     long long per = std::max<long long>(1, (want + (long long)n * nseg - 1) / ((long long)n * nseg));
     int r = (int)std::max<long long>(1, (h + per - 1) / per);
     r = std::max(r, std::min(4, h));
