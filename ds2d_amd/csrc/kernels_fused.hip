@@ -206,16 +206,19 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
             if (z == S.isz) ax += w * tap(S.isx - j);
             if (j == S.isx) cz += w * tap(S.isz - z);
             const float bx = o.bx;
-            if (PML && damp_cell_f(S, j, z)) {
-                // one hardware reciprocal (1 ulp) per direction instead of four IEEE divisions:
-                // the damping waves sit on every column's critical path
+            // A wave holding a damped cell runs the split update on all its lanes (in the
+            // undamped ones the damping and the loaded split are zero, so it is the unsplit
+            // update): the damping waves set the pace at every column's barrier.
+            const bool dv = PML && damp_cell_f(S, j, z);
+            if (PML && __any(dv)) {
+                // one hardware reciprocal (1 ulp) per direction instead of four IEEE divisions
                 const float d1 = 1.f - dt * o.dxv / 2, r2 = __builtin_amdgcn_rcpf(1.f + dt * o.dxv / 2);
                 const float d3 = 1.f - dt * o.dzv / 2, r4 = __builtin_amdgcn_rcpf(1.f + dt * o.dzv / 2);
                 const float vxx = (d1 * o.vxx + bx * idx * ax) * r2;
                 const float vxz = (d3 * (o.vx - o.vxx) + bx * idz * bz) * r4;
                 const float vzx = (d1 * (o.vz - o.vzz) + bx * idx * ex) * r2;
                 const float vzz = (d3 * o.vzz + bx * idz * cz) * r4;
-                if (zown && j >= x0 && j < x1) {
+                if (dv && zown && j >= x0 && j < x1) {
                     const uint32_t c = zo4 + (uint32_t)j * colb;
                     st(splitB(0), c, vxx);
                     st(splitB(3), c, vzz);
@@ -259,7 +262,8 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
                 const float dvxz = (c1 * (L2[L + 1] - L2[L]) + c2 * (L2[L + 2] - L2[L - 1]) +
                                     c3 * (L2[L + 3] - L2[L - 2]) + c4 * (L2[L + 4] - L2[L - 3])) * idz;
                 const float l = o.lam, mu = o.mu, l2m = l + 2 * mu;
-                if (PML && damp_cell_f(S, cs, z)) {
+                const bool ds = PML && damp_cell_f(S, cs, z);
+                if (PML && __any(ds)) {
                     const float d1 = 1.f - dt * o.dxs / 2, r2 = __builtin_amdgcn_rcpf(1.f + dt * o.dxs / 2);
                     const float d3 = 1.f - dt * o.dzs / 2, r4 = __builtin_amdgcn_rcpf(1.f + dt * o.dzs / 2);
                     const float a1 = (d1 * o.sxx + l2m * dvx) * r2;
@@ -268,9 +272,11 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
                     const float b2 = (d3 * o.szz + l2m * dvz) * r4;
                     const float e1 = (d1 * o.sxz + mu * dvzx) * r2;
                     const float e2 = (d3 * (txz - o.sxz) + mu * dvxz) * r4;
-                    st(splitB(4), c, a1);
-                    st(splitB(7), c, b2);
-                    st(splitB(8), c, e1);
+                    if (ds) {
+                        st(splitB(4), c, a1);
+                        st(splitB(7), c, b2);
+                        st(splitB(8), c, e1);
+                    }
                     txx = a1 + a2;
                     tzz = b1 + b2;
                     txz = e1 + e2;
