@@ -129,12 +129,13 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
         }
 }
 
-// Columns of a block of the LDS kernel: 128 when that divides a wider image
-// (row blocks x 128-column blocks: a smaller frame per pixel), else whole rows.
-__host__ __device__ inline int conv_lds_cols(int w) { return w > 128 && w % 128 == 0 ? 128 : w; }
+// Columns of a block of P pixels in the LDS kernel: P / 4 (blocks of 4 rows,
+// a smaller frame per pixel than whole rows) when that divides a wider image,
+// else whole rows.
+__host__ __device__ inline int conv_lds_cols(int w, int P) { return w > P / 4 && w % (P / 4) == 0 ? P / 4 : w; }
 
 // Same GEMM with the input staged through LDS: a block's P = 128 * WM pixels
-// are R = P / CB image rows of CB = conv_lds_cols(w) columns; per K step of 16
+// are R = P / CB image rows of CB = conv_lds_cols(w, P) columns; per K step of 16
 // channels the block loads them plus a one-pixel frame ((R + 2) x (CB + 2)
 // pixels, 32 B each, zero
 // outside the image, skip concatenation and upsampling resolved in the
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 31, kh = lane >> 5;
     constexpr int P = 4 * WM * 32, NC = WN * 32;
-    const int w = a.w, h = a.h, CB = conv_lds_cols(w), R = P / CB, wp = CB + 2;
+    const int w = a.w, h = a.h, CB = conv_lds_cols(w, P), R = P / CB, wp = CB + 2;
     u32x4* wl = tile + 2 * (R + 2) * wp;  // [9][NC][2]: 16 input channels of each tap and output channel
     const int ncb = w / CB, bpi = (h / R) * ncb;  // column blocks per row block, blocks per image
     const int nimg = blockIdx.x / bpi, rem = blockIdx.x % bpi;
@@ -231,17 +232,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
 }
 
-// The LDS kernel takes blocks of R = P / CB rows of CB = conv_lds_cols(w)
+// The LDS kernel takes blocks of R = P / CB rows of CB = conv_lds_cols(w, P)
 // columns: CB divides P, R divides h, channels come in 16s.
 bool conv_lds_ok(const ConvArgs& a, int P) {
     if (a.w <= 0 || a.h <= 0) return false;
-    const int cb = conv_lds_cols(a.w);
+    const int cb = conv_lds_cols(a.w, P);
     return P % cb == 0 && a.h % (P / cb) == 0 && a.c1 % 16 == 0 && a.c2 % 16 == 0;
 }
 
 // Dynamic LDS of the kernel: the framed pixel tile and the weights of nine taps.
 static size_t conv_lds_bytes(int w, int P, int nc) {
-    const int cb = conv_lds_cols(w);
+    const int cb = conv_lds_cols(w, P);
     return (size_t)(P / cb + 2) * (cb + 2) * 32 + 9 * (size_t)nc * 32;
 }
 
