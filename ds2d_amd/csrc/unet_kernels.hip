@@ -133,6 +133,10 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
 // a smaller frame per pixel than whole rows) when that divides a wider image,
 // else whole rows.
 __host__ __device__ inline int conv_lds_cols(int w, int P) { return w > P / 4 && w % (P / 4) == 0 ? P / 4 : w; }
+// 16-byte LDS items of one K step staged per thread: the framed tile (at most
+// 6 x (P / 4 + 2) pixels, two items each, for images at least 16 wide) and the
+// weights of nine taps for NC output channels (two items each).
+__host__ __device__ constexpr int conv_lds_stage(int P, int NC) { return (12 * (P / 4 + 2) + 18 * NC + 255) / 256; }
 
 // Same GEMM with the input staged through LDS: a block's P = 128 * WM pixels
 // are R = P / CB image rows of CB = conv_lds_cols(w, P) columns; per K step of 16
@@ -141,8 +145,10 @@ __host__ __device__ inline int conv_lds_cols(int w, int P) { return w > P / 4 &&
 // outside the image, skip concatenation and upsampling resolved in the
 // address) and the step's weights of all nine taps for the block's output
 // channels once; all nine taps then read both MFMA operands from LDS.
-template <int WM, int WN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_conv_lds(ConvArgs a) {
+// PIPE: stage the next K step in registers during the MFMAs (3 waves per SIMD)
+// instead of loading it after them (4 waves per SIMD).
+template <int WM, int WN, bool PIPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 3 : 4))) void k_conv_lds(ConvArgs a) {
     extern __shared__ u32x4 tile[];  // [(R + 2) * (CB + 2)][2]: 16 channels of one pixel; then the weights
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 31, kh = lane >> 5;
@@ -170,23 +176,69 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[m][j][r] = 0.f;
     const u32x4 zero = {0u, 0u, 0u, 0u};
-    for (int c = 0; c < ctot; c += 16) {
-        for (int q = tid; q < 2 * npx; q += 256) {
-            const int px = q >> 1, half = q & 1;
-            const int x = x0 - 1 + px / wp, z = zb0 + px % wp - 1;
+    // The items of K step c: the framed tile, then wl[(tap * NC + col) * 2 + half] =
+    // wt[tap][co0 + col][c + 8 * half .. + 8].  With PIPE they are loaded into
+    // registers while the MFMAs of step c - 16 run, and stored to LDS after them.
+    constexpr int NST = PIPE ? conv_lds_stage(P, NC) : 1;
+    const int ntot = 2 * npx + 18 * NC;
+    u32x4 stg[NST];
+    auto fetch = [&](int c) {
+#pragma unroll
+        for (int k = 0; k < NST; ++k) {
+            const int q = tid + 256 * k;
             u32x4 v = zero;
-            if (x >= 0 && x < h && z >= 0 && z < w)
-                v = c < a.c1 ? ld16(a.x1 + (((long long)nimg * h + x) * w + z) * a.c1 + c + 8 * half)
-                             : ld16(a.x2 + (((long long)nimg * h2 + (x >> a.up)) * w2 + (z >> a.up)) * a.c2 +
-                                    (c - a.c1) + 8 * half);
+            if (q < 2 * npx) {
+                const int px = q >> 1, half = q & 1;
+                const int x = x0 - 1 + px / wp, z = zb0 + px % wp - 1;
+                if (x >= 0 && x < h && z >= 0 && z < w)
+                    v = c < a.c1 ? ld16(a.x1 + (((long long)nimg * h + x) * w + z) * a.c1 + c + 8 * half)
+                                 : ld16(a.x2 + (((long long)nimg * h2 + (x >> a.up)) * w2 + (z >> a.up)) * a.c2 +
+                                        (c - a.c1) + 8 * half);
+            } else if (q < ntot) {
+                const int qw = q - 2 * npx, half = qw & 1, r = qw >> 1, tap = r / NC, col = r % NC;
+                v = ld16(a.wt + ((long long)tap * a.cout + co0 + col) * ctot + c + 8 * half);
+            }
+            stg[k] = v;
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int k = 0; k < NST; ++k) {
+            const int q = tid + 256 * k;
+            if (q < ntot) tile[q] = stg[k];
+        }
+    };
+    // without PIPE: the items of step c straight into LDS
+    auto load_step = [&](int c) {
+        for (int q = tid; q < ntot; q += 256) {
+            u32x4 v = zero;
+            if (q < 2 * npx) {
+                const int px = q >> 1, half = q & 1;
+                const int x = x0 - 1 + px / wp, z = zb0 + px % wp - 1;
+                if (x >= 0 && x < h && z >= 0 && z < w)
+                    v = c < a.c1 ? ld16(a.x1 + (((long long)nimg * h + x) * w + z) * a.c1 + c + 8 * half)
+                                 : ld16(a.x2 + (((long long)nimg * h2 + (x >> a.up)) * w2 + (z >> a.up)) * a.c2 +
+                                        (c - a.c1) + 8 * half);
+            } else {
+                const int qw = q - 2 * npx, half = qw & 1, r = qw >> 1, tap = r / NC, col = r % NC;
+                v = ld16(a.wt + ((long long)tap * a.cout + co0 + col) * ctot + c + 8 * half);
+            }
             tile[q] = v;
         }
-        // wl[(tap * NC + col) * 2 + half] = wt[tap][co0 + col][c + 8 * half .. + 8]
-        for (int q = tid; q < 9 * NC * 2; q += 256) {
-            const int half = q & 1, r = q >> 1, tap = r / NC, col = r % NC;
-            wl[q] = ld16(a.wt + ((long long)tap * a.cout + co0 + col) * ctot + c + 8 * half);
+    };
+    if constexpr (PIPE) {
+        fetch(0);
+        stash();
+    }
+    for (int c = 0; c < ctot; c += 16) {
+        const bool more = c + 16 < ctot;
+        if constexpr (PIPE) {
+            __syncthreads();
+            if (more) fetch(c + 16);
+        } else {
+            load_step(c);
+            __syncthreads();
         }
-        __syncthreads();
 #pragma unroll 3
         for (int tap = 0; tap < 9; ++tap) {
             const int off = (tap / 3 - 1) * wp + (tap % 3 - 1);
@@ -201,6 +253,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 for (int j = 0; j < WN; ++j) acc[m][j] = mfma(A[m], B[j], acc[m][j]);
         }
         __syncthreads();
+        if constexpr (PIPE) {
+            if (more) stash();
+        }
     }
     const long long img0 = (long long)nimg * h * w;
     const int c2o = a.cout - a.split;
@@ -235,9 +290,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // The LDS kernel takes blocks of R = P / CB rows of CB = conv_lds_cols(w, P)
 // columns: CB divides P, R divides h, channels come in 16s.
 bool conv_lds_ok(const ConvArgs& a, int P) {
-    if (a.w <= 0 || a.h <= 0) return false;
-    const int cb = conv_lds_cols(a.w, P);
-    return P % cb == 0 && a.h % (P / cb) == 0 && a.c1 % 16 == 0 && a.c2 % 16 == 0;
+    if (a.w < 16 || a.h <= 0) return false;
+    const int cb = conv_lds_cols(a.w, P), nc = P == 256 ? 64 : 32;
+    const int npx = (P / cb + 2) * (cb + 2);
+    return P % cb == 0 && a.h % (P / cb) == 0 && a.c1 % 16 == 0 && a.c2 % 16 == 0 &&
+           2 * npx + 18 * nc <= 256 * conv_lds_stage(P, nc);
 }
 
 // Dynamic LDS of the kernel: the framed pixel tile and the weights of nine taps.
@@ -250,12 +307,12 @@ void launch_conv(hipStream_t st, const ConvArgs& a) {
     const long long npix = (long long)a.n * a.h * a.w;
     if (a.cout % 64 == 0 && conv_lds_ok(a, 256)) {
         const size_t lds = conv_lds_bytes(a.w, 256, 64);
-        hipLaunchKernelGGL((k_conv_lds<2, 2>), dim3((unsigned)(npix / 256), a.cout / 64), dim3(256), lds, st, a);
+        hipLaunchKernelGGL((k_conv_lds<2, 2, true>), dim3((unsigned)(npix / 256), a.cout / 64), dim3(256), lds, st, a);
         return;
     }
     if (a.cout % 64 != 0 && conv_lds_ok(a, 512)) {
         const size_t lds = conv_lds_bytes(a.w, 512, 32);
-        hipLaunchKernelGGL((k_conv_lds<4, 1>), dim3((unsigned)(npix / 512), a.cout / 32), dim3(256), lds, st, a);
+        hipLaunchKernelGGL((k_conv_lds<4, 1, false>), dim3((unsigned)(npix / 512), a.cout / 32), dim3(256), lds, st, a);
         return;
     }
     if (a.cout % 64 == 0) {
