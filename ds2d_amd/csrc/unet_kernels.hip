@@ -330,17 +330,24 @@ void launch_conv(hipStream_t st, const ConvArgs& a) {
 constexpr int ZS = 64;  // z segment (pixels of K per row)
 constexpr int LP = 72;  // LDS row pitch in elements (144 B: 16-byte aligned rows)
 
+// LDS rows of the weight gradient are swizzled in 16-byte chunks: logical chunk
+// j < 8 (pixels 8j .. 8j + 7) of channel row c sits at chunk j ^ ((c >> 3) & 7);
+// chunk 8 (pixels 64, 65 of an input row) stays in place.  The transposing
+// writes of 8 channels x 2 pixels then fall in distinct banks.
+__device__ __forceinline__ int wg_chunk(int c, int j) { return j < 8 ? j ^ ((c >> 3) & 7) : j; }
+
 // Block = (TCO/32) x (TCI/32) x DS waves; wave (cot, cit, dsel) accumulates the
 // 32x32 (co, ci) tile of the taps dx = d - 1 with d = dsel, dsel + DS, ... < 3.
 // Input rows live in a 4-slot LDS ring (row r in slot r & 3, r >= -1) and dy
 // rows in two buffers, so the global loads of rows x + 2 (input) and x + 1
 // (dy) are issued before the MFMAs of row x and land in LDS after them, with
-// one barrier per row.
+// one barrier per row.  Each thread moves two pixels x 8 channels per item
+// (two 16-byte loads, eight 4-byte LDS writes of a channel's pixel pair).
 template <int TCO, int TCI, int DS>
-__global__ __launch_bounds__(64 * (TCO / 32) * (TCI / 32) * DS) void k_wgrad(WgradArgs a) {
+__global__ __launch_bounds__(64 * (TCO / 32) * (TCI / 32) * DS) __attribute__((amdgpu_waves_per_eu(4))) void k_wgrad(WgradArgs a) {
     constexpr int NT = 64 * (TCO / 32) * (TCI / 32) * DS;
     constexpr int GI = TCI / 8, GO = TCO / 8;
-    constexpr int QI = (ZS + 2) * GI, QO = ZS * GO;
+    constexpr int QI = (ZS + 2) / 2 * GI, QO = ZS / 2 * GO;  // items: (pixel pair, channel group)
     constexpr int NI = (QI + NT - 1) / NT, NO = (QO + NT - 1) / NT;
     constexpr int ND = 3 / DS;
     static_assert(DS == 1 || DS == 3, "taps split over 1 or 3 waves");
@@ -361,52 +368,61 @@ __global__ __launch_bounds__(64 * (TCO / 32) * (TCI / 32) * DS) void k_wgrad(Wgr
     const int xa = xb * a.rows, xe = min(a.h, xa + a.rows);
     const int h2 = a.h >> a.up, w2 = a.w >> a.up;
     const int nks = (min(ZS, a.w - zs) + 15) / 16;
+    const u32x4 zero = {0u, 0u, 0u, 0u};
 
-    // input row x: item q = (zz, g) holds in[n][x][zs + zz - 1][ci0 + 8g .. + 8], zero outside the image
-    u32x4 ri[NI], ro[NO];
+    // Four 4-byte words per 16-byte pixel load: word k holds channels 2k, 2k + 1.
+    // (v0, v1) = pixels (zz, zz + 1) of channels 8g .. 8g + 7 -> rows 8g + 2k, 8g + 2k + 1.
+    auto put_pair = [](bf16_t (*rows)[LP], int g, int zz, const u32x4& v0, const u32x4& v1) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int c = 8 * g + 2 * k, e = wg_chunk(c, zz >> 3) * 8 + (zz & 7);
+            *reinterpret_cast<unsigned*>(&rows[c][e]) = (v0[k] & 0xFFFFu) | (v1[k] << 16);
+            *reinterpret_cast<unsigned*>(&rows[c + 1][e]) = (v0[k] >> 16) | (v1[k] & 0xFFFF0000u);
+        }
+    };
+    // input row x: item q = (pair p, g) holds in[n][x][zs + 2p - 1 .. + 1][ci0 + 8g .. + 8], zero outside the image
+    u32x4 ri[NI][2], ro[NO][2];
+    auto in_px = [&](int x, int z, int c) -> u32x4 {
+        if (x < 0 || x >= a.h || z < 0 || z >= a.w) return zero;
+        return c < a.c1 ? ld16(a.x1 + (((long long)n * a.h + x) * a.w + z) * a.c1 + c)
+                        : ld16(a.x2 + (((long long)n * h2 + (x >> a.up)) * w2 + (z >> a.up)) * a.c2 + (c - a.c1));
+    };
     auto fetch_in = [&](int x) {
 #pragma unroll
         for (int k = 0; k < NI; ++k) {
-            const int q = tid + k * NT, zz = q / GI, g = q % GI;
-            const int z = zs + zz - 1, c = ci0 + g * 8;
-            ri[k] = u32x4{0u, 0u, 0u, 0u};
-            if (q < QI && x >= 0 && x < a.h && z >= 0 && z < a.w)
-                ri[k] = c < a.c1 ? ld16(a.x1 + (((long long)n * a.h + x) * a.w + z) * a.c1 + c)
-                                 : ld16(a.x2 + (((long long)n * h2 + (x >> a.up)) * w2 + (z >> a.up)) * a.c2 +
-                                        (c - a.c1));
+            const int q = tid + k * NT, p = q / GI, g = q % GI;
+            const int z = zs + 2 * p - 1, c = ci0 + g * 8;
+            ri[k][0] = ri[k][1] = zero;
+            if (q < QI) {
+                ri[k][0] = in_px(x, z, c);
+                ri[k][1] = in_px(x, z + 1, c);
+            }
         }
     };
     auto store_in = [&](int slot) {
 #pragma unroll
         for (int k = 0; k < NI; ++k) {
-            const int q = tid + k * NT, zz = q / GI, g = q % GI;
-            if (q < QI)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    Bs[slot][g * 8 + 2 * e][zz] = (bf16_t)(ri[k][e] & 0xFFFFu);
-                    Bs[slot][g * 8 + 2 * e + 1][zz] = (bf16_t)(ri[k][e] >> 16);
-                }
+            const int q = tid + k * NT, p = q / GI, g = q % GI;
+            if (q < QI) put_pair(Bs[slot], g, 2 * p, ri[k][0], ri[k][1]);
         }
     };
-    // dy row x: item q = (zz, g) holds dy[n][x][zs + zz][co0 + 8g .. + 8], zero beyond the image
+    // dy row x: item q = (pair p, g) holds dy[n][x][zs + 2p .. + 1][co0 + 8g .. + 8], zero beyond the image
     auto fetch_dy = [&](int x) {
 #pragma unroll
         for (int k = 0; k < NO; ++k) {
-            const int q = tid + k * NT, zz = q / GO, g = q % GO;
-            ro[k] = u32x4{0u, 0u, 0u, 0u};
-            if (q < QO && zs + zz < a.w) ro[k] = ld16(a.dy + (((long long)n * a.h + x) * a.w + zs + zz) * a.cout + co0 + g * 8);
+            const int q = tid + k * NT, p = q / GO, g = q % GO;
+            const int z = zs + 2 * p;
+            const bf16_t* src = a.dy + (((long long)n * a.h + x) * a.w + z) * a.cout + co0 + g * 8;
+            ro[k][0] = ro[k][1] = zero;
+            if (q < QO && z < a.w) ro[k][0] = ld16(src);
+            if (q < QO && z + 1 < a.w) ro[k][1] = ld16(src + a.cout);
         }
     };
     auto store_dy = [&](int buf) {
 #pragma unroll
         for (int k = 0; k < NO; ++k) {
-            const int q = tid + k * NT, zz = q / GO, g = q % GO;
-            if (q < QO)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    As[buf][g * 8 + 2 * e][zz] = (bf16_t)(ro[k][e] & 0xFFFFu);
-                    As[buf][g * 8 + 2 * e + 1][zz] = (bf16_t)(ro[k][e] >> 16);
-                }
+            const int q = tid + k * NT, p = q / GO, g = q % GO;
+            if (q < QO) put_pair(As[buf], g, 2 * p, ro[k][0], ro[k][1]);
         }
     };
 
@@ -437,13 +453,14 @@ __global__ __launch_bounds__(64 * (TCO / 32) * (TCI / 32) * DS) void k_wgrad(Wgr
         }
         for (int s = 0; s < nks; ++s) {
             const int base = 16 * s + 8 * kh;
-            const u32x4 A = *reinterpret_cast<const u32x4*>(&As[x & 1][cot * 32 + li][base]);
+            const int ca = cot * 32 + li, cb = cit * 32 + li, jb = base >> 3;
+            const u32x4 A = *reinterpret_cast<const u32x4*>(&As[x & 1][ca][wg_chunk(ca, jb) * 8]);
 #pragma unroll
             for (int j = 0; j < ND; ++j) {
                 const int d = dsel + j * DS;  // dx = d - 1: input row x + d - 1
-                const bf16_t* row = &Bs[(x + d + 3) & 3][cit * 32 + li][base];
-                const u32x4 q = *reinterpret_cast<const u32x4*>(row);
-                const unsigned e = *reinterpret_cast<const unsigned*>(row + 8);
+                const bf16_t* row = Bs[(x + d + 3) & 3][cb];
+                const u32x4 q = *reinterpret_cast<const u32x4*>(row + wg_chunk(cb, jb) * 8);
+                const unsigned e = *reinterpret_cast<const unsigned*>(row + wg_chunk(cb, jb + 1) * 8);
                 const u32x4 bm = q;                                                   // dz = -1
                 const u32x4 b0 = {__builtin_amdgcn_alignbyte(q[1], q[0], 2), __builtin_amdgcn_alignbyte(q[2], q[1], 2),
                                   __builtin_amdgcn_alignbyte(q[3], q[2], 2), __builtin_amdgcn_alignbyte(e, q[3], 2)};
@@ -454,10 +471,10 @@ __global__ __launch_bounds__(64 * (TCO / 32) * (TCI / 32) * DS) void k_wgrad(Wgr
             }
         }
         if (do_bias && bpart < NB) {
-            const u32x4* rp = reinterpret_cast<const u32x4*>(&As[x & 1][bco][bpart * (ZS / NB)]);
+            const bf16_t* rp = As[x & 1][bco];
 #pragma unroll
             for (int k = 0; k < ZS / NB / 8; ++k) {
-                const u32x4 v = rp[k];
+                const u32x4 v = *reinterpret_cast<const u32x4*>(rp + wg_chunk(bco, bpart * (ZS / NB / 8) + k) * 8);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     bacc += __uint_as_float(v[e] << 16);
