@@ -133,10 +133,6 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
 // a smaller frame per pixel than whole rows) when that divides a wider image,
 // else whole rows.
 __host__ __device__ inline int conv_lds_cols(int w, int P) { return w > P / 4 && w % (P / 4) == 0 ? P / 4 : w; }
-// 16-byte LDS items of one K step staged per thread: the framed tile (at most
-// 6 x (P / 4 + 2) pixels, two items each, for images at least 16 wide) and the
-// weights of nine taps for NC output channels (two items each).
-__host__ __device__ constexpr int conv_lds_stage(int P, int NC) { return (12 * (P / 4 + 2) + 18 * NC + 255) / 256; }
 
 // Same GEMM with the input staged through LDS: a block's P = 128 * WM pixels
 // are R = P / CB image rows of CB = conv_lds_cols(w, P) columns; per K step of 16
@@ -147,13 +143,15 @@ __host__ __device__ constexpr int conv_lds_stage(int P, int NC) { return (12 * (
 // channels once; all nine taps then read both MFMA operands from LDS.
 // PIPE: stage the next K step in registers during the MFMAs (3 waves per SIMD)
 // instead of loading it after them (4 waves per SIMD).
-template <int WM, int WN, bool PIPE>
+// CB = conv_lds_cols(w, P) is a template argument (P / 4, P / 8 or P / 16), so
+// the pixel <-> (row, column) arithmetic of the tile is by constants.
+template <int WM, int WN, bool PIPE, int CB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 3 : 4))) void k_conv_lds(ConvArgs a) {
     extern __shared__ u32x4 tile[];  // [(R + 2) * (CB + 2)][2]: 16 channels of one pixel; then the weights
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 31, kh = lane >> 5;
-    constexpr int P = 4 * WM * 32, NC = WN * 32;
-    const int w = a.w, h = a.h, CB = conv_lds_cols(w, P), R = P / CB, wp = CB + 2;
+    constexpr int P = 4 * WM * 32, NC = WN * 32, R = P / CB, wp = CB + 2;
+    const int w = a.w, h = a.h;
     u32x4* wl = tile + 2 * (R + 2) * wp;  // [9][NC][2]: 16 input channels of each tap and output channel
     const int ncb = w / CB, bpi = (h / R) * ncb;  // column blocks per row block, blocks per image
     const int nimg = blockIdx.x / bpi, rem = blockIdx.x % bpi;
@@ -161,7 +159,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 3 : 
     const int co0 = blockIdx.y * (WN * 32);
     const int ctot = a.c1 + a.c2;
     const int h2 = h >> a.up, w2 = w >> a.up;
-    const int npx = (R + 2) * wp;
+    constexpr int npx = (R + 2) * wp;
     int lp[WM];  // LDS pixel of each tile row's pixel at tap (0, 0)
 #pragma unroll
     for (int m = 0; m < WM; ++m) {
@@ -179,8 +177,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 3 : 
     // The items of K step c: the framed tile, then wl[(tap * NC + col) * 2 + half] =
     // wt[tap][co0 + col][c + 8 * half .. + 8].  With PIPE they are loaded into
     // registers while the MFMAs of step c - 16 run, and stored to LDS after them.
-    constexpr int NST = PIPE ? conv_lds_stage(P, NC) : 1;
-    const int ntot = 2 * npx + 18 * NC;
+    constexpr int ntot = 2 * npx + 18 * NC;
+    constexpr int NST = PIPE ? (ntot + 255) / 256 : 1;
     u32x4 stg[NST];
     auto fetch = [&](int c) {
 #pragma unroll
@@ -291,10 +289,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 3 : 
 // columns: CB divides P, R divides h, channels come in 16s.
 bool conv_lds_ok(const ConvArgs& a, int P) {
     if (a.w < 16 || a.h <= 0) return false;
-    const int cb = conv_lds_cols(a.w, P), nc = P == 256 ? 64 : 32;
-    const int npx = (P / cb + 2) * (cb + 2);
-    return P % cb == 0 && a.h % (P / cb) == 0 && a.c1 % 16 == 0 && a.c2 % 16 == 0 &&
-           2 * npx + 18 * nc <= 256 * conv_lds_stage(P, nc);
+    const int cb = conv_lds_cols(a.w, P);
+    return (cb == P / 4 || cb == P / 8 || cb == P / 16) && a.h % (P / cb) == 0 && a.c1 % 16 == 0 &&
+           a.c2 % 16 == 0;
 }
 
 // Dynamic LDS of the kernel: the framed pixel tile and the weights of nine taps.
@@ -303,16 +300,26 @@ static size_t conv_lds_bytes(int w, int P, int nc) {
     return (size_t)(P / cb + 2) * (cb + 2) * 32 + 9 * (size_t)nc * 32;
 }
 
+template <int WM, int WN, bool PIPE>
+static void launch_conv_lds(hipStream_t st, const ConvArgs& a, long long npix) {
+    constexpr int P = 4 * WM * 32;
+    const dim3 grid((unsigned)(npix / P), a.cout / (WN * 32));
+    const size_t lds = conv_lds_bytes(a.w, P, WN * 32);
+    switch (conv_lds_cols(a.w, P)) {
+        case P / 4: hipLaunchKernelGGL((k_conv_lds<WM, WN, PIPE, P / 4>), grid, dim3(256), lds, st, a); break;
+        case P / 8: hipLaunchKernelGGL((k_conv_lds<WM, WN, PIPE, P / 8>), grid, dim3(256), lds, st, a); break;
+        default: hipLaunchKernelGGL((k_conv_lds<WM, WN, PIPE, P / 16>), grid, dim3(256), lds, st, a); break;
+    }
+}
+
 void launch_conv(hipStream_t st, const ConvArgs& a) {
     const long long npix = (long long)a.n * a.h * a.w;
     if (a.cout % 64 == 0 && conv_lds_ok(a, 256)) {
-        const size_t lds = conv_lds_bytes(a.w, 256, 64);
-        hipLaunchKernelGGL((k_conv_lds<2, 2, true>), dim3((unsigned)(npix / 256), a.cout / 64), dim3(256), lds, st, a);
+        launch_conv_lds<2, 2, true>(st, a, npix);
         return;
     }
     if (a.cout % 64 != 0 && conv_lds_ok(a, 512)) {
-        const size_t lds = conv_lds_bytes(a.w, 512, 32);
-        hipLaunchKernelGGL((k_conv_lds<4, 1, false>), dim3((unsigned)(npix / 512), a.cout / 32), dim3(256), lds, st, a);
+        launch_conv_lds<4, 1, false>(st, a, npix);
         return;
     }
     if (a.cout % 64 == 0) {

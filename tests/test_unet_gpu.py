@@ -49,7 +49,7 @@ def test_conv_forward_matches_oracle(c1, c2, up, cout):
 @pytest.mark.parametrize("c1,c2,up,cout,h,w", [(32, 0, 0, 32, 16, 32), (64, 0, 0, 64, 8, 64), (32, 64, 1, 32, 16, 32),
                                                (64, 64, 0, 128, 8, 32), (128, 256, 1, 128, 16, 16),
                                                (32, 32, 0, 64, 4, 256), (32, 0, 0, 32, 8, 256),
-                                               (32, 64, 1, 32, 4, 384), (64, 64, 1, 64, 8, 128)])
+                                               (32, 64, 1, 32, 4, 384), (64, 64, 1, 64, 8, 128), (32, 0, 0, 32, 8, 64)])
 def test_conv_lds_kernel_matches_oracle(c1, c2, up, cout, h, w):
     """Shapes whose blocks are image rows (whole rows, or 4-row blocks of
     P / 4 columns on wider images) take the LDS-staged kernel (forward with
