@@ -319,7 +319,7 @@ void launch_conv(hipStream_t st, const ConvArgs& a) {
         return;
     }
     if (a.cout % 64 != 0 && conv_lds_ok(a, 512)) {
-        launch_conv_lds<4, 1, false>(st, a, npix);
+        launch_conv_lds<4, 1, true>(st, a, npix);
         return;
     }
     if (a.cout % 64 == 0) {
