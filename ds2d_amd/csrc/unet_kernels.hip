@@ -141,12 +141,12 @@ __host__ __device__ inline int conv_lds_cols(int w, int P) { return w > P / 4 &&
 // outside the image, skip concatenation and upsampling resolved in the
 // address) and the step's weights of all nine taps for the block's output
 // channels once; all nine taps then read both MFMA operands from LDS.
-// PIPE: stage the next K step in registers during the MFMAs (3 waves per SIMD)
-// instead of loading it after them (4 waves per SIMD).
-// CB = conv_lds_cols(w, P) is a template argument (P / 4, P / 8 or P / 16), so
-// the pixel <-> (row, column) arithmetic of the tile is by constants.
-template <int WM, int WN, bool PIPE, int CB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 3 : 4))) void k_conv_lds(ConvArgs a) {
+// The next K step is staged in registers while the MFMAs of the current one run
+// (3 waves per SIMD).  CB = conv_lds_cols(w, P) is a template argument (P / 4,
+// P / 8 or P / 16), so the pixel <-> (row, column) arithmetic of the tile is by
+// constants.
+template <int WM, int WN, int CB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_conv_lds(ConvArgs a) {
     extern __shared__ u32x4 tile[];  // [(R + 2) * (CB + 2)][2]: 16 channels of one pixel; then the weights
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 31, kh = lane >> 5;
@@ -175,10 +175,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 3 : 
             for (int r = 0; r < 16; ++r) acc[m][j][r] = 0.f;
     const u32x4 zero = {0u, 0u, 0u, 0u};
     // The items of K step c: the framed tile, then wl[(tap * NC + col) * 2 + half] =
-    // wt[tap][co0 + col][c + 8 * half .. + 8].  With PIPE they are loaded into
-    // registers while the MFMAs of step c - 16 run, and stored to LDS after them.
+    // wt[tap][co0 + col][c + 8 * half .. + 8].  They are loaded into registers
+    // while the MFMAs of step c - 16 run, and stored to LDS after them.
     constexpr int ntot = 2 * npx + 18 * NC;
-    constexpr int NST = PIPE ? (ntot + 255) / 256 : 1;
+    constexpr int NST = (ntot + 255) / 256;
     u32x4 stg[NST];
     auto fetch = [&](int c) {
 #pragma unroll
@@ -206,37 +206,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 3 : 
             if (q < ntot) tile[q] = stg[k];
         }
     };
-    // without PIPE: the items of step c straight into LDS
-    auto load_step = [&](int c) {
-        for (int q = tid; q < ntot; q += 256) {
-            u32x4 v = zero;
-            if (q < 2 * npx) {
-                const int px = q >> 1, half = q & 1;
-                const int x = x0 - 1 + px / wp, z = zb0 + px % wp - 1;
-                if (x >= 0 && x < h && z >= 0 && z < w)
-                    v = c < a.c1 ? ld16(a.x1 + (((long long)nimg * h + x) * w + z) * a.c1 + c + 8 * half)
-                                 : ld16(a.x2 + (((long long)nimg * h2 + (x >> a.up)) * w2 + (z >> a.up)) * a.c2 +
-                                        (c - a.c1) + 8 * half);
-            } else {
-                const int qw = q - 2 * npx, half = qw & 1, r = qw >> 1, tap = r / NC, col = r % NC;
-                v = ld16(a.wt + ((long long)tap * a.cout + co0 + col) * ctot + c + 8 * half);
-            }
-            tile[q] = v;
-        }
-    };
-    if constexpr (PIPE) {
-        fetch(0);
-        stash();
-    }
+    fetch(0);
+    stash();
     for (int c = 0; c < ctot; c += 16) {
         const bool more = c + 16 < ctot;
-        if constexpr (PIPE) {
-            __syncthreads();
-            if (more) fetch(c + 16);
-        } else {
-            load_step(c);
-            __syncthreads();
-        }
+        __syncthreads();
+        if (more) fetch(c + 16);
 #pragma unroll 3
         for (int tap = 0; tap < 9; ++tap) {
             const int off = (tap / 3 - 1) * wp + (tap % 3 - 1);
@@ -251,9 +226,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 3 : 
                 for (int j = 0; j < WN; ++j) acc[m][j] = mfma(A[m], B[j], acc[m][j]);
         }
         __syncthreads();
-        if constexpr (PIPE) {
-            if (more) stash();
-        }
+        if (more) stash();
     }
     const long long img0 = (long long)nimg * h * w;
     const int c2o = a.cout - a.split;
@@ -300,26 +273,26 @@ static size_t conv_lds_bytes(int w, int P, int nc) {
     return (size_t)(P / cb + 2) * (cb + 2) * 32 + 9 * (size_t)nc * 32;
 }
 
-template <int WM, int WN, bool PIPE>
+template <int WM, int WN>
 static void launch_conv_lds(hipStream_t st, const ConvArgs& a, long long npix) {
     constexpr int P = 4 * WM * 32;
     const dim3 grid((unsigned)(npix / P), a.cout / (WN * 32));
     const size_t lds = conv_lds_bytes(a.w, P, WN * 32);
     switch (conv_lds_cols(a.w, P)) {
-        case P / 4: hipLaunchKernelGGL((k_conv_lds<WM, WN, PIPE, P / 4>), grid, dim3(256), lds, st, a); break;
-        case P / 8: hipLaunchKernelGGL((k_conv_lds<WM, WN, PIPE, P / 8>), grid, dim3(256), lds, st, a); break;
-        default: hipLaunchKernelGGL((k_conv_lds<WM, WN, PIPE, P / 16>), grid, dim3(256), lds, st, a); break;
+        case P / 4: hipLaunchKernelGGL((k_conv_lds<WM, WN, P / 4>), grid, dim3(256), lds, st, a); break;
+        case P / 8: hipLaunchKernelGGL((k_conv_lds<WM, WN, P / 8>), grid, dim3(256), lds, st, a); break;
+        default: hipLaunchKernelGGL((k_conv_lds<WM, WN, P / 16>), grid, dim3(256), lds, st, a); break;
     }
 }
 
 void launch_conv(hipStream_t st, const ConvArgs& a) {
     const long long npix = (long long)a.n * a.h * a.w;
     if (a.cout % 64 == 0 && conv_lds_ok(a, 256)) {
-        launch_conv_lds<2, 2, true>(st, a, npix);
+        launch_conv_lds<2, 2>(st, a, npix);
         return;
     }
     if (a.cout % 64 != 0 && conv_lds_ok(a, 512)) {
-        launch_conv_lds<4, 1, true>(st, a, npix);
+        launch_conv_lds<4, 1>(st, a, npix);
         return;
     }
     if (a.cout % 64 == 0) {
