@@ -814,10 +814,38 @@ int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nst
     static const Cand cand[] = {{0, 0, 0},     {32, 256, 2},  {64, 256, 2},  {64, 256, 4},
                                 {128, 256, 4}, {64, 512, 4},  {64, 768, 2},  {64, 768, 4},
                                 {128, 768, 4}, {64, 704, 4},  {112, 704, 4}, {160, 704, 4}};
+    std::vector<Cand> cands(std::begin(cand), std::end(cand));
+    // 704-lane tiles whose chunk count just fills R rounds of one workgroup per CU
+    // (R = 2, 3, 4): the step time moves with the number of rounds, not with the width alone
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0) == hipSuccess && ncu > 0) {
+        const int pml = h->g.pml, nzp = h->g.allnz + 2 * pml, rb = (nzp + 687) / 688;
+        int max_nxp = 1;
+        for (int i = 0; i < nshots; ++i) max_nxp = std::max(max_nxp, shots[i].nx + 2 * pml);
+        auto nchunks = [&](int W) {
+            long long k = 0;
+            for (int i = 0; i < nshots; ++i) k += (shots[i].nx + 2 * pml + W - 1) / W;
+            return k * rb;
+        };
+        for (int R = 2; R <= 4; ++R) {
+            int lo = 16, hi = std::max(16, max_nxp);  // smallest W with nchunks(W) <= R * ncu
+            if (nchunks(hi) > (long long)R * ncu) continue;
+            while (lo < hi) {
+                const int mid = (lo + hi) / 2;
+                if (nchunks(mid) <= (long long)R * ncu)
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+            bool have = false;
+            for (const Cand& c : cands) have = have || (c.w == lo && c.t == 704 && c.d == 4);
+            if (!have) cands.push_back(Cand{lo, 704, 4});
+        }
+    }
     Cand best{64, 256, 2};
     double bt = 1e300;
     const int keep_w = h->chunk_w, keep_t = h->chunk_t, keep_d = h->chunk_d, keep_two = h->fwd_two;
-    for (const Cand& c : cand) {
+    for (const Cand& c : cands) {
         h->fwd_two = c.w == 0;
         h->chunk_w = c.w;
         h->chunk_t = c.t;

@@ -1,6 +1,6 @@
 """GPU vs the FP64 oracle at the full sizes of the benchmark configurations:
 C1 (one 291x141 window, 2500 steps), one window of the C3 batch (128 uneven
-windows on 2000x600, 3000 steps, the autotuned-size tile) and two shots of the
+windows on 2000x600, 3000 steps, an autotuner-size tile) and two shots of the
 C2 Born data (26 shots, 331x181, 2500 steps).  Tolerance: relative L2 <= 1e-5;
 the FP32 restatement of the oracle is printed beside it as the FP32 floor."""
 import numpy as np
@@ -40,12 +40,13 @@ def test_c1_full_size_matches_oracle():
 
 
 def test_c3_window_at_full_size_matches_oracle():
-    """The whole 128-window C3 batch runs on the GPU with the 160x704x4 tile; its
-    narrowest window is checked against the FP64 oracle over all 3000 steps."""
+    """The whole 128-window C3 batch runs on the GPU with the 240x704x4 tile (512
+    chunks: two rounds of 256 CUs); its narrowest window is checked against the
+    FP64 oracle over all 3000 steps."""
     cfg = models.c3_batched_elastic()
     s, wav = _solver(cfg, pml_rr=1e-3)
     s.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
-    s.set_chunk(160, 704, 4)
+    s.set_chunk(240, 704, 4)
     rz, rx, _ = s.batch_forward(cfg["shots"], mode=2)
     k = int(np.argmin([x.nx for x in cfg["shots"]]))
     off = sum(x.ntr for x in cfg["shots"][:k]) * cfg["nt"]
