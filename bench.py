@@ -47,7 +47,8 @@ def parse():
     ap.add_argument("--c5-iter", type=int, default=2, help="c5: CG iterations on the full configuration")
     ap.add_argument("--no-window-parity", action="store_true", help="c5: skip the FP64-oracle window comparison")
     ap.add_argument("--train-models", type=int, default=64, help="c4: training models (seeds 1000..)")
-    ap.add_argument("--eval-models", type=int, default=4, help="c4: held-out models (seeds 2000..)")
+    ap.add_argument("--eval-models", type=int, default=4, help="c4: held-out models (seeds eval-start..)")
+    ap.add_argument("--eval-start", type=int, default=2000, help="c4: seed of the first held-out model")
     ap.add_argument("--train-steps", type=int, default=1500, help="c4: Adam steps")
     ap.add_argument("--max-iter", type=int, default=60, help="c4: iteration cap of each solver")
     ap.add_argument("--target", type=float, default=1e-2, help="c4: relative data residual target")
@@ -276,9 +277,10 @@ def run_learned(a):
         if a.save_unet:
             np.save(a.save_unet, u.get_params())
     evals = []
-    for seed in range(2000, 2000 + a.eval_models):
+    for seed in range(a.eval_start, a.eval_start + a.eval_models):
         prob = LR.acoustic_problem(seed, **kw)
         r = LR.compare(s, u, prob, a.max_iter, a.target, log=log)
+        r["seed"] = seed
         evals.append(r)
         print(f"seed {seed}: plain {r['plain']['resid'][-1]:.4f} ({r['plain']['iters_to_target']}) learned "
               f"{r['learned']['resid'][-1]:.4f} ({r['learned']['iters_to_target']})", file=sys.stderr, flush=True)
@@ -326,7 +328,7 @@ def run_learned(a):
         kp, kl = LR.first_hit(rp, tgt), LR.first_hit(rl, tgt)
         n = len(rp) - 1
         rows.append({
-            "target": tgt, "iters_plain": kp, "iters_learned": kl,
+            "seed": e["seed"], "target": tgt, "iters_plain": kp, "iters_learned": kl,
             "time_plain_reference_scheme_s": round(fixed_ref + kp * per_iter_ref, 2),
             "time_plain_one_shot_per_launch_s": round(fixed_single + kp * per_iter_single, 2),
             "time_plain_batched_s": round(e["plain"]["wall_s"] * kp / n, 2),
@@ -343,7 +345,7 @@ def run_learned(a):
         "dtype": "f32 stencils, bf16 U-Net (fp32 accumulate)",
         "data": "synthetic: seeded 512x256 layered+faulted models with lenses; Born data of the true reflectivity",
         "config": {"workload": f"c4_learned_lsrtm: {kw['nx']}x{kw['nz']}, {kw['nshots']} shots, nt {kw['nt']}, "
-                               f"{a.train_models} training models, {a.eval_models} held-out, U-Net 4 levels x 32",
+                               f"{a.train_models} training models, {a.eval_models} held-out (seeds {a.eval_start}..), U-Net 4 levels x 32",
                    "target": a.target, "max_iter": a.max_iter, "per_model": rows,
                    "speedup_learned_vs_plain_both_batched": round(speed_learn, 3),
                    "speedup_vs_plain_one_shot_per_launch_resident_history": round(speed_single, 3),
