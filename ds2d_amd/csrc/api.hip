@@ -1016,7 +1016,7 @@ int acou_prepare(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouWS
     w.fused = false;
     w.tune = false;
     if (w.store && n <= FUSED_MAX_CELLS && h->acou_fused != 0) {
-        int width = ACOU_CHUNK_W;
+        int width = h->acou_fused > 1 ? h->acou_fused : ACOU_CHUNK_W;
         if (h->acou_fused < 0) {
             auto it = h->acou_tuned.find({n, (int)B.slots.size(), h->per_launch});
             if (it == h->acou_tuned.end()) {
@@ -1087,8 +1087,8 @@ std::vector<BatchShape> launch_groups(const ds2d_ctx* h, const Batch& B) {
 void acou_fwd(ds2d_ctx* h, const Batch& B, AcouWS& w, bool do_bg, bool do_born, bool rec_bg);
 
 // Autotuner of the resident-history Born pass: time ACOU_TUNE_STEPS steps of
-// the per-cell kernels and of the fused step at x widths 16, 32 and 64, keep
-// the fastest for this batch geometry (cached in the handle).
+// the per-cell kernels and of the fused step at several x widths, keep the
+// fastest for this batch geometry (cached in the handle).
 void acou_tune(ds2d_ctx* h, const Batch& B, AcouWS& w) {
     hipStream_t st = h->st;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1117,9 +1117,34 @@ void acou_tune(ds2d_ctx* h, const Batch& B, AcouWS& w) {
         if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return 1e30f;
         return ms;
     };
+    // widths 16, 32, 64 and those whose chunk count just fills 1, 2 or 3 rounds of one workgroup per CU
+    std::vector<int> widths = {16, 32, 64};
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0) == hipSuccess && ncu > 0) {
+        const int rows = ACOU_THREADS - 16;
+        int max_nxp = 16;
+        for (const SlotDesc& S : B.slots) max_nxp = std::max(max_nxp, S.nxp);
+        auto nchunks = [&](int W) {
+            long long k = 0;
+            for (const SlotDesc& S : B.slots) k += (long long)((S.nxp + W - 1) / W) * ((S.nzp + rows - 1) / rows);
+            return k;
+        };
+        for (int R = 1; R <= 3; ++R) {
+            int lo = 8, hi = max_nxp;  // smallest W with nchunks(W) <= R * ncu
+            if (nchunks(hi) > (long long)R * ncu) continue;
+            while (lo < hi) {
+                const int mid = (lo + hi) / 2;
+                if (nchunks(mid) <= (long long)R * ncu)
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+            if (std::find(widths.begin(), widths.end(), lo) == widths.end()) widths.push_back(lo);
+        }
+    }
     int best = 0;
     float bt = time_pass(0);
-    for (int width : {16, 32, 64}) {
+    for (int width : widths) {
         const float t = time_pass(width);
         if (t < bt) {
             bt = t;
@@ -1368,7 +1393,8 @@ int ds2d_lsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_
 
 int ds2d_set_fused(ds2d_t* h, int mode) {
     if (!h) return fail(DS2D_EINVAL, "null handle");
-    if (mode < -1 || mode > 1) return fail(DS2D_EINVAL, "mode must be -1, 0 or 1");
+    if (mode < -1 || (mode > 1 && mode < 8) || mode > 4096)
+        return fail(DS2D_EINVAL, "mode must be -1, 0, 1 or an x width in [8, 4096], got %d", mode);
     h->acou_fused = mode;
     return 0;
 }

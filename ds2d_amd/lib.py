@@ -331,7 +331,8 @@ class Solver:
         return pp.reshape(self.allnx, self.allnz), ps.reshape(self.allnx, self.allnz), res, st.as_dict()
 
     def set_fused(self, mode):
-        """Fused Born/adjoint steps of the resident-history passes: -1 auto, 0 never, 1 always."""
+        """Fused Born/adjoint steps of the resident-history passes: -1 auto (timed), 0 never,
+        1 always (x width 64), k in [8, 4096] always with x width k."""
         _check(self.lib.ds2d_set_fused(self.h, int(mode)))
 
     def set_batch(self, shots_per_launch):

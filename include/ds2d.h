@@ -248,8 +248,10 @@ int ds2d_elsrtm_learned(ds2d_t* h, ds2d_unet_t* u, int nshots, const ds2d_shot* 
  * per-rank shot loop does).  Applies to the plain Born/adjoint/CG entries. */
 int ds2d_set_batch(ds2d_t* h, int shots_per_launch);
 /* Resident-history Born and adjoint passes as one fused launch per step
- * (x-marching workgroups): -1 = auto (when the batch gives every CU several
- * x-chunks; default), 0 = never (per-cell kernels), 1 = always. */
+ * (x-marching workgroups): -1 = auto (default: the per-cell kernels and the
+ * fused step at several x widths are timed once per batch geometry), 0 =
+ * never (per-cell kernels), 1 = always (x width 64), k in [8, 4096] = always,
+ * with x width k. */
 int ds2d_set_fused(ds2d_t* h, int mode);
 
 /* Host-side nonlinear CG update of the reference (sjcgsolver, main.cpp:1232-1303):

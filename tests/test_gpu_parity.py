@@ -189,7 +189,7 @@ def _acou_case(nshots=3, nt=300):
     return allnx, allnz, pml, nt, vp, rho, refl, wav, shots
 
 
-@pytest.mark.parametrize("bg_mode,fused", [(1, 0), (1, 1), (2, 0)])
+@pytest.mark.parametrize("bg_mode,fused", [(1, 0), (1, 1), (1, 37), (2, 0)])
 def test_born_forward_and_adjoint(bg_mode, fused):
     """bg_mode 1: background increments resident in HBM (per-cell or fused
     adjoint kernels); 2: background rebuilt by reversal through the random
