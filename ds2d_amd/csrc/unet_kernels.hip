@@ -291,8 +291,8 @@ void launch_conv(hipStream_t st, const ConvArgs& a) {
         launch_conv_lds<2, 2>(st, a, npix);
         return;
     }
-    if (a.cout % 64 != 0 && conv_lds_ok(a, 512)) {
-        launch_conv_lds<4, 1>(st, a, npix);
+    if (a.cout % 64 != 0 && conv_lds_ok(a, 256)) {
+        launch_conv_lds<2, 1>(st, a, npix);
         return;
     }
     if (a.cout % 64 == 0) {
