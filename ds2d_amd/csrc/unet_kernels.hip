@@ -565,7 +565,49 @@ __global__ __launch_bounds__(256) void k_reduce_cols(const float* part, int nchu
     if (threadIdx.x == 0) out[e] = accumulate ? out[e] + red[0] : red[0];
 }
 
+// The same two levels on 4 consecutive elements per thread (len a multiple of 4,
+// 16-byte aligned rows and output), with the chunk grouping of the scalar form,
+// so every element is summed in the same order.
+__global__ void k_reduce4(const float4* part, int nchunk, long long len4, float4* out, int accumulate) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= len4) return;
+    float4 s = {0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < nchunk; ++c) {
+        const float4 v = part[(long long)c * len4 + e];
+        s.x += v.x, s.y += v.y, s.z += v.z, s.w += v.w;
+    }
+    if (accumulate) {
+        const float4 o = out[e];
+        s = float4{o.x + s.x, o.y + s.y, o.z + s.z, o.w + s.w};
+    }
+    out[e] = s;
+}
+
+__global__ void k_reduce_strided4(float4* part, int nchunk, int G, long long len4) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = blockIdx.y;
+    if (e >= len4) return;
+    float4 s = {0.f, 0.f, 0.f, 0.f};
+    for (int c = g; c < nchunk; c += G) {
+        const float4 v = part[(long long)c * len4 + e];
+        s.x += v.x, s.y += v.y, s.z += v.z, s.w += v.w;
+    }
+    part[(long long)g * len4 + e] = s;
+}
+
 void launch_reduce(hipStream_t st, float* part, int nchunk, long long len, float* out, int accumulate) {
+    if (len % 4 == 0 && len > 4096 && ((uintptr_t)part & 15) == 0 && ((uintptr_t)out & 15) == 0) {
+        const long long eb = (len + 255) / 256, len4 = len / 4, eb4 = (len4 + 255) / 256;
+        const int G = (int)std::min<long long>(nchunk, std::max<long long>(1, (2048 + eb - 1) / eb));
+        if (G > 1 && nchunk >= 4 * G) {
+            hipLaunchKernelGGL(k_reduce_strided4, dim3((unsigned)eb4, G), dim3(256), 0, st,
+                               reinterpret_cast<float4*>(part), nchunk, G, len4);
+            nchunk = G;
+        }
+        hipLaunchKernelGGL(k_reduce4, dim3((unsigned)eb4), dim3(256), 0, st, reinterpret_cast<const float4*>(part),
+                           nchunk, len4, reinterpret_cast<float4*>(out), accumulate);
+        return;
+    }
     if (len <= 4096 && nchunk >= 64) {
         hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)len), dim3(256), 0, st, part, nchunk, len, out, accumulate);
         return;
