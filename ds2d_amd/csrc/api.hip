@@ -82,6 +82,7 @@ struct ds2d_ctx {
     int per_launch = 0; // plain Born/adjoint: shots per time loop (0: all)
     int fwd_two = 0;    // autotuned: mode 2 runs the two-kernel step instead of the fused one
     int acou_fused = -1;  // fused Born/adjoint steps: -1 auto, 0 never, 1 whenever eligible
+    int src_fp64 = 1;     // reference-compat gradient: background and first-order Born field in FP64
     // (cells, slots, shots per launch) -> 0 per-cell kernels, else fused x width
     std::map<std::tuple<long long, int, int>, int> acou_tuned;
 
@@ -223,15 +224,16 @@ int build_batch(ds2d_ctx* h, int nshots, const ds2d_shot* shots, Batch& B) {
 }
 
 // Allocate (grow-only) and zero one 15-array elastic wavefield.
-int alloc_efield(ds2d_ctx* h, const char* tag, long long n, EField& f, bool zero = true) {
-    float** ptrs[15] = {&f.vx, &f.vz, &f.txx, &f.tzz, &f.txz, &f.vx_x, &f.vx_z, &f.vz_x, &f.vz_z,
-                        &f.txx_x, &f.txx_z, &f.tzz_x, &f.tzz_z, &f.txz_x, &f.txz_z};
-    float* base;
+template <typename T>
+int alloc_efield(ds2d_ctx* h, const char* tag, long long n, EFieldT<T>& f, bool zero = true) {
+    T** ptrs[15] = {&f.vx, &f.vz, &f.txx, &f.tzz, &f.txz, &f.vx_x, &f.vx_z, &f.vz_x, &f.vz_z,
+                    &f.txx_x, &f.txx_z, &f.tzz_x, &f.tzz_z, &f.txz_x, &f.txz_z};
+    T* base;
     std::string name = std::string("ef_") + tag;
     int rc = h->fbuf(name.c_str(), (size_t)n * 15, &base);
     if (rc) return rc;
     for (int k = 0; k < 15; ++k) *ptrs[k] = base + (size_t)k * n;
-    if (zero) HIP_TRY(hipMemsetAsync(base, 0, (size_t)n * 15 * sizeof(float), h->st));
+    if (zero) HIP_TRY(hipMemsetAsync(base, 0, (size_t)n * 15 * sizeof(T), h->st));
     return 0;
 }
 
@@ -472,22 +474,39 @@ int ds2d_set_shot(ds2d_t* h, const ds2d_shot* s, const float* rec_z, const float
 // ---------------------------------------------------------------------------
 // Reference ELSRTM gradient, batched (FD2DGPU.cu:4184-4879 per slot)
 // ---------------------------------------------------------------------------
-int ds2d_batch_gradient(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* obs_z, const float* obs_x,
-                        int iter, double* misfit) {
+}  // extern "C"
+
+namespace {
+
+// TB is the precision of the two source-side wavefields (the background and
+// the first-order Born field), whose velocities drive a Born source: in FP32
+// their rounding noise is differentiated by add_born_source and amplified in
+// the Born fields and images (relative L2 up to 3e-4 against FP64 after a few
+// hundred steps), in FP64 the images stay within 1e-5 of FP64.  The
+// second-order Born field, the receiver-side fields, the strains, records and
+// images are FP32.
+//
+// born_order > 0 runs the forward part only (born_modeling mode 1, plus mode 5
+// when born_order = 2) and returns the scatter records: the first-order Born
+// records Lm (order 1, FD2DGPU.cu:3460-3467 without the second field) or
+// Lm + Lm2 (order 2, scatter_add of FD2DGPU.cu:3147-3152) in out_z / out_x.
+template <typename TB>
+int batch_gradient_impl(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* obs_z, const float* obs_x,
+                        int iter, double* misfit, int born_order = 0, float* out_z = nullptr,
+                        float* out_x = nullptr) {
     int rc;
-    if ((rc = check_h(h))) return rc;
-    if (!h->have_model) return fail(DS2D_ESTATE, "no model");
-    if (h->g.physics != 0) return fail(DS2D_EINVAL, "the reference gradient is elastic");
-    if (!obs_z || !obs_x) return fail(DS2D_EINVAL, "observed records required");
     Batch B;
     if ((rc = build_batch(h, nshots, shots, B))) return rc;
     const BatchShape& sh = B.shape;
     const long long n = B.total_cells;
     hipStream_t st = h->st;
-    EField bg, b, b2, bk, bb;
+    EFieldT<TB> bg, b;
+    EField b2, bk, bb;
     Med m;
     Strain sg, sb;
-    if ((rc = alloc_efield(h, "bg", n, bg)) || (rc = alloc_efield(h, "b", n, b)) ||
+    const char* tg = sizeof(TB) == 8 ? "bg64" : "bg";
+    const char* tb = sizeof(TB) == 8 ? "b64" : "b";
+    if ((rc = alloc_efield(h, tg, n, bg)) || (rc = alloc_efield(h, tb, n, b)) ||
         (rc = alloc_efield(h, "b2", n, b2)) || (rc = alloc_efield(h, "bk", n, bk)) ||
         (rc = alloc_efield(h, "bb", n, bb)) || (rc = alloc_med(h, n, m)))
         return rc;
@@ -501,16 +520,17 @@ int ds2d_batch_gradient(ds2d_t* h, int nshots, const ds2d_shot* shots, const flo
     if ((rc = h->fbuf("recs", R * 4, &rs)) || (rc = h->fbuf("obs", R * 2, &obs))) return rc;
     float *rs_z = rs, *rs_x = rs + R, *rs2_z = rs + 2 * R, *rs2_x = rs + 3 * R, *ob_z = obs, *ob_x = obs + R;
     HIP_TRY(hipMemsetAsync(rs, 0, R * 4 * 4, st));
-    HIP_TRY(hipMemcpyAsync(ob_z, obs_z, B.total_rec * 4, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(ob_x, obs_x, B.total_rec * 4, hipMemcpyHostToDevice, st));
+    if (obs_z) HIP_TRY(hipMemcpyAsync(ob_z, obs_z, B.total_rec * 4, hipMemcpyHostToDevice, st));
+    if (obs_x) HIP_TRY(hipMemcpyAsync(ob_x, obs_x, B.total_rec * 4, hipMemcpyHostToDevice, st));
+    const bool full = born_order == 0, second = born_order != 1, edges = full && B.any_pml;
     const long long W = std::max<long long>(B.total_win, 1);
     float* win;
     if ((rc = h->fbuf("win7", (size_t)W * 7, &win))) return rc;
     HIP_TRY(hipMemsetAsync(win, 0, (size_t)W * 7 * 4, st));
-    float *e_bvx = nullptr, *e_bvz = nullptr, *e_cvx = nullptr, *e_cvz = nullptr;
-    if (B.any_pml) {
-        float* e;
-        if ((rc = h->fbuf("edges", (size_t)B.total_edge * 4, &e))) return rc;
+    TB *e_bvx = nullptr, *e_bvz = nullptr, *e_cvx = nullptr, *e_cvz = nullptr;
+    if (edges) {
+        TB* e;
+        if ((rc = h->fbuf(sizeof(TB) == 8 ? "edges64" : "edges", (size_t)B.total_edge * 4, &e))) return rc;
         e_bvx = e; e_bvz = e + B.total_edge; e_cvx = e + 2 * B.total_edge; e_cvz = e + 3 * B.total_edge;
     }
     launch_setup(st, sh, setup_args(h, true), m);
@@ -522,18 +542,33 @@ int ds2d_batch_gradient(ds2d_t* h, int nshots, const ds2d_shot* shots, const flo
         launch_born_src(st, sh, bg.vx, bg.vz, b, m, 1.f);
         launch_ref_v(st, sh, b, m, 1.f, 0);
         launch_ref_s(st, sh, b, m, sg, 1.f, 0);
-        launch_born_src(st, sh, b.vx, b.vz, b2, m, 1.f);
-        launch_ref_v(st, sh, b2, m, 1.f, 0);
-        launch_ref_s(st, sh, b2, m, sg, 1.f, 0);
-        launch_record(st, sh, b.vz, b.vx, rs_z, rs_x, b2.vz, b2.vx, rs2_z, rs2_x, it);
-        if (B.any_pml) {
+        if (second) {
+            launch_born_src(st, sh, b.vx, b.vz, b2, m, 1.f);
+            launch_ref_v(st, sh, b2, m, 1.f, 0);
+            launch_ref_s(st, sh, b2, m, sg, 1.f, 0);
+            launch_record(st, sh, b.vz, b.vx, rs_z, rs_x, b2.vz, b2.vx, rs2_z, rs2_x, it);
+        } else {
+            launch_record(st, sh, b.vz, b.vx, rs_z, rs_x, (const float*)nullptr, nullptr, nullptr, nullptr, it);
+        }
+        if (edges) {
             launch_edge_io(st, sh, bg.vx, bg.vz, e_bvx, e_bvz, it, 1);
             launch_edge_io(st, sh, b.vx, b.vz, e_cvx, e_cvz, it, 1);
         }
     }
     // scatter_add (FD2DGPU.cu:3143-3149): s = 1*s + s2
-    launch_axpby(st, rs_z, rs2_z, B.total_rec, 1.f, 1.f);
-    launch_axpby(st, rs_x, rs2_x, B.total_rec, 1.f, 1.f);
+    if (second) {
+        launch_axpby(st, rs_z, rs2_z, B.total_rec, 1.f, 1.f);
+        launch_axpby(st, rs_x, rs2_x, B.total_rec, 1.f, 1.f);
+    }
+    if (!full) {
+        if (B.total_rec > 0) {
+            HIP_TRY(hipMemcpyAsync(out_z, rs_z, B.total_rec * 4, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(out_x, rs_x, B.total_rec * 4, hipMemcpyDeviceToHost, st));
+        }
+        HIP_TRY(hipStreamSynchronize(st));
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     // NCC (FD2DGPU.cu:4532-4673) on the device, misfit finished on the host
     double* sums;
     if ((rc = h->fbuf("nccsum", (size_t)std::max(B.total_traces, 1) * 6, &sums))) return rc;
@@ -577,6 +612,79 @@ int ds2d_batch_gradient(ds2d_t* h, int nshots, const ds2d_shot* shots, const flo
     launch_gather(st, sh, win, W, h->d_grad7, A, 7, h->g.allnx, h->g.allnz);
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ds2d_batch_gradient(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* obs_z, const float* obs_x,
+                        int iter, double* misfit) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!h->have_model) return fail(DS2D_ESTATE, "no model");
+    if (h->g.physics != 0) return fail(DS2D_EINVAL, "the reference gradient is elastic");
+    if (!obs_z || !obs_x) return fail(DS2D_EINVAL, "observed records required");
+    return h->src_fp64 ? batch_gradient_impl<double>(h, nshots, shots, obs_z, obs_x, iter, misfit)
+                       : batch_gradient_impl<float>(h, nshots, shots, obs_z, obs_x, iter, misfit);
+}
+
+int ds2d_born(ds2d_t* h, int nshots, const ds2d_shot* shots, int order, float* rec_z, float* rec_x) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!h->have_model) return fail(DS2D_ESTATE, "no model");
+    if (h->g.physics != 0) return fail(DS2D_EINVAL, "the reference Born modelling is elastic");
+    if (order != 1 && order != 2) return fail(DS2D_EINVAL, "order must be 1 (Lm) or 2 (Lm + Lm2)");
+    if (!rec_z || !rec_x) return fail(DS2D_EINVAL, "output records required");
+    return h->src_fp64 ? batch_gradient_impl<double>(h, nshots, shots, nullptr, nullptr, 0, nullptr, order, rec_z, rec_x)
+                       : batch_gradient_impl<float>(h, nshots, shots, nullptr, nullptr, 0, nullptr, order, rec_z, rec_x);
+}
+
+// ELSRTM_CALCULATE_ALPHA (FD2DGPU.cu:4885-5035): r = Lm - d with the handle's
+// image m (first-order Born, born_modeling mode 1, then GPU_subtract with
+// alpha 1), q = Lg with the gradient images g; num += sum q r, den += sum q q
+// over every sample of both components (FP64 sums, as fenzi / fenmu).
+int ds2d_calculate_alpha(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* g_pp, const float* g_ps,
+                         const float* obs_z, const float* obs_x, double* num, double* den) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!g_pp || !g_ps || !obs_z || !obs_x || !num || !den) return fail(DS2D_EINVAL, "null argument");
+    long long R = 0;
+    for (int i = 0; i < nshots; ++i) R += (long long)shots[i].ntr * h->g.nt;
+    std::vector<float> rz(R), rx(R), qz(R), qx(R);
+    const size_t A = (size_t)h->g.allnx * h->g.allnz;
+    std::vector<float> keep_pp(A, 0.f), keep_ps(A, 0.f);
+    const bool had_image = h->have_image;
+    if (had_image) {
+        HIP_TRY(hipMemcpyAsync(keep_pp.data(), h->d_ipp, A * 4, hipMemcpyDeviceToHost, h->st));
+        HIP_TRY(hipMemcpyAsync(keep_ps.data(), h->d_ips, A * 4, hipMemcpyDeviceToHost, h->st));
+        HIP_TRY(hipStreamSynchronize(h->st));
+    }
+    if ((rc = ds2d_born(h, nshots, shots, 1, rz.data(), rx.data()))) return rc;
+    if ((rc = ds2d_set_image(h, g_pp, g_ps))) return rc;
+    rc = ds2d_born(h, nshots, shots, 1, qz.data(), qx.data());
+    // restore the handle's image (m) whatever the outcome
+    int rc2 = had_image ? ds2d_set_image(h, keep_pp.data(), keep_ps.data()) : ds2d_set_image(h, nullptr, nullptr);
+    h->have_image = had_image;
+    if (rc) return rc;
+    if (rc2) return rc2;
+    double a = 0, b = 0;
+    for (long long k = 0; k < R; ++k) {
+        const double ez = (double)rz[k] - obs_z[k], ex = (double)rx[k] - obs_x[k];
+        a += (double)qz[k] * ez + (double)qx[k] * ex;
+        b += (double)qz[k] * qz[k] + (double)qx[k] * qx[k];
+    }
+    *num += a;
+    *den += b;
+    return 0;
+}
+
+int ds2d_set_precision(ds2d_t* h, int source_fp64) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (source_fp64 != 0 && source_fp64 != 1) return fail(DS2D_EINVAL, "source_fp64 must be 0 or 1");
+    h->src_fp64 = source_fp64;
     return 0;
 }
 

@@ -34,11 +34,17 @@ struct SlotDesc {
 };
 
 // The reference's split-PML elastic wavefield (15 arrays, FD2DGPU.cuh:280-294).
-struct EField {
-    float *vx, *vz, *txx, *tzz, *txz;
-    float *vx_x, *vx_z, *vz_x, *vz_z;
-    float *txx_x, *txx_z, *tzz_x, *tzz_z, *txz_x, *txz_z;
+// The reference-compat gradient keeps the two source-side wavefields (the
+// background and the first-order Born field, whose velocities drive a Born
+// source) in FP64; every other wavefield is FP32.
+template <typename T>
+struct EFieldT {
+    T *vx, *vz, *txx, *tzz, *txz;
+    T *vx_x, *vx_z, *vz_x, *vz_z;
+    T *txx_x, *txx_z, *tzz_x, *tzz_z, *txz_x, *txz_z;
 };
+using EField = EFieldT<float>;
+using EFieldD = EFieldT<double>;
 
 // Strain outputs of the reconstruction stress kernels (vx_gx, vz_gz, vx_gz, vz_gx).
 struct Strain {

@@ -15,21 +15,36 @@ struct BatchShape {
 };
 
 // --- reference-compatible elastic kernels (kernels_ref.hip) ---------------
+// Overloads on EFieldD run the wavefield in FP64 (the source-side fields of
+// the reference-compat gradient); media, strains and records stay FP32.
 void launch_ref_v(hipStream_t st, const BatchShape& b, EField f, Med m, float dir, int recon);
+void launch_ref_v(hipStream_t st, const BatchShape& b, EFieldD f, Med m, float dir, int recon);
 void launch_ref_s(hipStream_t st, const BatchShape& b, EField f, Med m, Strain s, float dir, int recon);
+void launch_ref_s(hipStream_t st, const BatchShape& b, EFieldD f, Med m, Strain s, float dir, int recon);
 void launch_adj_v(hipStream_t st, const BatchShape& b, EField f, Med m, float dir);
 void launch_adj_s(hipStream_t st, const BatchShape& b, EField f, Med m, float dir);
 void launch_born_src(hipStream_t st, const BatchShape& b, const float* vx, const float* vz, EField tgt, Med m,
                      float dir);
+void launch_born_src(hipStream_t st, const BatchShape& b, const double* vx, const double* vz, EFieldD tgt, Med m,
+                     float dir);
+void launch_born_src(hipStream_t st, const BatchShape& b, const double* vx, const double* vz, EField tgt, Med m,
+                     float dir);
 void launch_born_adj(hipStream_t st, const BatchShape& b, EField src, EField tgt, Med m, float dir);
 void launch_inject_src(hipStream_t st, const BatchShape& b, float* txx, float* tzz, const float* wav, int it);
+void launch_inject_src(hipStream_t st, const BatchShape& b, double* txx, double* tzz, const float* wav, int it);
 void launch_record(hipStream_t st, const BatchShape& b, const float* vz, const float* vx, float* rz, float* rx,
+                   const float* vz2, const float* vx2, float* rz2, float* rx2, int it);
+void launch_record(hipStream_t st, const BatchShape& b, const double* vz, const double* vx, float* rz, float* rx,
                    const float* vz2, const float* vx2, float* rz2, float* rx2, int it);
 void launch_inject_res(hipStream_t st, const BatchShape& b, float* vz, float* vx, const float* rz, const float* rx,
                        int it);
 void launch_edge_io(hipStream_t st, const BatchShape& b, float* fa, float* fb, float* ea, float* eb, int it,
                     int save);
+void launch_edge_io(hipStream_t st, const BatchShape& b, double* fa, double* fb, double* ea, double* eb, int it,
+                    int save);
 void launch_image(hipStream_t st, const BatchShape& b, Strain sg, Strain sb, EField bk, EField bb, EField bg, Med m,
+                  float* w7, long long wstride);
+void launch_image(hipStream_t st, const BatchShape& b, Strain sg, Strain sb, EField bk, EField bb, EFieldD bg, Med m,
                   float* w7, long long wstride);
 
 // --- setup and reductions (kernels_setup.hip) ------------------------------

@@ -87,6 +87,9 @@ SIGNATURES = {
                                      C.POINTER(Stats)]),
     "ds2d_elsrtm_learned": (C.c_int, [_H, _H, C.c_int, C.POINTER(Shot), _F, _F, C.c_int, _F, _F, _D,
                                       C.POINTER(C.c_int), C.POINTER(Stats)]),
+    "ds2d_set_precision": (C.c_int, [_H, C.c_int]),
+    "ds2d_born": (C.c_int, [_H, C.c_int, C.POINTER(Shot), C.c_int, _F, _F]),
+    "ds2d_calculate_alpha": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, _F, _F, _D, _D]),
     "ds2d_set_batch": (C.c_int, [_H, C.c_int]),
     "ds2d_set_fused": (C.c_int, [_H, C.c_int]),
     "ds2d_unet_create": (C.c_int, [C.POINTER(UnetCfg), C.POINTER(_H)]),
@@ -259,6 +262,45 @@ class Solver:
         ox = np.ascontiguousarray(obs_x, np.float32)
         _check(self.lib.ds2d_batch_gradient(self.h, len(shots), arr, fptr(oz), fptr(ox), it, C.byref(mf)))
         return mf.value
+
+    def set_precision(self, source_fp64):
+        """Reference-compat gradient: background and first-order Born field in FP64 (1) or FP32 (0)."""
+        _check(self.lib.ds2d_set_precision(self.h, 1 if source_fp64 else 0))
+
+    def born(self, shots, order=2):
+        """Reference Born records of the handle's image: Lm (order 1) or Lm + Lm2 (order 2)."""
+        rz, rx = self._rec(shots), self._rec(shots)
+        _check(self.lib.ds2d_born(self.h, len(shots), _shots_array(shots), int(order), fptr(rz), fptr(rx)))
+        return rz, rx
+
+    def calculate_alpha(self, shots, g_pp, g_ps, obs_z, obs_x):
+        """ELSRTM_CALCULATE_ALPHA: (num, den) = (<Lg, Lm - d>, <Lg, Lg>) with m the handle's image."""
+        num, den = C.c_double(0.0), C.c_double(0.0)
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (g_pp, g_ps, obs_z, obs_x)]
+        _check(self.lib.ds2d_calculate_alpha(self.h, len(shots), _shots_array(shots), *[fptr(a) for a in arrs],
+                                             C.byref(num), C.byref(den)))
+        return num.value, den.value
+
+    def set_shot(self, shot, rec_z=None, rec_x=None):
+        """ds2d_set_shot: the current shot (main.cpp:994-1025) and its observed records."""
+        self._shot_keep = _shots_array([shot])
+        rz = None if rec_z is None else np.ascontiguousarray(rec_z, np.float32)
+        rx = None if rec_x is None else np.ascontiguousarray(rec_x, np.float32)
+        _check(self.lib.ds2d_set_shot(self.h, self._shot_keep, fptr(rz), fptr(rx)))
+        self._cur_ntr = shot.ntr
+
+    def shot_gradient(self, it):
+        """ds2d_shot_gradient on the current shot; returns the NCC misfit increment."""
+        mf = C.c_double(0.0)
+        _check(self.lib.ds2d_shot_gradient(self.h, int(it), C.byref(mf)))
+        return mf.value
+
+    def forward(self):
+        """ds2d_forward: elastic forward records (vz, vx) of the current shot."""
+        rz = np.zeros(self._cur_ntr * self.nt, np.float32)
+        rx = np.zeros_like(rz)
+        _check(self.lib.ds2d_forward(self.h, fptr(rz), fptr(rx)))
+        return rz, rx
 
     def take_gradients(self):
         A = self.allnx * self.allnz

@@ -118,6 +118,24 @@ int ds2d_batch_gradient(ds2d_t* h, int nshots, const ds2d_shot* shots, const flo
                         const float* obs_x, int iter, double* misfit);
 int ds2d_take_gradients(ds2d_t* h, float* pp, float* ps, float* illum, float* b_pp, float* b_ps,
                         float* b2_pp, float* b2_ps);
+/* Precision of the reference-compat gradient's two source-side wavefields
+ * (the background and the first-order Born field, whose velocities drive
+ * add_born_source): 1 = FP64 (default), 0 = FP32 as the reference.  Every
+ * other wavefield, the records and the images are FP32 either way.  No
+ * reference counterpart (the reference computes all fields in FP32). */
+int ds2d_set_precision(ds2d_t* h, int source_fp64);
+/* Reference Born modelling alone, batched: order 1 returns the first-order
+ * scatter records Lm (born_modeling mode 1 + record_scatter, FD2DGPU.cu:1820-1830,
+ * 3460-3467), order 2 returns Lm + Lm2 (modes 0/3/4/5 + scatter_add,
+ * FD2DGPU.cu:1834-1860, 3147-3152) -- the records the gradient's NCC consumes.
+ * The image is the handle's (ds2d_set_image); records concatenated in shot order. */
+int ds2d_born(ds2d_t* h, int nshots, const ds2d_shot* shots, int order, float* rec_z, float* rec_x);
+/* ELSRTM_CALCULATE_ALPHA (FD2DGPU.cu:4885-5035): with m the handle's image and
+ * g the given gradient images, r = Lm - d and q = Lg (first-order Born);
+ * *num += <q, r>, *den += <q, q> in FP64 over all shots (fenzi, fenmu).  The
+ * handle's image is left as it was. */
+int ds2d_calculate_alpha(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* g_pp, const float* g_ps,
+                         const float* obs_z, const float* obs_x, double* num, double* den);
 
 /* Elastic forward modelling of the current shot: vz, vx at (rec_ix, igz). */
 int ds2d_forward(ds2d_t* h, float* rec_z, float* rec_x);

@@ -235,6 +235,145 @@ void elsrtm_shot(const orc_shot* s, const R* vp, const R* vs, const R* rho, cons
     }
 }
 
+// Born modelling alone (ELSRTM_CALCULATE_ALPHA's forward loop, FD2DGPU.cu:
+// 4915-4921: born_modeling mode 1 + record_scatter): first-order scatter
+// records Lm for order 1; for order 2 the schedule of ELSRTM_ROT_EXP_IMAGE_
+// SINGLESHOT's forward loop (modes 0/3/4/5) and scatter_add, Lm + Lm2.
+template <typename R>
+void elsrtm_born(const orc_shot* s, const R* vp, const R* vs, const R* rho, const R* wav, const R* img_pp,
+                 const R* img_ps, int col0, int order, R* rz, R* rxx) {
+    Setup<R> S = make_setup<R>(s, vp, vs, rho);
+    const Grid<R>& g = S.g;
+    const size_t n = g.n();
+    const int nt = s->nt, ntr = s->ntr;
+    std::vector<R> dmp(n, 0), dms(n, 0);
+    pad_window(img_pp, s->allnz, col0, s->nx, S.nz, s->pml, dmp.data());
+    pad_window(img_ps, s->allnz, col0, s->nx, S.nz, s->pml, dms.data());
+    Elastic<R> bg, b, b2;
+    bg.init(n); b.init(n); b2.init(n);
+    std::vector<R> r2z((size_t)ntr * nt, 0), r2x(r2z.size(), 0);
+    const size_t src = g.at(s->isx, s->isz);
+    for (int it = 0; it < nt; ++it) {
+        bg.txx[src] += wav[it];
+        bg.tzz[src] += wav[it];
+        update_v(g, bg, S.m, 1);
+        update_s(g, bg, S.m, 1);
+        add_born_source(g, bg.vx, bg.vz, b, dmp, dms, S.m, 1);
+        update_v(g, b, S.m, 1);
+        update_s(g, b, S.m, 1);
+        if (order == 2) {
+            add_born_source(g, b.vx, b.vz, b2, dmp, dms, S.m, 1);
+            update_v(g, b2, S.m, 1);
+            update_s(g, b2, S.m, 1);
+            record_pair(g, s, b2.vz, b2.vx, it, r2z.data(), r2x.data());
+        }
+        record_pair(g, s, b.vz, b.vx, it, rz, rxx);
+    }
+    if (order == 2)
+        for (size_t k = 0; k < r2z.size(); ++k) {
+            rz[k] = (R)1 * rz[k] + r2z[k];
+            rxx[k] = (R)1 * rxx[k] + r2x[k];
+        }
+}
+
+// Single kernels on caller-provided padded arrays (15-array elastic state in
+// the order vx, vz, txx, tzz, txz, vx_x, vx_z, vz_x, vz_z, txx_x, txx_z,
+// tzz_x, tzz_z, txz_x, txz_z), for the pin tests.
+template <typename R>
+void load15(Elastic<R>& f, const R* a, size_t n) {
+    f.init(n);
+    auto all = f.all();
+    for (size_t k = 0; k < all.size(); ++k) std::copy(a + k * n, a + (k + 1) * n, all[k]->begin());
+}
+template <typename R>
+void store15(Elastic<R>& f, R* a, size_t n) {
+    auto all = f.all();
+    for (size_t k = 0; k < all.size(); ++k) std::copy(all[k]->begin(), all[k]->end(), a + k * n);
+}
+template <typename R>
+Medium<R> medium_of(size_t n, const R* vp, const R* vs, const R* rho, const R* lam, const R* mu, const R* dampx,
+                    const R* dampz) {
+    Medium<R> m;
+    auto cp = [&](std::vector<R>& v, const R* a) { if (a) v.assign(a, a + n); else v.assign(n, 0); };
+    cp(m.vp, vp); cp(m.vs, vs); cp(m.rho, rho); cp(m.lam, lam); cp(m.mu, mu); cp(m.dampx, dampx); cp(m.dampz, dampz);
+    return m;
+}
+// which: 0 update_v, 1 update_s, 2 recon_v, 3 recon_s (strain4 out), 4 adjoint_v,
+// 5 adjoint_s, 6 add_born_source (velocities of src15 into the stress sums of
+// f15), 7 add_born_source_adjoint (stresses of src15 into the velocity sums of f15).
+template <typename R>
+void kernel(int which, int nxp, int nzp, int pml, R dx, R dz, R dt, const R* vp, const R* vs, const R* rho,
+            const R* lam, const R* mu, const R* dampx, const R* dampz, const R* dmp, const R* dms, int dir,
+            const R* src15, R* f15, R* strain4) {
+    Grid<R> g{nxp, nzp, pml, dx, dz, dt};
+    const size_t n = g.n();
+    Medium<R> m = medium_of<R>(n, vp, vs, rho, lam, mu, dampx, dampz);
+    Elastic<R> f;
+    load15(f, f15, n);
+    std::vector<R> Dp(n, 0), Ds(n, 0);
+    if (dmp) Dp.assign(dmp, dmp + n);
+    if (dms) Ds.assign(dms, dms + n);
+    Strain<R> st;
+    st.init(n);
+    Elastic<R> sf;
+    if (src15) load15(sf, src15, n);
+    switch (which) {
+        case 0: update_v(g, f, m, dir); break;
+        case 1: update_s(g, f, m, dir); break;
+        case 2: recon_v(g, f, m, dir); break;
+        case 3: recon_s(g, f, m, dir, st); break;
+        case 4: adjoint_v(g, f, m, dir); break;
+        case 5: adjoint_s(g, f, m, dir); break;
+        case 6: add_born_source(g, sf.vx, sf.vz, f, Dp, Ds, m, dir); break;
+        case 7: add_born_source_adjoint(g, sf, f.vx, f.vz, Dp, Ds, m, dir); break;
+        default: break;
+    }
+    store15(f, f15, n);
+    if (strain4) {
+        std::copy(st.gx_vx.begin(), st.gx_vx.end(), strain4);
+        std::copy(st.gz_vz.begin(), st.gz_vz.end(), strain4 + n);
+        std::copy(st.gz_vx.begin(), st.gz_vx.end(), strain4 + 2 * n);
+        std::copy(st.gx_vz.begin(), st.gx_vz.end(), strain4 + 3 * n);
+    }
+}
+// edge_io on one padded field; bands [nt][up | down | left | right] per field.
+template <typename R>
+void edge_io_x(int nxp, int nzp, int pml, int nt, R* field, R* bands, int it, int save) {
+    Grid<R> g{nxp, nzp, pml, 1, 1, 1};
+    const int nx = nxp - 2 * pml, nz = nzp - 2 * pml;
+    EdgeBands<R> e;
+    e.init(nt, nx, nz, NOP);
+    const size_t nu = e.up.size(), nl = e.left.size();
+    std::copy(bands, bands + nu, e.up.begin());
+    std::copy(bands + nu, bands + 2 * nu, e.down.begin());
+    std::copy(bands + 2 * nu, bands + 2 * nu + nl, e.left.begin());
+    std::copy(bands + 2 * nu + nl, bands + 2 * nu + 2 * nl, e.right.begin());
+    std::vector<R> f(field, field + g.n());
+    edge_io(g, f, e, it, save != 0);
+    std::copy(f.begin(), f.end(), field);
+    std::copy(e.up.begin(), e.up.end(), bands);
+    std::copy(e.down.begin(), e.down.end(), bands + nu);
+    std::copy(e.left.begin(), e.left.end(), bands + 2 * nu);
+    std::copy(e.right.begin(), e.right.end(), bands + 2 * nu + nl);
+}
+// image_gradient (+ image_illum when il) of one window into global images.
+template <typename R>
+void image_x(int nxp, int nzp, int pml, const R* strain4, const R* bk15, const R* vp, const R* vs, const R* rho,
+             int col0, int allnz, R* pp, R* ps, R* il) {
+    Grid<R> g{nxp, nzp, pml, 1, 1, 1};
+    const size_t n = g.n();
+    Strain<R> st;
+    st.gx_vx.assign(strain4, strain4 + n);
+    st.gz_vz.assign(strain4 + n, strain4 + 2 * n);
+    st.gz_vx.assign(strain4 + 2 * n, strain4 + 3 * n);
+    st.gx_vz.assign(strain4 + 3 * n, strain4 + 4 * n);
+    Elastic<R> bk;
+    load15(bk, bk15, n);
+    Medium<R> m = medium_of<R>(n, vp, vs, rho, nullptr, nullptr, nullptr, nullptr);
+    image_gradient(g, st, bk, m, col0, allnz, pp, ps);
+    if (il) image_illum(g, bk, col0, allnz, il);
+}
+
 // ---------------------------------------------------------------------------
 // Acoustic, reference layout (FD2DGPU.cu:253-340, 496-544): split p, single
 // damping array (dampX then dampZ into the same array, rr = 1e-6), rho = 1000,
@@ -242,6 +381,65 @@ void elsrtm_shot(const orc_shot* s, const R* vp, const R* vs, const R* rho, cons
 // velocity stagger is mirrored relative to the elastic kernels and vz uses
 // dt/dx (FD2DGPU.cu:318) — kept.
 // ---------------------------------------------------------------------------
+// FD2DGPU.cu:505-519: dampX then dampZ write the same array, so cells in both
+// strips (the corners) keep the z profile.
+template <typename R>
+void acoustic_ref_damp(const Grid<R>& g, const R* vt, R rr, R* damp) {
+    const int pml = g.pml;
+    const size_t n = g.n();
+    std::vector<R> dxa(n, 0), dza(n, 0);
+    damp_profile(g, vt, rr, dxa.data(), dza.data());
+    for (int ix = 0; ix < g.nxp; ++ix)
+        for (int iz = 0; iz < g.nzp; ++iz) {
+            size_t c = g.at(ix, iz);
+            damp[c] = 0;
+            if (ix < pml || ix > g.nxp - pml) damp[c] = dxa[c];
+            if (iz < pml || iz > g.nzp - pml) damp[c] = dza[c];
+        }
+}
+
+// One step of FD2DGPU.cu:253-325: GPUcalculate_p on [nop-1, n-nop-1]^2, then
+// GPUcalculate_vx (ix in [nop, nxp-nop], iz in [nop, nzp-nop-1]) and
+// GPUcalculate_vz (ix in [nop, nxp-nop-1], iz in [nop, nzp-nop]); vz uses
+// dt/dx (FD2DGPU.cu:318).
+template <typename R>
+void acoustic_ref_step(const Grid<R>& g, const R* k, const R* rr, const R* damp, R* px, R* pz, R* p, R* vx,
+                       R* vz) {
+    const R dt = g.dt, dx = g.dx, dz = g.dz;
+    const int nxp = g.nxp, nzp = g.nzp;
+#pragma omp parallel for
+    for (int ix = NOP - 1; ix <= nxp - NOP - 1; ++ix)
+        for (int iz = NOP - 1; iz <= nzp - NOP - 1; ++iz) {
+            size_t c = g.at(ix, iz);
+            R d1 = 1 - dt * damp[c] / 2, d2 = 1 + dt * damp[c] / 2;
+            R tvx = 0, tvz = 0;
+            for (int i = 1; i <= NOP; ++i) {
+                tvx += (R)C8[i] * (vx[c + (size_t)i * nzp] - vx[c - (size_t)(i - 1) * nzp]);
+                tvz += (R)C8[i] * (vz[c + i] - vz[c - i + 1]);
+            }
+            px[c] = (d1 * px[c] - k[c] * (dt / dx) * tvx) / d2;
+            pz[c] = (d1 * pz[c] - k[c] * (dt / dz) * tvz) / d2;
+            p[c] = px[c] + pz[c];
+        }
+#pragma omp parallel for
+    for (int ix = NOP; ix <= nxp - NOP; ++ix)
+        for (int iz = NOP; iz <= nzp - NOP; ++iz) {
+            size_t c = g.at(ix, iz);
+            R d1 = 1 - dt * damp[c] / 2, d2 = 1 + dt * damp[c] / 2;
+            if (iz <= nzp - NOP - 1) {  // GPUcalculate_vx bounds
+                R tp = 0;
+                for (int i = 1; i <= NOP; ++i)
+                    tp += (R)C8[i] * (p[c + (size_t)(i - 1) * nzp] - p[c - (size_t)i * nzp]);
+                vx[c] = (d1 * vx[c] - ((R)1 / rr[c]) * (dt / dx) * tp) / d2;
+            }
+            if (ix <= nxp - NOP - 1) {  // GPUcalculate_vz bounds
+                R tp = 0;
+                for (int i = 1; i <= NOP; ++i) tp += (R)C8[i] * (p[c + i - 1] - p[c - i]);
+                vz[c] = (d1 * vz[c] - ((R)1 / rr[c]) * (dt / dx) * tp) / d2;
+            }
+        }
+}
+
 template <typename R>
 void acoustic_ref_forward(const orc_shot* s, const R* vel, const R* rho, const R* wav, int sxnum, R* rec_p) {
     const int pml = s->pml, nx = s->nx, nz = s->allnz;
@@ -251,55 +449,13 @@ void acoustic_ref_forward(const orc_shot* s, const R* vel, const R* rho, const R
     pad_window(vel, s->allnz, s->ngx_min, nx, nz, pml, vt.data());
     std::vector<R> rhop(n, 0);
     if (rho) pad_window(rho, s->allnz, s->ngx_min, nx, nz, pml, rhop.data());
-    // FD2DGPU.cu:505-519: dampX then dampZ write the same array.
-    {
-        std::vector<R> dxa(n, 0), dza(n, 0);
-        damp_profile(g, vt.data(), (R)s->rr, dxa.data(), dza.data());
-        for (int ix = 0; ix < g.nxp; ++ix)
-            for (int iz = 0; iz < g.nzp; ++iz) {
-                size_t c = g.at(ix, iz);
-                if (ix < pml || ix > g.nxp - pml) damp[c] = dxa[c];
-                if (iz < pml || iz > g.nzp - pml) damp[c] = dza[c];
-            }
-    }
+    acoustic_ref_damp(g, vt.data(), (R)s->rr, damp.data());
     for (size_t c = 0; c < n; ++c) {  // FD2DGPU.cu:496-502
         rr[c] = s->rho_const > 0 ? (R)s->rho_const : rhop[c];
         k[c] = vt[c] * vt[c] * rr[c];
     }
-    const R dt = g.dt, dx = g.dx, dz = g.dz;
-    const int nxp = g.nxp, nzp = g.nzp;
     for (int it = 0; it < s->nt; ++it) {
-#pragma omp parallel for
-        for (int ix = NOP - 1; ix <= nxp - NOP - 1; ++ix)
-            for (int iz = NOP - 1; iz <= nzp - NOP - 1; ++iz) {
-                size_t c = g.at(ix, iz);
-                R d1 = 1 - dt * damp[c] / 2, d2 = 1 + dt * damp[c] / 2;
-                R tvx = 0, tvz = 0;
-                for (int i = 1; i <= NOP; ++i) {
-                    tvx += (R)C8[i] * (vx[c + (size_t)i * nzp] - vx[c - (size_t)(i - 1) * nzp]);
-                    tvz += (R)C8[i] * (vz[c + i] - vz[c - i + 1]);
-                }
-                px[c] = (d1 * px[c] - k[c] * (dt / dx) * tvx) / d2;
-                pz[c] = (d1 * pz[c] - k[c] * (dt / dz) * tvz) / d2;
-                p[c] = px[c] + pz[c];
-            }
-#pragma omp parallel for
-        for (int ix = NOP; ix <= nxp - NOP; ++ix)
-            for (int iz = NOP; iz <= nzp - NOP; ++iz) {
-                size_t c = g.at(ix, iz);
-                R d1 = 1 - dt * damp[c] / 2, d2 = 1 + dt * damp[c] / 2;
-                if (iz <= nzp - NOP - 1) {  // GPUcalculate_vx bounds
-                    R tp = 0;
-                    for (int i = 1; i <= NOP; ++i)
-                        tp += (R)C8[i] * (p[c + (size_t)(i - 1) * nzp] - p[c - (size_t)i * nzp]);
-                    vx[c] = (d1 * vx[c] - ((R)1 / rr[c]) * (dt / dx) * tp) / d2;
-                }
-                if (ix <= nxp - NOP - 1) {  // GPUcalculate_vz bounds
-                    R tp = 0;
-                    for (int i = 1; i <= NOP; ++i) tp += (R)C8[i] * (p[c + i - 1] - p[c - i]);
-                    vz[c] = (d1 * vz[c] - ((R)1 / rr[c]) * (dt / dx) * tp) / d2;
-                }
-            }
+        acoustic_ref_step(g, k.data(), rr.data(), damp.data(), px.data(), pz.data(), p.data(), vx.data(), vz.data());
         vz[g.at(sxnum + pml, pml)] += wav[it];
         for (int ix = 0; ix < nx; ++ix) rec_p[(size_t)ix * s->nt + it] = p[g.at(ix + pml, pml)];
     }
@@ -833,6 +989,25 @@ int orc_threads(int n) {
     int orc_acoustic_ref_forward_##SUF(const orc_shot* s, const R* vel, const R* rho, const R* wav, int sxnum, \
                                        R* rec) {                                                               \
         acoustic_ref_forward<R>(s, vel, rho, wav, sxnum, rec); return 0; }                                     \
+    int orc_elsrtm_born_##SUF(const orc_shot* s, const R* vp, const R* vs, const R* rho, const R* wav,         \
+                              const R* ipp, const R* ips, int col0, int order, R* rz, R* rx) {                 \
+        elsrtm_born<R>(s, vp, vs, rho, wav, ipp, ips, col0, order, rz, rx); return 0; }                        \
+    int orc_kernel_##SUF(int which, int nxp, int nzp, int pml, double dx, double dz, double dt, const R* vp,   \
+                         const R* vs, const R* rho, const R* lam, const R* mu, const R* dampx, const R* dampz, \
+                         const R* dmp, const R* dms, int dir, const R* src15, R* f15, R* strain4) {             \
+        kernel<R>(which, nxp, nzp, pml, (R)dx, (R)dz, (R)dt, vp, vs, rho, lam, mu, dampx, dampz, dmp, dms, dir, \
+                  src15, f15, strain4); return 0; }                                                             \
+    int orc_edge_io_##SUF(int nxp, int nzp, int pml, int nt, R* field, R* bands, int it, int save) {           \
+        edge_io_x<R>(nxp, nzp, pml, nt, field, bands, it, save); return 0; }                                   \
+    int orc_image_##SUF(int nxp, int nzp, int pml, const R* st4, const R* bk15, const R* vp, const R* vs,       \
+                        const R* rho, int col0, int allnz, R* pp, R* ps, R* il) {                              \
+        image_x<R>(nxp, nzp, pml, st4, bk15, vp, vs, rho, col0, allnz, pp, ps, il); return 0; }                \
+    int orc_acoustic_ref_damp_##SUF(int nxp, int nzp, int pml, const R* vt, double rr, R* damp) {              \
+        Grid<R> g{nxp, nzp, pml, 1, 1, 1}; acoustic_ref_damp<R>(g, vt, (R)rr, damp); return 0; }              \
+    int orc_acoustic_ref_step_##SUF(int nxp, int nzp, double dx, double dz, double dt, const R* k,             \
+                                    const R* rr, const R* damp, R* px, R* pz, R* p, R* vx, R* vz) {           \
+        Grid<R> g{nxp, nzp, 0, (R)dx, (R)dz, (R)dt};                                                          \
+        acoustic_ref_step<R>(g, k, rr, damp, px, pz, p, vx, vz); return 0; }                                   \
     int orc_acou_forward_##SUF(const orc_shot* s, const R* vp, const R* rho, const R* refl, const R* wav,     \
                                R* rbg, R* rborn, R* final3) {                                                  \
         acou_forward<R>(s, vp, rho, refl, wav, rbg, rborn, final3); return 0; }                               \

@@ -234,3 +234,77 @@ def eborn_adjoint(shot, vp, vs, rho, wav, res_z, res_x, prec="d"):
     args = [_c(a, prec) for a in (vp, vs, rho, wav, res_z, res_x)]
     getattr(load(), f"orc_eborn_adjoint_{prec}")(C.byref(shot), *[P(a) for a in args], P(gpp), P(gps))
     return gpp, gps
+
+
+# --- single kernels on padded arrays (pin tests) ---------------------------
+KERNELS = {"update_v": 0, "update_s": 1, "recon_v": 2, "recon_s": 3, "adjoint_v": 4, "adjoint_s": 5,
+           "add_born_source": 6, "add_born_source_adjoint": 7}
+FIELDS15 = ["vx", "vz", "txx", "tzz", "txz", "vx_x", "vx_z", "vz_x", "vz_z", "txx_x", "txx_z", "tzz_x", "tzz_z",
+            "txz_x", "txz_z"]
+
+
+def kernel(name, f15, *, pml=0, dx=1.0, dz=1.0, dt=1.0, vp=None, vs=None, rho=None, lam=None, mu=None,
+           dampx=None, dampz=None, dmp=None, dms=None, direction=1, src15=None, prec="d"):
+    """Apply one oracle kernel to a (15, nxp, nzp) state; returns (state, strain4)."""
+    f = _c(f15, prec).copy()
+    _, nxp, nzp = f.shape
+    st = np.zeros((4, nxp, nzp), dtype_of(prec))
+    arrs = [_c(a, prec) for a in (vp, vs, rho, lam, mu, dampx, dampz, dmp, dms)]
+    src = _c(src15, prec)
+    getattr(load(), f"orc_kernel_{prec}")(KERNELS[name], nxp, nzp, pml, C.c_double(dx), C.c_double(dz),
+                                          C.c_double(dt), *[P(a) for a in arrs], direction, P(src), P(f), P(st))
+    return f, st
+
+
+def edge_io(field, bands, pml, nt, it, save, prec="d"):
+    f = _c(field, prec).copy()
+    b = _c(bands, prec).copy()
+    nxp, nzp = f.shape
+    getattr(load(), f"orc_edge_io_{prec}")(nxp, nzp, pml, nt, P(f), P(b), it, 1 if save else 0)
+    return f, b
+
+
+def edge_band_size(nxp, nzp, pml, nt, N=4):
+    nx, nz = nxp - 2 * pml, nzp - 2 * pml
+    return 2 * nt * nx * 2 * N + 2 * nt * 2 * N * nz
+
+
+def image(strain4, bk15, vp, vs, rho, pml, col0, allnx, allnz, pp=None, ps=None, il=None, prec="d"):
+    t = dtype_of(prec)
+    pp = np.zeros((allnx, allnz), t) if pp is None else _c(pp, prec).copy()
+    ps = np.zeros((allnx, allnz), t) if ps is None else _c(ps, prec).copy()
+    il = np.zeros((allnx, allnz), t) if il is None else _c(il, prec).copy()
+    s4, b = _c(strain4, prec), _c(bk15, prec)
+    _, nxp, nzp = s4.shape
+    args = [_c(a, prec) for a in (vp, vs, rho)]
+    getattr(load(), f"orc_image_{prec}")(nxp, nzp, pml, P(s4), P(b), *[P(a) for a in args], col0, allnz, P(pp),
+                                         P(ps), P(il))
+    return pp, ps, il
+
+
+def acoustic_ref_damp(vt, pml, rr, prec="d"):
+    v = _c(vt, prec)
+    nxp, nzp = v.shape
+    d = np.zeros_like(v)
+    getattr(load(), f"orc_acoustic_ref_damp_{prec}")(nxp, nzp, pml, P(v), C.c_double(rr), P(d))
+    return d
+
+
+def acoustic_ref_step(k, rr, damp, state5, dx, dz, dt, prec="d"):
+    """state5 = (px, pz, p, vx, vz), each (nxp, nzp); one FD2DGPU.cu:253-325 step."""
+    s = _c(state5, prec).copy()
+    _, nxp, nzp = s.shape
+    args = [_c(a, prec) for a in (k, rr, damp)]
+    getattr(load(), f"orc_acoustic_ref_step_{prec}")(nxp, nzp, C.c_double(dx), C.c_double(dz), C.c_double(dt),
+                                                     *[P(a) for a in args], *[P(s[i]) for i in range(5)])
+    return s
+
+
+def elsrtm_born(shot, vp, vs, rho, wav, img_pp, img_ps, col0, order, prec="d"):
+    """Compat Born records: Lm (order 1) or Lm + Lm2 (order 2)."""
+    t = dtype_of(prec)
+    rz = np.zeros(shot.ntr * shot.nt, t)
+    rx = np.zeros_like(rz)
+    args = [_c(a, prec) for a in (vp, vs, rho, wav, img_pp, img_ps)]
+    getattr(load(), f"orc_elsrtm_born_{prec}")(C.byref(shot), *[P(a) for a in args], col0, order, P(rz), P(rx))
+    return rz, rx

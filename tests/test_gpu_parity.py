@@ -106,11 +106,19 @@ def test_graph_capture_matches_eager():
     assert orc.rel_l2(ez[shots[0].ntr * nt:], oz) <= TOL
 
 
+@pytest.mark.parametrize("source_fp64", [1, 0])
 @pytest.mark.parametrize("boundary", [1, 0])
 @pytest.mark.parametrize("it", [0, 1])
-def test_elsrtm_gradient(boundary, it):
+def test_elsrtm_gradient(boundary, it, source_fp64):
     """Reference per-shot ELSRTM gradient (Born + 2nd-order Born, NCC, three
-    imaging terms) for two shots in one batch, vs the oracle."""
+    imaging terms) for two shots in one batch, vs the FP64 oracle.
+
+    Default precision (background and first-order Born field in FP64, every
+    other wavefield FP32): all seven images and the misfit within 1e-5.  With
+    every field in FP32, as the reference runs, the background's rounding
+    noise is amplified through the Born sources: the oracle's own FP32
+    restatement is up to 3e-4 off FP64 there, and the GPU must stay within
+    1.5x of that restatement's error (and 1e-5 where it is below)."""
     allnx, allnz, pml, nt = 56, 32, 20, 260
     vp, vs, _ = elastic_model(allnx, allnz, 13)
     wav = wavelet(nt)
@@ -129,6 +137,7 @@ def test_elsrtm_gradient(boundary, it):
         obs_x.append(ox.astype(np.float32))
     obs_z, obs_x = np.concatenate(obs_z), np.concatenate(obs_x)
     s = _solver(allnx, allnz, pml, nt, 40, wav, rho_const=2000.0)
+    s.set_precision(source_fp64)
     s.set_model(vp, vs, None)
     s.set_image(img_pp, img_ps)
     mf = s.batch_gradient(shots, obs_z, obs_x, it)
@@ -155,11 +164,9 @@ def test_elsrtm_gradient(boundary, it):
             assert np.abs(g[k]).max() == 0, names[k]
             continue
         e = orc.rel_l2(g[k], ref)
-        # The reference schedule is FP32-sensitive: an FP32 restatement of it
-        # already drifts from FP64 by e32; FP32 on the GPU must do no worse.
         e32 = orc.rel_l2(ogf[k], ref)
-        tol = max(TOL, 4 * e32)
-        print(f"{names[k]}: rel {e:.2e} (fp32 oracle {e32:.2e}, tol {tol:.1e})")
+        tol = TOL if source_fp64 else max(TOL, 1.5 * e32)
+        print(f"source_fp64={source_fp64} {names[k]}: rel {e:.2e} (fp32 oracle {e32:.2e}, tol {tol:.1e})")
         assert e <= tol, names[k]
     # the misfit is a function of the FP32 scatter records: same 1e-5 bar
     assert abs(mf - omf) <= TOL * max(1.0, abs(omf))
