@@ -2127,47 +2127,6 @@ int ds2d_sjcg_update(int n, float* m, float* cg, const float* g1, float* g0, int
     return 0;
 }
 
-// laplace.cpp:82-106: (1/dx^2)(5-point Laplacian) on [2, n-2), zero frame.
-int ds2d_laplace_filter(int nz, int nx, int order, float dx, float* dat) {
-    (void)order;
-    if (!dat || nz <= 0 || nx <= 0 || dx == 0) return fail(DS2D_EINVAL, "bad laplace arguments");
-    std::vector<float> t1(dat, dat + (size_t)nx * nz), t2((size_t)nx * nz, 0.f);
-    for (int ix = 2; ix < nx - 2; ++ix)
-        for (int iz = 2; iz < nz - 2; ++iz) {
-            const size_t c = (size_t)ix * nz + iz;
-            t2[c] = (float)((1.0 / dx / dx) * (t1[c + nz] + t1[c - nz] - 2 * t1[c] + t1[c + 1] + t1[c - 1] - 2 * t1[c]));
-        }
-    std::memcpy(dat, t2.data(), t2.size() * 4);
-    return 0;
-}
-
-// laplace.cpp:3-57 band filter with its stated intent (padded stride nz+4,
-// zero frame): out = Sx(orig - (Sx o Sz)^4 orig), S = (1/4, 1/2, 1/4).
-int ds2d_laplace_band(int nz, int nx, float* dat) {
-    if (!dat || nz <= 0 || nx <= 0) return fail(DS2D_EINVAL, "bad laplace arguments");
-    const int s = nz + 4;
-    std::vector<float> t0((size_t)(nx + 4) * s, 0.f), t1(t0.size(), 0.f), t2(t0.size(), 0.f);
-    auto I = [&](int ix, int iz) { return (size_t)(ix + 2) * s + (iz + 2); };
-    for (int ix = 0; ix < nx; ++ix)
-        for (int iz = 0; iz < nz; ++iz) t0[I(ix, iz)] = t1[I(ix, iz)] = dat[(size_t)ix * nz + iz];
-    for (int k = 0; k < 4; ++k) {
-        for (int ix = 0; ix < nx; ++ix)
-            for (int iz = 0; iz < nz; ++iz)
-                t2[I(ix, iz)] = t1[I(ix, iz)] * 0.5f + t1[I(ix, iz - 1)] * 0.25f + t1[I(ix, iz + 1)] * 0.25f;
-        for (int ix = 0; ix < nx; ++ix)
-            for (int iz = 0; iz < nz; ++iz)
-                t1[I(ix, iz)] = t2[I(ix, iz)] * 0.5f + t2[I(ix - 1, iz)] * 0.25f + t2[I(ix + 1, iz)] * 0.25f;
-    }
-    for (int ix = 0; ix < nx; ++ix)
-        for (int iz = 0; iz < nz; ++iz) t2[I(ix, iz)] = t0[I(ix, iz)] - t1[I(ix, iz)];
-    for (int ix = 0; ix < nx; ++ix)
-        for (int iz = 0; iz < nz; ++iz)
-            t1[I(ix, iz)] = t2[I(ix, iz)] * 0.5f + t2[I(ix - 1, iz)] * 0.25f + t2[I(ix + 1, iz)] * 0.25f;
-    for (int ix = 0; ix < nx; ++ix)
-        for (int iz = 0; iz < nz; ++iz) dat[(size_t)ix * nz + iz] = t1[I(ix, iz)];
-    return 0;
-}
-
 // main.cpp:461-473 ricker1.
 int ds2d_ricker(int nt, float f, float dt, float* s) {
     if (!s || nt <= 0 || f <= 0 || dt <= 0) return fail(DS2D_EINVAL, "bad ricker arguments");
