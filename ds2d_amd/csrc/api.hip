@@ -1046,7 +1046,8 @@ int ds2d_acoustic_ref_forward(ds2d_t* h, int sxnum, int ngx_min, float* rec_p) {
 namespace {
 
 struct AcouWS {
-    AcouField bg, bn, lm;
+    AcouFieldD bg;              // FP64 background
+    AcouField bn, lm;
     AcouField bn2, lm2;         // second state of the fused (ping-pong) Born / adjoint steps
     AcouMed m;
     float *grad_pad = nullptr;  // per-slot padded gradient (n)
@@ -1093,14 +1094,15 @@ int acou_prepare(ds2d_ctx* h, const Batch& B, const float* refl_glob_dev, AcouWS
     w.R = std::max<long long>(B.total_rec, 1);
     w.W = std::max<long long>(B.total_win, 1);
     float* base;
-    if ((rc = h->fbuf("acou", (size_t)n * 19, &base))) return rc;
-    w.bg = AcouField{base, base + n, base + 2 * n};
-    w.bn = AcouField{base + 3 * n, base + 4 * n, base + 5 * n};
-    w.lm = AcouField{base + 6 * n, base + 7 * n, base + 8 * n};
-    w.grad_pad = base + 9 * n;
-    float *bu = base + 10 * n, *kk = base + 11 * n, *refl = base + 12 * n;
-    w.bn2 = AcouField{base + 13 * n, base + 14 * n, base + 15 * n};
-    w.lm2 = AcouField{base + 16 * n, base + 17 * n, base + 18 * n};
+    double* bgd;
+    if ((rc = h->fbuf("acou", (size_t)n * 16, &base)) || (rc = h->fbuf("acoubg", (size_t)n * 3, &bgd))) return rc;
+    w.bg = AcouFieldD{bgd, bgd + n, bgd + 2 * n};
+    w.bn = AcouField{base, base + n, base + 2 * n};
+    w.lm = AcouField{base + 3 * n, base + 4 * n, base + 5 * n};
+    w.grad_pad = base + 6 * n;
+    float *bu = base + 7 * n, *kk = base + 8 * n, *refl = base + 9 * n;
+    w.bn2 = AcouField{base + 10 * n, base + 11 * n, base + 12 * n};
+    w.lm2 = AcouField{base + 13 * n, base + 14 * n, base + 15 * n};
     Med m;
     if ((rc = alloc_med(h, n, m))) return rc;
     SetupArgs a = setup_args(h, false);
@@ -1271,7 +1273,8 @@ void acou_fwd(ds2d_ctx* h, const Batch& B, AcouWS& w, bool do_bg, bool do_born, 
         w.tune = false;
         acou_tune(h, B, w);
     }
-    (void)hipMemsetAsync(w.bg.vx, 0, (size_t)w.n * 6 * 4, st);  // bg + bn
+    (void)hipMemsetAsync(w.bg.vx, 0, (size_t)w.n * 3 * 8, st);
+    (void)hipMemsetAsync(w.bn.vx, 0, (size_t)w.n * 3 * 4, st);
     if (w.fused && !do_bg && do_born && !rec_bg) {
         // Born field only, from the resident history: one fused launch per step
         AcouFusedArgs even = acou_fused_args(h, w, w.bn, w.bn2), odd = acou_fused_args(h, w, w.bn2, w.bn);

@@ -83,24 +83,31 @@ void launch_dot(hipStream_t st, const float* x, const float* y, long long n, dou
 struct AcouField {
     float *vx, *vz, *p;
 };
+// The background of the plain acoustic Born pair is carried in FP64: its
+// increment drives the Born source at every step, and FP32 rounding of a
+// 2500-step background alone puts the Born records 1e-5 off FP64.  Only the
+// increment, rounded once to FP32, reaches the Born field and the history.
+struct AcouFieldD {
+    double *vx, *vz, *p;
+};
 struct AcouMed {
     const float *b, *k, *m;       // buoyancy, modulus, reflectivity window (zero frame)
     const float *damp, *rho;      // reference acoustic layout only
 };
-void launch_acou_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, const float* wav,
+void launch_acou_v(hipStream_t st, const BatchShape& b, AcouFieldD bg, AcouField born, AcouMed m, const float* wav,
                    int it, float sign, int do_bg, int do_born);
 // hist_mode: 0 none, 1 store the background increment, 2 read it (do_bg = 0).
 // Records (rec_bg / rec_born, may be null) are taken inside, through rmap.
-void launch_acou_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, float* hist,
+void launch_acou_p(hipStream_t st, const BatchShape& b, AcouFieldD bg, AcouField born, AcouMed m, float* hist,
                    float* rec_bg, float* rec_born, int it, int do_bg, int do_born, int hist_mode, const int* rmap);
 // Backward step it: (injection of res[it] already in lam.p) adj_p, then adj_v,
 // which also injects res[it - 1] for the next step.  Step nt - 1 is injected
 // by launch_acou_inj.
 void launch_acou_inj(hipStream_t st, const BatchShape& b, float* lp, const float* res, int it);
 // hist != null: gradient from the stored increment, background untouched
-void launch_acou_adj_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m, float* grad,
+void launch_acou_adj_p(hipStream_t st, const BatchShape& b, AcouFieldD bg, AcouField lam, AcouMed m, float* grad,
                        const float* hist, int it);
-void launch_acou_adj_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m,
+void launch_acou_adj_v(hipStream_t st, const BatchShape& b, AcouFieldD bg, AcouField lam, AcouMed m,
                        const float* wav, int it, int do_bg, const int* rmap, const float* res);
 struct ChunkDesc;
 // Fused plain acoustic steps (kernels_acoustic_fused.hip): state A -> B, each

@@ -28,13 +28,27 @@ __device__ __forceinline__ bool tile_cell(const SlotDesc& S, Tile& t) {
     return true;
 }
 
-__device__ __forceinline__ float dpx(const float* f, size_t c, int s) {
-    return DS2D_C1 * (f[c + s] - f[c]) + DS2D_C2 * (f[c + 2 * s] - f[c - s]) +
-           DS2D_C3 * (f[c + 3 * s] - f[c - 2 * s]) + DS2D_C4 * (f[c + 4 * s] - f[c - 3 * s]);
+template <typename R>
+__device__ __forceinline__ R dpx(const R* f, size_t c, int s) {
+    return (R)DS2D_C1 * (f[c + s] - f[c]) + (R)DS2D_C2 * (f[c + 2 * s] - f[c - s]) +
+           (R)DS2D_C3 * (f[c + 3 * s] - f[c - 2 * s]) + (R)DS2D_C4 * (f[c + 4 * s] - f[c - 3 * s]);
 }
-__device__ __forceinline__ float dmx(const float* f, size_t c, int s) {
-    return DS2D_C1 * (f[c] - f[c - s]) + DS2D_C2 * (f[c + s] - f[c - 2 * s]) +
-           DS2D_C3 * (f[c + 2 * s] - f[c - 3 * s]) + DS2D_C4 * (f[c + 3 * s] - f[c - 4 * s]);
+template <typename R>
+__device__ __forceinline__ R dmx(const R* f, size_t c, int s) {
+    return (R)DS2D_C1 * (f[c] - f[c - s]) + (R)DS2D_C2 * (f[c + s] - f[c - 2 * s]) +
+           (R)DS2D_C3 * (f[c + 2 * s] - f[c - 3 * s]) + (R)DS2D_C4 * (f[c + 3 * s] - f[c - 4 * s]);
+}
+// v += sign (dt/d) b D+ (p + w delta_src) of the FP64 background at cell c.
+__device__ __forceinline__ void bg_v_step(const SlotDesc& S, AcouFieldD bg, size_t c, size_t s, double w, double bu,
+                                          double ax, double az, double sign) {
+    const size_t src = (size_t)S.off + (size_t)S.isx * S.nzp + S.isz;
+    auto P = [&](size_t j) { return j == src ? bg.p[j] + w : bg.p[j]; };
+    const double a = (double)DS2D_C1 * (P(c + s) - P(c)) + (double)DS2D_C2 * (P(c + 2 * s) - P(c - s)) +
+                     (double)DS2D_C3 * (P(c + 3 * s) - P(c - 2 * s)) + (double)DS2D_C4 * (P(c + 4 * s) - P(c - 3 * s));
+    const double b = (double)DS2D_C1 * (P(c + 1) - P(c)) + (double)DS2D_C2 * (P(c + 2) - P(c - 1)) +
+                     (double)DS2D_C3 * (P(c + 3) - P(c - 2)) + (double)DS2D_C4 * (P(c + 4) - P(c - 3));
+    bg.vx[c] += sign * (ax * bu * a);
+    bg.vz[c] += sign * (az * bu * b);
 }
 
 }  // namespace
@@ -48,26 +62,16 @@ __device__ __forceinline__ bool hist_cell(const SlotDesc& S, int ix, int iz, int
     return true;
 }
 
-// bg: v += sign * (dt/d) b D+ (p + w delta_src);  born: dv += (dt/d) b D+ dp.
-__global__ __launch_bounds__(256) void k_acou_v(AcouField bg, AcouField bn, AcouMed m, const float* __restrict__ wav,
-                                                const SlotDesc* __restrict__ slots, float ax, float az, int it,
-                                                float sign, int do_bg, int do_born) {
+// bg (FP64): v += sign * (dt/d) b D+ (p + w delta_src);  born: dv += (dt/d) b D+ dp.
+__global__ __launch_bounds__(256) void k_acou_v(AcouFieldD bg, AcouField bn, AcouMed m, const float* __restrict__ wav,
+                                                const SlotDesc* __restrict__ slots, float ax, float az, double axd,
+                                                double azd, int it, float sign, int do_bg, int do_born) {
     const SlotDesc S = slots[blockIdx.z];
     Tile t;
     if (!tile_cell(S, t)) return;
     const size_t c = t.c, s = S.nzp;
     const float bu = m.b[c];
-    if (do_bg) {
-        const size_t src = (size_t)S.off + (size_t)S.isx * S.nzp + S.isz;
-        const float w = wav[it];
-        auto P = [&](size_t j) { return j == src ? bg.p[j] + w : bg.p[j]; };
-        const float a = DS2D_C1 * (P(c + s) - P(c)) + DS2D_C2 * (P(c + 2 * s) - P(c - s)) +
-                        DS2D_C3 * (P(c + 3 * s) - P(c - 2 * s)) + DS2D_C4 * (P(c + 4 * s) - P(c - 3 * s));
-        const float b = DS2D_C1 * (P(c + 1) - P(c)) + DS2D_C2 * (P(c + 2) - P(c - 1)) +
-                        DS2D_C3 * (P(c + 3) - P(c - 2)) + DS2D_C4 * (P(c + 4) - P(c - 3));
-        bg.vx[c] += sign * (ax * bu * a);
-        bg.vz[c] += sign * (az * bu * b);
-    }
+    if (do_bg) bg_v_step(S, bg, c, s, (double)wav[it], (double)bu, axd, azd, (double)sign);
     if (do_born) {
         bn.vx[c] += ax * bu * dpx(bn.p, c, (int)s);
         bn.vz[c] += az * bu * dpx(bn.p, c, 1);
@@ -77,10 +81,10 @@ __global__ __launch_bounds__(256) void k_acou_v(AcouField bg, AcouField bn, Acou
 // bg: dp = k (dt/dx D- vx + dt/dz D- vz); p += dp (hist_mode 1 also stores dp
 // on the physical window); without bg, hist_mode 2 reads dp back.
 // born: dpb += k D- dv + 2 m dp.
-__global__ __launch_bounds__(256) void k_acou_p(AcouField bg, AcouField bn, AcouMed m, float* hist,
-                                                const SlotDesc* __restrict__ slots, float ax, float az, int it,
-                                                int do_bg, int do_born, int hist_mode, const int* __restrict__ rmap,
-                                                float* rbg, float* rbn, int nt) {
+__global__ __launch_bounds__(256) void k_acou_p(AcouFieldD bg, AcouField bn, AcouMed m, float* hist,
+                                                const SlotDesc* __restrict__ slots, float ax, float az, double axd,
+                                                double azd, int it, int do_bg, int do_born, int hist_mode,
+                                                const int* __restrict__ rmap, float* rbg, float* rbn, int nt) {
     const SlotDesc S = slots[blockIdx.z];
     Tile t;
     if (!tile_cell(S, t)) return;
@@ -91,8 +95,9 @@ __global__ __launch_bounds__(256) void k_acou_p(AcouField bg, AcouField bn, Acou
     const bool inside = hist_cell(S, t.ix, t.iz, it, h);
     float dp = 0.f;
     if (do_bg) {
-        dp = kk * (ax * dmx(bg.vx, c, s) + az * dmx(bg.vz, c, 1));
-        bg.p[c] += dp;
+        const double d = (double)kk * (axd * dmx(bg.vx, c, s) + azd * dmx(bg.vz, c, 1));
+        bg.p[c] += d;
+        dp = (float)d;
         if (hist_mode == 1 && inside) hist[h] = dp;
     } else if (hist_mode == 2 && inside) {
         dp = hist[h];
@@ -106,7 +111,7 @@ __global__ __launch_bounds__(256) void k_acou_p(AcouField bg, AcouField bn, Acou
         const int r = rmap[S.rmap_off + t.ix];
         if (r >= 0) {
             const size_t o = (size_t)S.rec_off + (size_t)r * nt + it;
-            if (rbg) rbg[o] = bg.p[c];
+            if (rbg) rbg[o] = (float)bg.p[c];
             if (rbn) rbn[o] = bn.p[c];
         }
     }
@@ -116,9 +121,10 @@ __global__ __launch_bounds__(256) void k_acou_p(AcouField bg, AcouField bn, Acou
 //   d = k D- v^{n+1} (reconstruct) or the stored increment (hist != null);
 //   grad += 2 d lp;  reconstruct: p^n = p^{n+1} - d;
 //   lv += -(dt/d) D+ (k lp) with k lp masked to the update region.
-__global__ __launch_bounds__(256) void k_acou_adj_p(AcouField bg, AcouField lm, AcouMed m, float* grad,
+__global__ __launch_bounds__(256) void k_acou_adj_p(AcouFieldD bg, AcouField lm, AcouMed m, float* grad,
                                                     const float* __restrict__ hist,
-                                                    const SlotDesc* __restrict__ slots, float ax, float az, int it) {
+                                                    const SlotDesc* __restrict__ slots, float ax, float az,
+                                                    double axd, double azd, int it) {
     const SlotDesc S = slots[blockIdx.z];
     Tile t;
     if (!tile_cell(S, t)) return;
@@ -128,9 +134,8 @@ __global__ __launch_bounds__(256) void k_acou_adj_p(AcouField bg, AcouField lm, 
         size_t h = 0;
         if (hist_cell(S, t.ix, t.iz, it, h)) grad[c] += 2 * hist[h] * lm.p[c];
     } else {
-        const float kk = m.k[c];
-        const float d = kk * (ax * dmx(bg.vx, c, s) + az * dmx(bg.vz, c, 1));
-        grad[c] += 2 * d * lm.p[c];
+        const double d = (double)m.k[c] * (axd * dmx(bg.vx, c, s) + azd * dmx(bg.vz, c, 1));
+        grad[c] += 2 * (float)d * lm.p[c];
         bg.p[c] -= d;
     }
     auto W = [&](int jx, int jz) {
@@ -149,27 +154,16 @@ __global__ __launch_bounds__(256) void k_acou_adj_p(AcouField bg, AcouField lm, 
 // Backward kernel 2: (reconstruct only) background velocity reversal with the
 // transient source, v^n = v^{n+1} - (dt/d) b D+ (p^n + w_n delta_src);
 //   lp += -(dt/d) D- (b lv) with b lv masked to the update region.
-__global__ __launch_bounds__(256) void k_acou_adj_v(AcouField bg, AcouField lm, AcouMed m,
+__global__ __launch_bounds__(256) void k_acou_adj_v(AcouFieldD bg, AcouField lm, AcouMed m,
                                                     const float* __restrict__ wav, const SlotDesc* __restrict__ slots,
-                                                    float ax, float az, int it, int do_bg,
+                                                    float ax, float az, double axd, double azd, int it, int do_bg,
                                                     const int* __restrict__ rmap, const float* __restrict__ res,
                                                     int nt) {
     const SlotDesc S = slots[blockIdx.z];
     Tile t;
     if (!tile_cell(S, t)) return;
     const size_t c = t.c, s = S.nzp;
-    if (do_bg) {
-        const size_t src = (size_t)S.off + (size_t)S.isx * S.nzp + S.isz;
-        const float w = wav[it];
-        auto P = [&](size_t j) { return j == src ? bg.p[j] + w : bg.p[j]; };
-        const float a = DS2D_C1 * (P(c + s) - P(c)) + DS2D_C2 * (P(c + 2 * s) - P(c - s)) +
-                        DS2D_C3 * (P(c + 3 * s) - P(c - 2 * s)) + DS2D_C4 * (P(c + 4 * s) - P(c - 3 * s));
-        const float b = DS2D_C1 * (P(c + 1) - P(c)) + DS2D_C2 * (P(c + 2) - P(c - 1)) +
-                        DS2D_C3 * (P(c + 3) - P(c - 2)) + DS2D_C4 * (P(c + 4) - P(c - 3));
-        const float bu = m.b[c];
-        bg.vx[c] += -1.f * (ax * bu * a);
-        bg.vz[c] += -1.f * (az * bu * b);
-    }
+    if (do_bg) bg_v_step(S, bg, c, s, (double)wav[it], (double)m.b[c], axd, azd, -1.0);
     const int ix = t.ix, iz = t.iz;
     auto UX = [&](int jx, int jz) {
         const size_t j = (size_t)S.off + (size_t)jx * s + jz;
@@ -204,29 +198,33 @@ __global__ void k_acou_inj(float* lp, const float* __restrict__ res, const int* 
 
 static dim3 cgrid(const BatchShape& b) { return dim3(ceil_div(b.nzp, BZ), ceil_div(b.max_nxp, BX), b.nslots); }
 
-void launch_acou_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, const float* wav,
+// FP64 background coefficients dt/dx, dt/dz from the FP32 grid constants
+static double dxd(const BatchShape& b) { return (double)b.dt / (double)b.dx; }
+static double dzd(const BatchShape& b) { return (double)b.dt / (double)b.dz; }
+
+void launch_acou_v(hipStream_t st, const BatchShape& b, AcouFieldD bg, AcouField born, AcouMed m, const float* wav,
                    int it, float sign, int do_bg, int do_born) {
     hipLaunchKernelGGL(k_acou_v, cgrid(b), dim3(BZ, BX), 0, st, bg, born, m, wav, b.slots, b.dt / b.dx, b.dt / b.dz,
-                       it, sign, do_bg, do_born);
+                       dxd(b), dzd(b), it, sign, do_bg, do_born);
 }
-void launch_acou_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField born, AcouMed m, float* hist,
+void launch_acou_p(hipStream_t st, const BatchShape& b, AcouFieldD bg, AcouField born, AcouMed m, float* hist,
                    float* rec_bg, float* rec_born, int it, int do_bg, int do_born, int hist_mode, const int* rmap) {
     hipLaunchKernelGGL(k_acou_p, cgrid(b), dim3(BZ, BX), 0, st, bg, born, m, hist, b.slots, b.dt / b.dx, b.dt / b.dz,
-                       it, do_bg, do_born, hist_mode, rmap, rec_bg, rec_born, b.nt);
+                       dxd(b), dzd(b), it, do_bg, do_born, hist_mode, rmap, rec_bg, rec_born, b.nt);
 }
 void launch_acou_inj(hipStream_t st, const BatchShape& b, float* lp, const float* res, int it) {
     hipLaunchKernelGGL(k_acou_inj, dim3(ceil_div(b.max_ntr, 64), b.nslots), dim3(64), 0, st, lp, res, b.rix,
                        b.slots, b.nt, it);
 }
-void launch_acou_adj_p(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m, float* grad,
+void launch_acou_adj_p(hipStream_t st, const BatchShape& b, AcouFieldD bg, AcouField lam, AcouMed m, float* grad,
                        const float* hist, int it) {
     hipLaunchKernelGGL(k_acou_adj_p, cgrid(b), dim3(BZ, BX), 0, st, bg, lam, m, grad, hist, b.slots, b.dt / b.dx,
-                       b.dt / b.dz, it);
+                       b.dt / b.dz, dxd(b), dzd(b), it);
 }
-void launch_acou_adj_v(hipStream_t st, const BatchShape& b, AcouField bg, AcouField lam, AcouMed m,
+void launch_acou_adj_v(hipStream_t st, const BatchShape& b, AcouFieldD bg, AcouField lam, AcouMed m,
                        const float* wav, int it, int do_bg, const int* rmap, const float* res) {
     hipLaunchKernelGGL(k_acou_adj_v, cgrid(b), dim3(BZ, BX), 0, st, bg, lam, m, wav, b.slots, b.dt / b.dx,
-                       b.dt / b.dz, it, do_bg, rmap, res, b.nt);
+                       b.dt / b.dz, dxd(b), dzd(b), it, do_bg, rmap, res, b.nt);
 }
 
 // ---------------------------------------------------------------------------

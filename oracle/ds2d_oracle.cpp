@@ -884,7 +884,10 @@ void laplace_order(int nz, int nx, R dx, R* dat) {
     for (int ix = 2; ix < nx - 2; ++ix)
         for (int iz = 2; iz < nz - 2; ++iz) {
             size_t c = (size_t)ix * nz + iz;
-            t2[c] = ((R)1.0 / dx / dx) * (t1[c + nz] + t1[c - nz] - 2 * t1[c] + t1[c + 1] + t1[c - 1] - 2 * t1[c]);
+            // laplace.cpp:95: `1.0/dx/dx` is a double expression; the float sum is
+            // widened, multiplied in double and rounded once to R.
+            const R sum = t1[c + nz] + t1[c - nz] - 2 * t1[c] + t1[c + 1] + t1[c - 1] - 2 * t1[c];
+            t2[c] = (R)((1.0 / (double)dx / (double)dx) * (double)sum);
         }
     std::copy(t2.begin(), t2.end(), dat);
 }

@@ -9,9 +9,12 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle", "libds2d_oracle.so")
+# the same source built with FMA contraction (a second valid FP32 order)
+ORACLE_FMA = os.path.join(ROOT, "oracle", "libds2d_oracle_fma.so")
 REF_LAPLACE = os.path.join(ROOT, "oracle", "_ref", "libref_laplace.so")
 
 _lib = None
+_lib_fma = None
 
 
 class OrcShot(C.Structure):
@@ -31,6 +34,15 @@ def load():
         _lib.orc_rb_uniform.restype = C.c_double
         _lib.orc_rb_uniform.argtypes = [C.c_ulonglong, C.c_int, C.c_int, C.c_int]
     return _lib
+
+
+def load_fma():
+    global _lib_fma
+    if _lib_fma is None:
+        if not os.path.exists(ORACLE_FMA):
+            raise RuntimeError("oracle not built: make -C oracle")
+        _lib_fma = C.CDLL(ORACLE_FMA)
+    return _lib_fma
 
 
 def threads(n=0):
@@ -74,7 +86,11 @@ def elastic_forward(shot, vp, vs, rho, wav, prec="d", final=False):
 
 
 def elsrtm_shot(shot, vp, vs, rho, wav, img_pp, img_ps, obs_z, obs_x, it, col0, grads7=None, prec="d"):
+    """prec "d" (FP64), "f" (FP32, separate multiply and add) or "f_fma"
+    (FP32 with multiply-adds fused)."""
     lib = load()
+    if prec == "f_fma":
+        lib, prec = load_fma(), "f"
     t = dtype_of(prec)
     A = shot.allnx * shot.allnz
     if grads7 is None:

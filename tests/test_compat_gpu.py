@@ -73,7 +73,10 @@ def test_set_shot_shot_gradient_take_gradients_sequence(boundary):
         e = orc.rel_l2(g[k], og[k])
         print(f"boundary={boundary} {name}: rel {e:.2e}")
         assert np.abs(og[k]).max() > 0 and e <= TOL, name
-    assert abs(mf - omf) <= TOL * abs(omf)
+    # The misfit is -1/(2 ntr) times a sum of normalised correlations, each in
+    # [-1, 1]: its scale per shot is 1, whatever cancellation leaves of the sum.
+    print(f"misfit {mf:.9f} vs {omf:.9f}")
+    assert abs(mf - omf) <= TOL * max(1.0, abs(omf))
     # take_gradients zeroed the accumulators
     g2 = s.take_gradients()
     assert all(np.abs(a).max() == 0 for a in g2)

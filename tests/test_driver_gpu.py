@@ -14,7 +14,7 @@ EXE = os.path.join(ROOT, "ds2d_amd", "bin", "ds2d_lsrtm")
 NX, NZ, PML = 96, 64, 20
 
 
-def _params(tmp, mode, extra):
+def _params(tmp, mode, extra, maxiter=2):
     vp = np.full((NX, NZ), 2500.0, np.float32)
     vp[:, NZ // 2:] = 3000.0
     vs = (vp / 1.8).astype(np.float32)
@@ -28,7 +28,7 @@ def _params(tmp, mode, extra):
     kv.update(extra)
     text = (f"10 10 {PML} {NX} {NZ} 1 100\n"
             f"{tmp / 'vp.bin'} {tmp / 'vs.bin'} {tmp / 'vz.su'} {tmp / 'vx.su'} {img}\n"
-            f"15 2\n2 2\n0 0 1 1\n" + "\n".join(f"{k}={v}" for k, v in kv.items()) + "\n")
+            f"15 {maxiter}\n2 2\n0 0 1 1\n" + "\n".join(f"{k}={v}" for k, v in kv.items()) + "\n")
     p = tmp / f"par_{mode}_{extra.get('opt', 'x')}.txt"
     p.write_text(text)
     return p, img
@@ -94,3 +94,28 @@ def test_driver_plain_cg_matches_library(tmp_path):
     assert len(res) == 3 and res[2] < res[1] < res[0] == pytest.approx(1.0)
     np.testing.assert_allclose(res, res_lib, rtol=1e-4)
     assert np.linalg.norm(im - im_lib) <= 1e-4 * np.linalg.norm(im_lib)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(EXE), reason="driver not built")
+def test_driver_adam_matches_oracle_adam(tmp_path):
+    """The driver's Adam (lsrtm_all_iteration, main.cpp:1092-1130) over three
+    iterations equals the oracle's FP32 Adam applied to the summed gradients
+    the driver wrote (all_grad_* = g + b_g + b_g2; adding zeros to the sum
+    is exact), with the moments carried across iterations."""
+    import oracle_lib as orc
+    par, img = _params(tmp_path, "ref", {"opt": "adam", "batch": "3"}, maxiter=3)
+    _run("synth", par)
+    _run(par)
+    A = NX * NZ
+    z = np.zeros(A, np.float32)
+    for comp in ("pp", "ps"):
+        s0, r0, im = z.copy(), z.copy(), z.copy()
+        for it in range(3):
+            g = np.fromfile(str(img / f"all_grad_{comp}_{it}_{NX}_{NZ}.dat"), np.float32)
+            got = np.fromfile(str(img / f"sum_image_{comp}_{it}_{NX}_{NZ}.dat"), np.float32)
+            assert np.abs(g).max() > 0
+            s0, r0, im = orc.adam_update(g, z, z, s0, r0, im, it, prec="f")
+            err = float(np.linalg.norm(got - im) / np.linalg.norm(im))
+            print(f"adam {comp} iter {it}: rel {err:.2e}, max |step| {np.abs(im).max():.3e}")
+            assert err <= 1e-6, (comp, it)

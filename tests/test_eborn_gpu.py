@@ -65,8 +65,7 @@ def test_eborn_forward_and_adjoint_match_oracle(bg_mode):
         off += n
     epp, eps = orc.rel_l2(gpp.ravel(), ogpp), orc.rel_l2(gps.ravel(), ogps)
     print(f"bg_mode={bg_mode} gradient rel pp {epp:.2e} ps {eps:.2e}")
-    tol = TOL if bg_mode == 1 else 1e-4
-    assert epp <= tol and eps <= tol
+    assert epp <= TOL and eps <= TOL
     # adjoint consistency of the GPU pair itself
     lhs = float(rz.astype(np.float64) @ res_z + rx.astype(np.float64) @ res_x)
     rhs = float(pp.ravel().astype(np.float64) @ gpp.ravel() + ps.ravel().astype(np.float64) @ gps.ravel())

@@ -60,10 +60,12 @@ def test_c3_window_at_full_size_matches_oracle():
 
 
 def test_c2_born_data_at_full_size_matches_oracle():
-    """Background and Born records of shots 0 and 13.  Over 2500 steps through
-    the +-30 % random boundary the Born record (a small scattered field) carries
-    the FP32 rounding of the background increments that drive it; the bar is
-    1e-5 or twice the FP32 restatement's own error, whichever is larger."""
+    """Background and Born records of shots 0 and 13 (2500 steps through the
+    +-30 % random boundary) within 1e-5 of FP64.  An all-FP32 restatement
+    misses that bar on the Born records (1.0-1.2e-5): the FP32 rounding of
+    the background increments that drive the Born source accumulates over
+    the steps.  The GPU carries the background in FP64 and rounds each
+    increment once to FP32."""
     cfg = models.c2_acoustic_lsrtm()
     s, wav = _solver(cfg, physics=1)
     s.set_model(cfg["vp"], None, cfg["rho"])
@@ -79,4 +81,4 @@ def test_c2_born_data_at_full_size_matches_oracle():
         f32b, f32n = orc.rel_l2(fbg, obg), orc.rel_l2(fbn, obn)
         print(f"C2 shot {k}: background rel {eb:.2e}, Born rel {en:.2e} "
               f"(FP32 oracle {f32b:.2e}, {f32n:.2e})")
-        assert eb <= max(TOL, 2 * f32b) and en <= max(TOL, 2 * f32n)
+        assert eb <= TOL and en <= TOL

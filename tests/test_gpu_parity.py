@@ -117,8 +117,11 @@ def test_elsrtm_gradient(boundary, it, source_fp64):
     other wavefield FP32): all seven images and the misfit within 1e-5.  With
     every field in FP32, as the reference runs, the background's rounding
     noise is amplified through the Born sources: the oracle's own FP32
-    restatement is up to 3e-4 off FP64 there, and the GPU must stay within
-    1.5x of that restatement's error (and 1e-5 where it is below)."""
+    restatements are up to 3e-4 off FP64 there, and two equally valid FP32
+    orders (separate multiply and add; fused multiply-add, which CUDA
+    compilers emit by default) land up to 2x apart.  The GPU must stay within
+    that envelope: no further from FP64 than the worse of the two FP32
+    restatements (and within 1e-5 where both are)."""
     allnx, allnz, pml, nt = 56, 32, 20, 260
     vp, vs, _ = elastic_model(allnx, allnz, 13)
     wav = wavelet(nt)
@@ -144,6 +147,7 @@ def test_elsrtm_gradient(boundary, it, source_fp64):
     g = s.take_gradients()
     og = np.zeros(7 * allnx * allnz)
     ogf = np.zeros(7 * allnx * allnz, np.float32)
+    ogm = np.zeros(7 * allnx * allnz, np.float32)
     omf = 0.0
     off = 0
     for sh in shots:
@@ -153,10 +157,13 @@ def test_elsrtm_gradient(boundary, it, source_fp64):
                                  sh.ngx_min, grads7=og)
         ogf, _ = orc.elsrtm_shot(osh, vp, vs, None, wav, img_pp, img_ps, obs_z[off:off + n], obs_x[off:off + n], it,
                                  sh.ngx_min, grads7=ogf, prec="f")
+        ogm, _ = orc.elsrtm_shot(osh, vp, vs, None, wav, img_pp, img_ps, obs_z[off:off + n], obs_x[off:off + n], it,
+                                 sh.ngx_min, grads7=ogm, prec="f_fma")
         omf += m1
         off += n
     og = og.reshape(7, allnx, allnz)
     ogf = ogf.reshape(7, allnx, allnz)
+    ogm = ogm.reshape(7, allnx, allnz)
     names = ["pp", "ps", "illum", "b_pp", "b_ps", "b2_pp", "b2_ps"]
     for k in range(7):
         ref = og[k]
@@ -165,8 +172,10 @@ def test_elsrtm_gradient(boundary, it, source_fp64):
             continue
         e = orc.rel_l2(g[k], ref)
         e32 = orc.rel_l2(ogf[k], ref)
-        tol = TOL if source_fp64 else max(TOL, 1.5 * e32)
-        print(f"source_fp64={source_fp64} {names[k]}: rel {e:.2e} (fp32 oracle {e32:.2e}, tol {tol:.1e})")
+        e32m = orc.rel_l2(ogm[k], ref)
+        tol = TOL if source_fp64 else max(TOL, e32, e32m)
+        print(f"source_fp64={source_fp64} {names[k]}: rel {e:.2e} (fp32 oracles {e32:.2e} / fma {e32m:.2e}, "
+              f"tol {tol:.2e})")
         assert e <= tol, names[k]
     # the misfit is a function of the FP32 scatter records: same 1e-5 bar
     assert abs(mf - omf) <= TOL * max(1.0, abs(omf))
@@ -200,7 +209,7 @@ def _acou_case(nshots=3, nt=300):
 def test_born_forward_and_adjoint(bg_mode, fused):
     """bg_mode 1: background increments resident in HBM (per-cell or fused
     adjoint kernels); 2: background rebuilt by reversal through the random
-    boundary (FP32 reversal drift)."""
+    boundary (the FP64 background reverses without drift)."""
     allnx, allnz, pml, nt, vp, rho, refl, wav, shots = _acou_case()
     s = _solver(allnx, allnz, pml, nt, allnx, wav, physics=1, bg_mode=bg_mode)
     s.set_fused(fused)
@@ -220,9 +229,7 @@ def test_born_forward_and_adjoint(bg_mode, fused):
         off += n
     eg = orc.rel_l2(g.ravel(), og)
     print(f"bg_mode={bg_mode} gradient rel {eg:.2e}")
-    # reversal in FP32 drifts through the random boundary (2e-5 bound); the
-    # resident history keeps the forward increments and meets the 1e-5 bar
-    assert eg <= (TOL if bg_mode == 1 else 2e-5)
+    assert eg <= TOL
     # adjoint consistency of the GPU pair itself
     lhs = float(np.dot(rbn.astype(np.float64), res.astype(np.float64)))
     rhs = float(np.dot(refl.ravel().astype(np.float64), g.ravel().astype(np.float64)))

@@ -131,12 +131,13 @@ def test_laplace_quadratic_closed_form():
 
 
 @pytest.mark.skipif(not orc.ref_laplace_available(), reason="reference tree not mounted: _ref not built")
-def test_laplace_matches_compiled_reference():
+@pytest.mark.parametrize("shape,dx", [((40, 23), 10.0), ((251, 101), 10.0), ((512, 256), 12.5), ((5, 5), 1.0)])
+def test_laplace_matches_compiled_reference(shape, dx):
     rng = np.random.default_rng(2)
-    img = rng.standard_normal((40, 23)).astype(np.float32)
-    ref = orc.ref_laplace_order(img, 10.0)
-    ours = orc.laplace_order(img, 10.0, prec="f")
-    np.testing.assert_allclose(ours, ref, rtol=2e-6, atol=1e-7 * np.abs(ref).max())
+    img = rng.standard_normal(shape).astype(np.float32)
+    ref = orc.ref_laplace_order(img, dx)
+    ours = orc.laplace_order(img, dx, prec="f")
+    assert np.array_equal(ours, ref)  # same arithmetic order: bit-identical
     ref_c = orc.ref_cutM(img, 3, 20, 4, 35)
     ours_c = img.copy()
     ours_c[:4] = 0

@@ -172,3 +172,26 @@ def test_learned_lsrtm_beats_plain_cg_on_held_out_models():
     out0 = LR.compare(s, u0, prob, niter)
     print(f"untrained: plain {out0['plain']['resid'][-1]:.4f} learned {out0['learned']['resid'][-1]:.4f}")
     assert out0["learned"]["resid"][-1] <= out0["plain"]["resid"][0]
+
+
+def test_learned_solver_falls_back_at_once_with_a_zero_network():
+    """A U-Net whose parameters are all zero returns z = 0 for every input, so
+    the first learned direction has L p = 0 and cannot lower the residual:
+    the solver must take the plain CGLS step from the start, count no
+    learned step, and follow ds2d_lsrtm_cg's iterates."""
+    from ds2d_amd import learned as LR
+    from ds2d_amd.lib import Unet
+    kw = dict(nx=96, nz=48, nshots=6, nt=700, pml=30, dt=0.0008)
+    prob = LR.acoustic_problem(903, **kw)
+    s = LR.make_solver(prob)
+    s.set_model(prob["vp"], None, prob["rho"])
+    _, d, _ = s.born_forward(prob["shots"], prob["refl"])
+    u = Unet(levels=4, base=32, max_batch=1, max_h=kw["nx"], max_w=kw["nz"], seed=1)
+    u.set_params(np.zeros(U.num_params(4, 32), np.float32))
+    niter = 5
+    img_p, rp, _ = s.lsrtm_cg(prob["shots"], d, niter)
+    img_l, rl, st = s.lsrtm_learned(u, prob["shots"], d, niter)
+    print(f"zero network: learned steps {st['learned_steps']}, residual {list(rl)} vs CG {list(rp)}")
+    assert st["learned_steps"] == 0
+    np.testing.assert_allclose(rl, rp, rtol=1e-5)
+    assert rel(img_l, img_p) <= 1e-5
