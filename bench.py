@@ -15,8 +15,8 @@ Other workloads (--workload): c1 = single-shot elastic forward on 251x101
 preconditioner: training pairs from the plain method, U-Net training, and
 plain vs learned LSRTM time to the residual target on held-out models;
 c5 = PP/PS elastic Born CG LSRTM of 256 uneven windows on 4000x1000 (a step
-= one CG iteration), with the FP64-oracle comparison on a 500x250 window;
-unet = U-Net training throughput.
+= one CG iteration); unet = U-Net training throughput; compat = the
+reference-compat ELSRTM gradient main.cpp runs (26 shots on the C1 model).
 
 Prints one JSON line: value = Gcell-updates/s (one cell-update = one padded
 cell advanced one step for one wavefield); roofline of the dominant kernel
@@ -43,9 +43,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "unet"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "unet", "compat"])
     ap.add_argument("--c5-iter", type=int, default=2, help="c5: CG iterations on the full configuration")
-    ap.add_argument("--no-window-parity", action="store_true", help="c5: skip the FP64-oracle window comparison")
     ap.add_argument("--train-models", type=int, default=64, help="c4: training models (seeds 1000..)")
     ap.add_argument("--eval-models", type=int, default=4, help="c4: held-out models (seeds eval-start..)")
     ap.add_argument("--eval-start", type=int, default=2000, help="c4: seed of the first held-out model")
@@ -60,10 +59,12 @@ def parse():
     ap.add_argument("--mode", type=int, default=2, help="2 fused kernel (default), 1 two fast kernels, 0 reference layout")
     ap.add_argument("--graph", action="store_true", help="capture each time loop in one hipGraph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=5.0, help="CPU baseline: seconds per timed run (3 runs)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "k_fused_step_c3_traffic.json"),
                     help="rocprof FETCH/WRITE summary (JSON) to embed as roofline.traffic when its kernel and "
                          "workload match the run")
+    ap.add_argument("--time-block", type=int, default=0, choices=[0, 1, 2],
+                    help="mode 2: time steps per launch (0 = what the autotuner picked, or 1)")
     ap.add_argument("--tile", default=None,
                     help="mode 2: fixed fused tile 'width,threads,depth' instead of the autotuner")
     ap.add_argument("--copy-probe", action="store_true",
@@ -77,8 +78,39 @@ def _orc():
     return orc
 
 
+def host_info():
+    """CPU model and the logical CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    return model, nproc
+
+
+def _cpu_sample(run, nt, probe, seconds, repeats=3):
+    """Steps that fit ~seconds (from a probe run), then the best of `repeats` runs."""
+    tp = run(probe)
+    steps = int(min(nt, max(probe, probe * seconds / max(tp, 1e-6))))
+    return steps, min(run(steps) for _ in range(repeats))
+
+
+def _cpu_record(value, threads, sample):
+    model, nproc = host_info()
+    return {"value": value, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "sample": sample, "cpu_model": model, "nproc": nproc, "repeats": "best of 3"}
+
+
 def cpu_baseline_elastic(cfg, seconds, wav):
-    """FP32 oracle elastic forward on shot 0 for as many steps as fit ~seconds."""
+    """FP32 oracle elastic forward on shot 0: as many steps as fit ~seconds, best of 3."""
     orc = _orc()
     threads = orc.threads(0)
     shot = cfg["shots"][0]
@@ -90,18 +122,14 @@ def cpu_baseline_elastic(cfg, seconds, wav):
         orc.elastic_forward(o, cfg["vp"], cfg["vs"], cfg["rho"], wav[:nsteps], prec="f")
         return time.perf_counter() - t
 
-    probe = min(40, cfg["nt"])
-    tp = run(probe)
-    steps = int(min(cfg["nt"], max(probe, probe * seconds / max(tp, 1e-6))))
-    t = run(steps)
-    return {"value": round(nxp * nzp * steps / t / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"FP32 oracle elastic forward, shot 0 only ({shot.nx}-column window, {nxp}x{nzp} padded), "
-                      f"{steps} of {cfg['nt']} steps, {t:.1f} s, OpenMP threads={threads}"}
+    steps, t = _cpu_sample(run, cfg["nt"], min(40, cfg["nt"]), seconds)
+    return _cpu_record(round(nxp * nzp * steps / t / 1e9, 4), threads,
+                       f"FP32 oracle elastic forward, shot 0 only ({shot.nx}-column window, {nxp}x{nzp} padded), "
+                       f"{steps} of {cfg['nt']} steps, best of 3 runs: {t:.2f} s, OpenMP threads={threads}")
 
 
 def cpu_baseline_born(cfg, seconds, wav):
-    """FP32 oracle plain Born forward (background + Born) on shot 0."""
+    """FP32 oracle plain Born forward (background + Born) on shot 0, best of 3."""
     orc = _orc()
     threads = orc.threads(0)
     shot = cfg["shots"][0]
@@ -113,14 +141,10 @@ def cpu_baseline_born(cfg, seconds, wav):
         orc.acou_forward(o, cfg["vp"], cfg["rho"], cfg["refl"], wav[:nsteps], prec="f")
         return time.perf_counter() - t
 
-    probe = min(100, cfg["nt"])
-    tp = run(probe)
-    steps = int(min(cfg["nt"], max(probe, probe * seconds / max(tp, 1e-6))))
-    t = run(steps)
-    return {"value": round(2 * nxp * nzp * steps / t / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"FP32 oracle acoustic background+Born forward, shot 0 ({nxp}x{nzp} padded), {steps} of "
-                      f"{cfg['nt']} steps, {t:.1f} s, OpenMP threads={threads}"}
+    steps, t = _cpu_sample(run, cfg["nt"], min(100, cfg["nt"]), seconds)
+    return _cpu_record(round(2 * nxp * nzp * steps / t / 1e9, 4), threads,
+                       f"FP32 oracle acoustic background+Born forward, shot 0 ({nxp}x{nzp} padded), {steps} of "
+                       f"{cfg['nt']} steps, best of 3 runs: {t:.2f} s, OpenMP threads={threads}")
 
 
 def roofline(stats):
@@ -160,6 +184,9 @@ def run_forward(a, cfg):
         s.set_chunk(chunk, threads, depth)
     elif a.mode == 2:  # untimed: pick the fused kernel's tile (or the two-kernel step) for this batch
         chunk, threads, depth, _ = s.autotune(cfg["shots"], nsteps=min(120, cfg["nt"]))
+    if a.mode == 2 and a.time_block:
+        s.set_time_block(a.time_block)
+    tblock = s.time_block() if a.mode == 2 and chunk else 1
     for _ in range(a.warmup):
         s.batch_forward(cfg["shots"], mode=a.mode, fetch=False)
     stats = []
@@ -179,9 +206,11 @@ def run_forward(a, cfg):
         "config": {
             "workload": cfg["desc"], "shots": len(cfg["shots"]), "nt": cfg["nt"], "cell_updates_per_step": cu,
             "kernels": KERNELS[1 if (a.mode == 2 and chunk == 0) else a.mode] +
-                       (" (autotuned)" if a.mode == 2 else ""), "parallelism": "shots batched on one GPU",
+                       (", two time steps per launch (temporal blocking)" if tblock == 2 else "") +
+                       (" (autotuned)" if a.mode == 2 and not a.tile else ""), "parallelism": "shots batched on one GPU",
             "autotuned_tile": None if chunk is None else {"x_width": chunk, "threads": threads,
-                                                          "prefetch_columns": depth},
+                                                          "prefetch_columns": depth,
+                                                          "time_steps_per_launch": tblock},
             "hipgraph": a.graph},
         "roofline": roofline(stats),
     }
@@ -375,9 +404,8 @@ def run_c5(a):
     windows on the 4000x1000 model, all shots batched.  Plain CG for
     --c5-iter iterations (and the learned solver with --load-unet, the U-Net
     applied to the PP and PS gradients): residual history, seconds per
-    iteration, Gcell-updates/s.  Then the same solver on a 500x250 window of
-    the model (4 shots, 2 iterations) against CGLS driven by the FP64 oracle:
-    image error vs the oracle."""
+    iteration, Gcell-updates/s.  (The same solver on a 500x250 window of
+    the model against CGLS on the FP64 oracle is a GPU test.)"""
     from ds2d_amd import Solver, Unet, models
     from ds2d_amd.lib import ricker
 
@@ -416,7 +444,6 @@ def run_c5(a):
                       "image_rel_err_vs_true_reflectivity": round(float(np.linalg.norm(img - ref) /
                                                                         np.linalg.norm(ref)), 4)}
         log(f"c5 {name}: residual {res[-1]:.4f} after {a.c5_iter} iterations, {el:.1f} s")
-    window = None if a.no_window_parity else c5_window_parity(cfg, log)
     p = runs["plain"]
     return {"metric": "Gcell-updates/s/GPU", "value": p["gcell_per_s"], "unit": "Gcell-updates/s", "n_gpus": 1,
             "steps": a.c5_iter, "warmup": 0, "ms_per_step": round(p["wall_s"] * 1e3 / max(a.c5_iter, 1), 1),
@@ -428,66 +455,58 @@ def run_c5(a):
                                    f"{cfg['nt']} steps, PP/PS elastic Born CG LSRTM",
                        "padded_cells_per_wavefield": int(sum((x.nx + 2 * cfg["pml"]) * (cfg["nz"] + 2 * cfg["pml"])
                                                              for x in shots)),
-                       "born_data_s": round(t_fwd, 2), "solvers": runs, "window_parity": window,
+                       "born_data_s": round(t_fwd, 2), "solvers": runs,
                        "note": "a step = one CG iteration over all shots; 1e-2 is not reached in this many "
                                "iterations (see resid)"}}
 
 
-def c5_window_parity(cfg, log):
-    """GPU PP/PS CG vs CGLS on the FP64 oracle's Born pair, 500x250 window of
-    the C5 model, 4 full-window shots, 800 steps, 2 iterations."""
-    from ds2d_amd import Solver
+def run_compat(a):
+    """The reference-compat per-shot gradient that main.cpp runs
+    (ELSRTM_ROT_EXP_IMAGE_SINGLESHOT through ds2d_batch_gradient): 26 shots
+    10 nodes apart on the C1 model (251x101, pml 20, dt 0.8 ms, 2500 steps),
+    iteration 1 (Born + second-order Born, NCC residual, source-side
+    reconstruction through the saved PML edge bands, receiver adjoint,
+    receiver-side Born adjoint, the three imaging terms), constant density
+    2000 as the reference.  A step = one gradient over all shots.
+    Cell-updates per padded cell and time step: bg, b, b2 forward; bg, b
+    reconstructed; bk, bb adjoint = 7 wavefields."""
+    from ds2d_amd import Solver, models
     from ds2d_amd.lib import ricker
-    from ds2d_amd.models import surface_shot
-    orc = _orc()
-    x0, z0, nx, nz, nt, pml, niter = 1750, 0, 500, 250, 800, 40, 2
-    sl = (slice(x0, x0 + nx), slice(z0, z0 + nz))
-    vp, vs, rho = (np.ascontiguousarray(cfg[k][sl]) for k in ("vp", "vs", "rho"))
-    rpp, rps = (np.ascontiguousarray(cfg[k][sl]) for k in ("refl_pp", "refl_ps"))
+    cfg = models.c1_elastic_forward()
+    nx, nz, pml, nt = cfg["nx"], cfg["nz"], cfg["pml"], a.nt or cfg["nt"]
+    shots = [models.surface_shot(nx, 0, pml, min(10 * i, nx - 1), 2, 0, shot_id=i, boundary=0)
+             for i in range(a.nshots or 26)]
     wav = ricker(nt, cfg["f0"], cfg["dt"])
-    shots = [surface_shot(nx, 0, pml, 62 + 125 * i, 2, 1, shot_id=i, boundary=1) for i in range(4)]
     s = Solver(dx=cfg["dx"], dz=cfg["dz"], dt=cfg["dt"], nt=nt, allnx=nx, allnz=nz, pml=pml, nx_max=nx,
-               wavelet=wav, physics=0)
-    s.set_model(vp, vs, rho)
-    dz, dx, _ = s.eborn_forward(shots, rpp, rps)
-    gpp, gps, res, _ = s.elsrtm_cg(shots, dz, dx, niter)
+               wavelet=wav, pml_rr=1e-3, rho_const=2000.0)
+    # observed data: elastic forward of a 3 % faster model (untimed)
+    s.set_model(cfg["vp"] * np.float32(1.03), cfg["vs"], None)
+    oz, ox, _ = s.batch_forward(shots, mode=2)
+    s.set_model(cfg["vp"], cfg["vs"], None)
+    rng = np.random.default_rng(4)
+    s.set_image((0.02 * rng.standard_normal((nx, nz))).astype(np.float32),
+                (0.02 * rng.standard_normal((nx, nz))).astype(np.float32))
+    for _ in range(a.warmup):
+        s.batch_gradient(shots, oz, ox, 1)
+        s.take_gradients()
     t0 = time.perf_counter()
-    osh = [orc.make_shot(x, nx, nz, pml, nt, cfg["dx"], cfg["dz"], cfg["dt"]) for x in shots]
-    n = [x.ntr * nt for x in shots]
-    A, R = nx * nz, sum(n)
-
-    def L(m):
-        out = [orc.eborn_forward(o, vp, vs, rho, m[:A].reshape(nx, nz), m[A:].reshape(nx, nz), wav) for o in osh]
-        return np.concatenate([z for z, _ in out] + [x for _, x in out])
-
-    def LT(r):
-        g, off = np.zeros(2 * A), 0
-        for o, k in zip(osh, n):
-            a_, b_ = orc.eborn_adjoint(o, vp, vs, rho, wav, r[off:off + k], r[R + off:R + off + k])
-            g[:A] += a_
-            g[A:] += b_
-            off += k
-        return g
-
-    d = np.concatenate([dz, dx]).astype(np.float64)
-    x, r = np.zeros(2 * A), d.copy()
-    sv = LT(r)
-    p, gamma, hist = sv.copy(), float(sv @ sv), [1.0]
-    for _ in range(niter):
-        q = L(p)
-        alpha = gamma / float(q @ q)
-        x += alpha * p
-        r -= alpha * q
-        hist.append(float(np.linalg.norm(r) / np.linalg.norm(d)))
-        sv = LT(r)
-        gnew = float(sv @ sv)
-        p, gamma = sv + (gnew / gamma) * p, gnew
-    img = np.concatenate([gpp.ravel(), gps.ravel()])
-    err = float(np.linalg.norm(img - x) / np.linalg.norm(x))
-    log(f"c5 window parity: image rel err {err:.2e} vs FP64 oracle ({time.perf_counter() - t0:.1f} s CPU)")
-    return {"window": f"x {x0}..{x0 + nx}, z {z0}..{z0 + nz}", "shots": 4, "nt": nt, "iterations": niter,
-            "image_rel_err_vs_fp64_oracle": err, "resid_gpu": [float(v) for v in res], "resid_oracle": hist,
-            "oracle_threads": orc.threads(0)}
+    for _ in range(a.steps):
+        s.batch_gradient(shots, oz, ox, 1)
+    elapsed = time.perf_counter() - t0
+    g = s.take_gradients()
+    assert all(np.isfinite(x).all() for x in g) and np.abs(g[0]).max() > 0
+    cells = len(shots) * (nx + 2 * pml) * (nz + 2 * pml)
+    cu = 7 * cells * nt
+    return {"metric": "Gcell-updates/s/GPU", "value": round(cu * a.steps / elapsed / 1e9, 3),
+            "unit": "Gcell-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": round(elapsed / a.steps * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 (background and first-order Born field f64)",
+            "data": "synthetic: seeded layered+faulted elastic model (seed 1), observed data from a 3 % faster model",
+            "config": {"workload": f"compat_gradient: 251x101, dx=dz=10 m, pml 20, dt 0.8 ms, {nt} steps, "
+                                   f"{len(shots)} shots 10 nodes apart, reference ELSRTM gradient (iteration 1)",
+                       "shots": len(shots), "nt": nt, "cell_updates_per_step": cu,
+                       "kernels": "reference split-PML layout, per-cell kernels (ds2d_batch_gradient)",
+                       "parallelism": "shots batched on one GPU"}}
 
 
 def run_unet(a):
@@ -560,6 +579,8 @@ def main():
         out = run_c5(a)
     elif a.workload == "unet":
         out = run_unet(a)
+    elif a.workload == "compat":
+        out = run_compat(a)
     else:
         cfg = models.c2_acoustic_lsrtm(nt=a.nt or 2500, nshots=a.nshots or 26)
         cfg["seed"] = 2
@@ -572,12 +593,16 @@ def main():
         tr = json.load(open(a.traffic_json))
         cfgd = out.get("config", {})
         tile = cfgd.get("autotuned_tile") or {}
-        key = [tile.get("x_width"), tile.get("threads"), tile.get("prefetch_columns")]
+        key = [tile.get("x_width"), tile.get("threads"), tile.get("prefetch_columns"),
+               tile.get("time_steps_per_launch", 1)]
         # per-launch traffic depends on the batch and the tile, not on the number of steps
         if (tr.get("workload_prefix") and str(cfgd.get("workload", "")).startswith(tr["workload_prefix"])
                 and tr.get("shots") == cfgd.get("shots")):
             for e in tr.get("entries", [tr]):
-                if e.get("tile") == key:
+                t = list(e.get("tile") or [])
+                if len(t) == 3:  # entries without a time block are one step per launch
+                    t.append(1)
+                if t == key:
                     out["roofline"]["traffic"] = e.get("traffic_bytes_per_launch")
     print(json.dumps(out), flush=True)
 

@@ -78,11 +78,14 @@ struct ds2d_ctx {
     int chunk_w = 0;    // fused-kernel x-chunk width (0: default)
     int chunk_t = 0;    // fused-kernel workgroup size (256/512/704/768; 0: default)
     int chunk_d = 0;    // fused-kernel prefetch depth in columns (2 or 4; 0: default 2)
+    int tblock = 1;     // fused elastic forward: time steps per launch (1 or 2)
+    double stop_resid = 0;  // CG solvers stop once the relative data residual is at most this (0: never)
     int use_graph = 0;  // capture time loops into one hipGraph per loop
     int per_launch = 0; // plain Born/adjoint: shots per time loop (0: all)
     int fwd_two = 0;    // autotuned: mode 2 runs the two-kernel step instead of the fused one
     int acou_fused = -1;  // fused Born/adjoint steps: -1 auto, 0 never, 1 whenever eligible
     int src_fp64 = 1;     // reference-compat gradient: background and first-order Born field in FP64
+    std::vector<int> shot_status;  // per shot of the last reference-compat gradient: 1 = left out (non-finite)
     // (cells, slots, shots per launch) -> 0 per-cell kernels, else fused x width
     std::map<std::tuple<long long, int, int>, int> acou_tuned;
 
@@ -578,9 +581,18 @@ int batch_gradient_impl(ds2d_t* h, int nshots, const ds2d_shot* shots, const flo
     std::vector<double> hs((size_t)B.total_traces * 6);
     if (!hs.empty()) HIP_TRY(hipMemcpyAsync(hs.data(), sums, hs.size() * 8, hipMemcpyDeviceToHost, st));
     launch_ncc_apply(st, sh, rs_z, rs_x, ob_z, ob_x, sums, d_tb, iter);
+    // per-shot failure detection: a shot whose residual (and so its adjoint
+    // images) is not finite is left out of the misfit and the gathered images
+    int* d_bad;
+    if ((rc = h->fbuf("shotbad", (size_t)nshots, &d_bad))) return rc;
+    HIP_TRY(hipMemsetAsync(d_bad, 0, (size_t)nshots * 4, st));
+    launch_slot_nonfinite(st, sh, rs_z, (long long)R, 2, 0, d_bad);
+    std::vector<int> bad(nshots, 0);
+    HIP_TRY(hipMemcpyAsync(bad.data(), d_bad, (size_t)nshots * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (misfit && iter != 0) {
         for (int s = 0; s < nshots; ++s) {
+            if (bad[s]) continue;
             const int ntr = shots[s].ntr;
             for (int r = 0; r < ntr; ++r) {
                 const double* q = &hs[(size_t)(B.tbase[s] + r) * 6];
@@ -609,9 +621,17 @@ int batch_gradient_impl(ds2d_t* h, int nshots, const ds2d_shot* shots, const flo
         launch_image(st, sh, sg, sb, bk, bb, bg, m, win, W);
     }
     const long long A = (long long)h->g.allnx * h->g.allnz;
-    launch_gather(st, sh, win, W, h->d_grad7, A, 7, h->g.allnx, h->g.allnz);
+    launch_slot_nonfinite(st, sh, win, W, 7, 1, d_bad);
+    launch_gather(st, sh, win, W, h->d_grad7, A, 7, h->g.allnx, h->g.allnz, d_bad);
+    HIP_TRY(hipMemcpyAsync(bad.data(), d_bad, (size_t)nshots * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipGetLastError());
+    h->shot_status.assign(bad.begin(), bad.end());
+    int nbad = 0;
+    for (int s = 0; s < nshots; ++s) nbad += bad[s] != 0;
+    if (nbad)
+        return fail(DS2D_ESHOT, "%d of %d shots gave non-finite residuals or images and were left out of the misfit "
+                    "and the gradients (ds2d_shot_status lists them)", nbad, nshots);
     return 0;
 }
 
@@ -693,6 +713,16 @@ int ds2d_shot_gradient(ds2d_t* h, int iter, double* misfit) {
     if ((rc = check_h(h))) return rc;
     if (!h->have_shot) return fail(DS2D_ESTATE, "no shot set");
     return ds2d_batch_gradient(h, 1, &h->cur, h->cur_oz.data(), h->cur_ox.data(), iter, misfit);
+}
+
+int ds2d_shot_status(ds2d_t* h, int nshots, int* status) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!status) return fail(DS2D_EINVAL, "null status");
+    if (nshots != (int)h->shot_status.size())
+        return fail(DS2D_EINVAL, "the last gradient had %zu shots, not %d", h->shot_status.size(), nshots);
+    std::copy(h->shot_status.begin(), h->shot_status.end(), status);
+    return 0;
 }
 
 int ds2d_take_gradients(ds2d_t* h, float* pp, float* ps, float* il, float* bpp, float* bps, float* b2pp,
@@ -804,6 +834,48 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
             return q;
         };
         const FusedArgs even = mk(f, g2), odd = mk(g2, f);
+        if (h->tblock == 2 && nt >= 2) {
+            // temporal blocking of depth 2: steps (it, it+1) per launch; an odd
+            // last step runs the single-step kernel on its own chunk list
+            const int T2 = fused2_threads_ok(T) ? T : 704;
+            const int D2 = D >= 2 ? 2 : 1;
+            std::vector<ChunkDesc> ch2;
+            const int rows2 = T2 - 2 * FUSED2_HALO;
+            for (int s = 0; s < nshots; ++s)
+                for (int x0 = 0; x0 < B.slots[s].nxp; x0 += W)
+                    for (int z0 = 0; z0 < B.slots[s].nzp; z0 += rows2)
+                        ch2.push_back(ChunkDesc{s, x0, std::min(x0 + W, B.slots[s].nxp), z0});
+            ChunkDesc* d_ch2;
+            if ((rc = h->fbuf("chunks2", ch2.size(), &d_ch2))) return rc;
+            HIP_TRY(hipMemcpyAsync(d_ch2, ch2.data(), ch2.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice, st));
+            float* scratch = nullptr;
+            if (B.any_pml && (rc = h->fbuf("f2split", (size_t)n * 5, &scratch))) return rc;
+            int k = 0;  // launch parity: even launches read f, odd ones read g2
+            auto two = [&](int it) {
+                launch_fused2_step(st, sh, (k & 1) ? odd : even, scratch, d_ch2, (int)ch2.size(), rmap, it, T2, D2,
+                                   B.any_pml);
+                ++k;
+            };
+            stats_loop_begin(h);
+            int it = 0;
+            for (; it + 1 < nt; it += 2) {
+                const bool tk = smp.take(it);
+                if (tk) smp.mark(st, 0);
+                two(it);
+                if (tk) { smp.mark(st, 1); smp.mark(st, 2); }
+            }
+            if (it < nt) {
+                launch_fused_step(st, sh, (k & 1) ? odd : even, d_ch, (int)ch.size(), rmap, it, T, D, B.any_pml);
+                ++k;
+            }
+            stats_loop_end(h);
+            launches = (long long)(nt / 2);
+            // per padded cell and launch (two steps): the unsplit state and parameters once (13 floats);
+            // each PML cell reads dampx, dampz twice and moves 4 x 5 splits (A read, scratch write + read,
+            // B write) = 24 floats
+            bytes_v = (double)B.total_cells * 13 * 4 + pml_cells * 24 * 4;
+            bytes_s = 0;
+        } else {
         stats_loop_begin(h);
         if (h->use_graph) {
             // whole time loop as one graph; per-launch time = loop time / nt
@@ -825,6 +897,7 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
         // Each PML cell also reads dampx, dampz and reads + writes one split of each of the 5 components.
         bytes_v = (double)B.total_cells * 13 * 4 + pml_cells * 12 * 4;
         bytes_s = 0;
+        }
     } else {
         float* par;
         int* rmap;
@@ -888,6 +961,22 @@ int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec
     return batch_forward_n(h, nshots, shots, rec_z, rec_x, mode, stats, h ? h->g.nt : 0);
 }
 
+int ds2d_set_time_block(ds2d_t* h, int steps) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (steps != 1 && steps != 2) return fail(DS2D_EINVAL, "time steps per launch must be 1 or 2, got %d", steps);
+    h->tblock = steps;
+    return 0;
+}
+
+int ds2d_get_time_block(ds2d_t* h, int* steps) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!steps) return fail(DS2D_EINVAL, "null argument");
+    *steps = h->tblock;
+    return 0;
+}
+
 int ds2d_set_graph(ds2d_t* h, int on) {
     int rc;
     if ((rc = check_h(h))) return rc;
@@ -901,7 +990,7 @@ int ds2d_set_chunk(ds2d_t* h, int width, int threads, int depth) {
     if (width < 0 || (width > 0 && width < 8)) return fail(DS2D_EINVAL, "chunk width must be 0 or >= 8");
     if (threads != 0 && !fused_threads_ok(threads))
         return fail(DS2D_EINVAL, "threads must be 0, 256, 512, 704 or 768");
-    if (depth != 0 && depth != 2 && depth != 4) return fail(DS2D_EINVAL, "depth must be 0, 2 or 4");
+    if (depth != 0 && depth != 1 && depth != 2 && depth != 4) return fail(DS2D_EINVAL, "depth must be 0, 1, 2 or 4");
     h->chunk_w = width;
     h->chunk_t = threads;
     h->chunk_d = depth;
@@ -917,11 +1006,12 @@ int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nst
     int rc;
     if ((rc = check_h(h))) return rc;
     struct Cand {
-        int w, t, d;
+        int w, t, d, tb;  // x width (0: two-kernel step), lanes, prefetch columns, time steps per launch
     };
-    static const Cand cand[] = {{0, 0, 0},     {32, 256, 2},  {64, 256, 2},  {64, 256, 4},
-                                {128, 256, 4}, {64, 512, 4},  {64, 768, 2},  {64, 768, 4},
-                                {128, 768, 4}, {64, 704, 4},  {112, 704, 4}, {160, 704, 4}};
+    static const Cand cand[] = {{0, 0, 0, 1},     {32, 256, 2, 1},  {64, 256, 2, 1},  {64, 256, 4, 1},
+                                {128, 256, 4, 1}, {64, 512, 4, 1},  {64, 768, 2, 1},  {64, 768, 4, 1},
+                                {128, 768, 4, 1}, {64, 704, 4, 1},  {112, 704, 4, 1}, {160, 704, 4, 1},
+                                {64, 704, 1, 2},  {128, 704, 1, 2}, {128, 704, 2, 2}};
     std::vector<Cand> cands(std::begin(cand), std::end(cand));
     // 704-lane tiles whose chunk count just fills R rounds of one workgroup per CU
     // (R = 2, 3, 4): the step time moves with the number of rounds, not with the width alone
@@ -947,17 +1037,23 @@ int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nst
             }
             bool have = false;
             for (const Cand& c : cands) have = have || (c.w == lo && c.t == 704 && c.d == 4);
-            if (!have) cands.push_back(Cand{lo, 704, 4});
+            if (!have) cands.push_back(Cand{lo, 704, 4, 1});
+            // the same widths two steps per launch (704 lanes own 672 rows; 640-row windows stay one block)
+            cands.push_back(Cand{lo, 704, 1, 2});
+            cands.push_back(Cand{lo, 704, 2, 2});
         }
     }
-    Cand best{64, 256, 2};
+    Cand best{64, 256, 2, 1};
     double bt = 1e300;
-    const int keep_w = h->chunk_w, keep_t = h->chunk_t, keep_d = h->chunk_d, keep_two = h->fwd_two;
+    const int keep_w = h->chunk_w, keep_t = h->chunk_t, keep_d = h->chunk_d, keep_two = h->fwd_two,
+              keep_tb = h->tblock;
     for (const Cand& c : cands) {
+        if (c.tb == 2 && nsteps < 2) continue;
         h->fwd_two = c.w == 0;
         h->chunk_w = c.w;
         h->chunk_t = c.t;
         h->chunk_d = c.d;
+        h->tblock = c.tb;
         ds2d_stats s{};
         double t = 1e300;  // best of two timed runs after one warm run
         for (int rep = 0; rep < 3; ++rep) {
@@ -966,6 +1062,7 @@ int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nst
                 h->chunk_t = keep_t;
                 h->chunk_d = keep_d;
                 h->fwd_two = keep_two;
+                h->tblock = keep_tb;
                 return rc;
             }
             if (rep > 0) t = std::min(t, s.stencil_ms);
@@ -979,6 +1076,7 @@ int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nst
     h->chunk_w = best.w;
     h->chunk_t = best.t;
     h->chunk_d = best.d;
+    h->tblock = best.tb;
     if (best_width) *best_width = best.w;
     if (best_threads) *best_threads = best.t;
     if (best_depth) *best_depth = best.d;
@@ -1068,6 +1166,8 @@ constexpr int ACOU_CHUNK_W = 64, ACOU_THREADS = 256, ACOU_TUNE_STEPS = 48;
 
 // Chunk list of the fused acoustic steps for x width W (per launch group).
 int acou_build_chunks(ds2d_ctx* h, const Batch& B, AcouWS& w, int W) {
+    if (W < 1 || W > ACOU_FUSED_MAX_W) return fail(DS2D_EINVAL, "fused acoustic x width %d outside [1, %d]", W,
+                                                   ACOU_FUSED_MAX_W);
     std::vector<ChunkDesc> ch;
     const int ns = (int)B.slots.size();
     const int gs = h->per_launch > 0 ? std::min(h->per_launch, ns) : ns;
@@ -1240,7 +1340,7 @@ void acou_tune(ds2d_ctx* h, const Batch& B, AcouWS& w) {
             return k;
         };
         for (int R = 1; R <= 3; ++R) {
-            int lo = 8, hi = max_nxp;  // smallest W with nchunks(W) <= R * ncu
+            int lo = 8, hi = std::min(max_nxp, ACOU_FUSED_MAX_W);  // smallest W with nchunks(W) <= R * ncu
             if (nchunks(hi) > (long long)R * ncu) continue;
             while (lo < hi) {
                 const int mid = (lo + hi) / 2;
@@ -1352,6 +1452,16 @@ long long padded_cells(const Batch& B) {
     return cu;
 }
 
+// Stop rule of the CG solvers (ds2d_set_stop): once the relative residual
+// after iteration k is at most the target, the remaining entries of resid
+// repeat it and the solver returns.
+bool reached(const ds2d_ctx* h, double rr, double dd, double* resid, int k, int niter) {
+    if (!(h->stop_resid > 0) || std::sqrt(rr / dd) > h->stop_resid) return false;
+    if (resid)
+        for (int j = k + 1; j < niter; ++j) resid[j + 1] = resid[k + 1];
+    return true;
+}
+
 // CGLS iterations on ||L x - d||^2 (image of N floats, records of Rn).  On
 // entry: x, r = d - L x, rr = |r|^2, dd = |d|^2 and s = L^T r.  born(p)
 // leaves L p in q; adjoint(r) leaves L^T r in s.  resid[k] = |r_k| / |d|.
@@ -1378,6 +1488,7 @@ int cgls_loop(ds2d_ctx* h, long long N, long long Rn, float* x, float* r, float*
         rr = host_dot(h, r, r, Rn, &rc);
         if (rc) return rc;
         if (resid) resid[k + 1] = std::sqrt(rr / dd);
+        if (reached(h, rr, dd, resid, k, niter)) break;
         if ((rc = adjoint(r))) return rc;  // s = L^T r
         const double gnew = host_dot(h, s, s, N, &rc);
         if (rc) return rc;
@@ -1504,9 +1615,16 @@ int ds2d_lsrtm_cg(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* d_
 
 int ds2d_set_fused(ds2d_t* h, int mode) {
     if (!h) return fail(DS2D_EINVAL, "null handle");
-    if (mode < -1 || (mode > 1 && mode < 8) || mode > 4096)
-        return fail(DS2D_EINVAL, "mode must be -1, 0, 1 or an x width in [8, 4096], got %d", mode);
+    if (mode < -1 || (mode > 1 && mode < 8) || mode > ACOU_FUSED_MAX_W)
+        return fail(DS2D_EINVAL, "mode must be -1, 0, 1 or an x width in [8, %d], got %d", ACOU_FUSED_MAX_W, mode);
     h->acou_fused = mode;
+    return 0;
+}
+
+int ds2d_set_stop(ds2d_t* h, double target) {
+    if (!h) return fail(DS2D_EINVAL, "null handle");
+    if (!(target >= 0) || target >= 1) return fail(DS2D_EINVAL, "stop target must be in [0, 1), got %g", target);
+    h->stop_resid = target;
     return 0;
 }
 
@@ -1900,6 +2018,7 @@ int learned_cg(ds2d_ctx* h, Precond& P, int nch, long long A, long long Rn, floa
                 if (rc) return rc;
                 if (resid) resid[k + 1] = std::sqrt(rr / dd);
                 ++accepted;
+                if (reached(h, rr, dd, resid, k, niter)) break;
                 if ((rc = adjoint(r, gn))) return rc;
                 if ((rc = P.apply_n(h, nch, gn, zn))) return rc;
                 const double zg_new = dot(zn, gn, N), zg_old = dot(zn, g, N);
@@ -1931,6 +2050,7 @@ int learned_cg(ds2d_ctx* h, Precond& P, int nch, long long A, long long Rn, floa
         rr = dot(r, r, Rn);
         if (rc) return rc;
         if (resid) resid[k + 1] = std::sqrt(rr / dd);
+        if (reached(h, rr, dd, resid, k, niter)) break;
         if ((rc = adjoint(r, g))) return rc;
         const double gnew = dot(g, g, N);
         if (rc) return rc;

@@ -67,7 +67,10 @@ void launch_combine_damp(hipStream_t st, const BatchShape& b, const float* dampx
 // Deterministic gather of per-slot window images into global images:
 // glob[k][gx][z] += sum over slots covering gx (slot order) of win[k].
 void launch_gather(hipStream_t st, const BatchShape& b, const float* win, long long wstride, float* glob,
-                   long long gstride, int nimg, int allnx, int allnz);
+                   long long gstride, int nimg, int allnx, int allnz, const int* skip = nullptr);
+// Per-slot non-finite flags of records (windows = 0) or image windows (windows = 1).
+void launch_slot_nonfinite(hipStream_t st, const BatchShape& b, const float* a, long long astride, int narr,
+                           int windows, int* flags);
 // Per-trace FP64 sums for the NCC residual: out[slot_trace][6] =
 // {|d_x|^2, |D_x|^2, d_x.D_x, |d_z|^2, |D_z|^2, d_z.D_z}.
 void launch_ncc_sums(hipStream_t st, const BatchShape& b, const float* sz, const float* sx, const float* oz,
@@ -204,11 +207,23 @@ constexpr long long FUSED_MAX_CELLS = (1LL << 30) - (1LL << 20);
 // FUSED_HALO rows above and below its block are loaded to feed the z taps,
 // so it owns threads - 2*FUSED_HALO rows.
 constexpr int FUSED_HALO = 8;
+// Widest x chunk of the fused acoustic steps (the adjoint stages the
+// receiver residuals of its chunk's columns in LDS).
+constexpr int ACOU_FUSED_MAX_W = 1024;
 bool fused_threads_ok(int threads);
 // depth: columns of operands in flight per lane (2 or 4)
 // pml = false compiles the damping-strip code out; valid only when no window
 // of the batch has a PML (boundary != 0 for every slot).
 void launch_fused_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, const ChunkDesc* chunks,
                        int nchunks, const int* rmap, int it, int threads, int depth, bool pml);
+// Two time steps per launch (steps it and it+1, it+1 < nt): workgroups own
+// threads - 2*FUSED2_HALO rows.  scratch: 5 fields of n floats (the damped
+// cells' intermediate splits; unused when pml is false).  depth: columns of
+// operands in flight per lane (1 or 2).
+constexpr int FUSED2_HALO = 16;
+bool fused2_threads_ok(int threads);
+void launch_fused2_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, float* scratch,
+                        const ChunkDesc* chunks, int nchunks, const int* rmap, int it, int threads, int depth,
+                        bool pml);
 
 }  // namespace ds2d

@@ -57,6 +57,16 @@ __device__ __forceinline__ void put(float* p, uint32_t boff, float v) {
     *reinterpret_cast<float*>(reinterpret_cast<char*>(p) + boff) = v;
 }
 
+// A copy in a register distinct from the source (early clobber).  A loaded
+// operand that enters an x-tap queue stays live there for eight columns;
+// copying it lets the next prefetch reuse the load register, so the loop
+// latch never moves (and waits for) freshly loaded registers.
+__device__ __forceinline__ float fresh(float x) {
+    float r;
+    asm("v_mov_b32 %0, %1" : "=&v"(r) : "v"(x));
+    return r;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -81,7 +91,9 @@ __global__ __launch_bounds__(T) void k_acou_born_fused(AcouFusedArgs a, const Ch
 
     struct Col {
         float p4, vx, vz, bu, kk, mm, dp;
+        int rec;  // trace recorded at column j - 4 (receiver row lane only; -1 none)
     };
+    const int x0 = C.x0, x1 = C.x1, j0 = x0 - 4;
     auto ldc = [&](const float* p, int col) { return at(p, g.zc4 + (uint32_t)min(max(col, 0), nxp - 1) * g.colb); };
     auto fetch = [&](int j) {
         Col o;
@@ -94,11 +106,14 @@ __global__ __launch_bounds__(T) void k_acou_born_fused(AcouFusedArgs a, const Ch
         o.mm = ldc(a.m, cs);
         const long long h = (g.zin && cs >= 0 && cs < nxp) ? hist_index(S, cs, z, it) : -1;
         o.dp = h >= 0 ? a.hist[h] : 0.f;
+        // prefetched with the operands: a load at the store would wait for
+        // every prefetch issued before it
+        o.rec = -1;
+        if (a.rec && z == S.igz && cs >= x0 && cs < x1) o.rec = rmap[S.rmap_off + cs];
         return o;
     };
     const float c1 = DS2D_C1, c2 = DS2D_C2, c3 = DS2D_C3, c4 = DS2D_C4;
     float qp[9], qvx[9], qvz[5];
-    const int x0 = C.x0, x1 = C.x1, j0 = x0 - 4;
 #pragma unroll
     for (int k = 0; k < 8; ++k) qp[k + 1] = ldc(Ap, j0 - 4 + k);
 #pragma unroll
@@ -114,7 +129,7 @@ __global__ __launch_bounds__(T) void k_acou_born_fused(AcouFusedArgs a, const Ch
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) qvz[k] = qvz[k + 1];
-        qp[8] = o.p4;
+        qp[8] = fresh(o.p4);
         lds[p][0][L] = qp[4];   // p of column j
         lds[p][1][L] = qvz[0];  // new vz of column j - 4
         __syncthreads();
@@ -147,10 +162,7 @@ __global__ __launch_bounds__(T) void k_acou_born_fused(AcouFusedArgs a, const Ch
                 pn = pn + (bd + 2 * o.mm * o.dp);
             }
             put(Bp, g.zo4 + (uint32_t)cs * g.colb, pn);
-            if (z == S.igz && a.rec) {
-                const int r = rmap[S.rmap_off + cs];
-                if (r >= 0) a.rec[S.rec_off + (long long)r * nt + it] = pn;
-            }
+            if (o.rec >= 0) a.rec[S.rec_off + (long long)o.rec * nt + it] = pn;
         }
     };
     const int jend = j0 + ((x1 + 4 - j0 + 1) & ~1);
@@ -187,7 +199,6 @@ __global__ __launch_bounds__(T) void k_acou_adj_fused(AcouFusedArgs a, const Chu
 
     struct Col {
         float lp4, k4, lvx, lvz, bu, dp, gr;
-        int rec4;
     };
     auto ldc = [&](const float* p, int col) { return at(p, g.zc4 + (uint32_t)min(max(col, 0), nxp - 1) * g.colb); };
     auto fetch = [&](int j) {
@@ -202,20 +213,45 @@ __global__ __launch_bounds__(T) void k_acou_adj_fused(AcouFusedArgs a, const Chu
         const long long h = (g.zin && cs >= 0 && cs < nxp) ? hist_index(S, cs, z, it) : -1;
         o.dp = h >= 0 ? a.hist[h] : 0.f;
         o.gr = ldc(a.grad, cs);
-        o.rec4 = (z == S.igz && jl >= 0 && jl < nxp) ? rmap[S.rmap_off + jl] : -1;
         return o;
     };
-    // lp of column col after the receiver injection of this step
-    auto inject = [&](float v, int rec) { return rec >= 0 ? v + a.res[S.rec_off + (long long)rec * nt + it] : v; };
     const float c1 = DS2D_C1, c2 = DS2D_C2, c3 = DS2D_C3, c4 = DS2D_C4;
     float qw[9], qlp[9], qux[9], quz[5];
     const int x0 = C.x0, x1 = C.x1, j0 = x0 - 4;
+    const int jend = j0 + ((x1 + 4 - j0 + 1) & ~1);
+    // Residual of this step at the receivers of columns [j0 - 4, jend + 4),
+    // staged once per launch (the host keeps chunks at most ACOU_FUSED_MAX_W
+    // columns wide): the receiver row then needs no dependent load per column.
+    __shared__ float resv[ACOU_FUSED_MAX_W + 24];
+    __shared__ int resk[ACOU_FUSED_MAX_W + 24];
+    const int cbase = j0 - 4, ncols = jend + 4 - cbase;
+    for (int q = L; q < ncols; q += T) {
+        const int col = cbase + q;
+        int r = -1;
+        float v = 0.f;
+        if (col >= 0 && col < nxp) {
+            r = rmap[S.rmap_off + col];
+            if (r >= 0) v = a.res[S.rec_off + (long long)r * nt + it];
+        }
+        resk[q] = r;
+        resv[q] = v;
+    }
+    __syncthreads();
+    // lp of column col after the receiver injection of this step
+    const bool rrow = z == S.igz;
+    auto inject = [&](float v, int col) { return (rrow && resk[col - cbase] >= 0) ? v + resv[col - cbase] : v; };
+    // prefill: issue the 16 loads first (clamped, unconditional), then combine
+    float plp[8], pk[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        plp[k] = ldc(Ap, j0 - 4 + k);
+        pk[k] = ldc(a.k, j0 - 4 + k);
+    }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int col = j0 - 4 + k;
-        const int rec = (z == S.igz && col >= 0 && col < nxp) ? rmap[S.rmap_off + col] : -1;
-        const float lp = inject(ldc(Ap, col), rec);
-        qw[k + 1] = in_u_f(S, col, z) ? ldc(a.k, col) * lp : 0.f;
+        const float lp = inject(plp[k], col);
+        qw[k + 1] = in_u_f(S, col, z) ? pk[k] * lp : 0.f;
         qlp[k + 1] = lp;
     }
 #pragma unroll
@@ -233,7 +269,7 @@ __global__ __launch_bounds__(T) void k_acou_adj_fused(AcouFusedArgs a, const Chu
 #pragma unroll
         for (int k = 0; k < 4; ++k) quz[k] = quz[k + 1];
         const int jl = j + 4;
-        const float lp4 = inject(o.lp4, o.rec4);
+        const float lp4 = fresh(inject(o.lp4, jl));
         qw[8] = in_u_f(S, jl, z) ? o.k4 * lp4 : 0.f;
         qlp[8] = lp4;
         lds[p][0][L] = qw[4];   // k lp of column j
@@ -275,7 +311,6 @@ __global__ __launch_bounds__(T) void k_acou_adj_fused(AcouFusedArgs a, const Chu
             put(a.grad, c, gr);
         }
     };
-    const int jend = j0 + ((x1 + 4 - j0 + 1) & ~1);
     Col cA = fetch(j0), cB = fetch(j0 + 1);
     for (int j = j0; j < jend; j += 2) {
         column(j, cA, 0);

@@ -46,7 +46,19 @@ struct Col {
     float vxx, vzz;              // old vx_x, vz_z of column j (PML lanes only)
     float dxs, dzs;              // damping of column j-4 (stress, PML lanes only)
     float sxx, szz, sxz;         // old txx_x, tzz_z, txz_x of column j-4 (PML lanes only)
+    int rec;                     // trace recorded at column j (receiver row lane only; -1 none)
 };
+
+// A copy in a register distinct from the source (early clobber).  Operands
+// that enter the x-tap queues stay live there for eight more columns; when
+// the queue held the loaded register itself, the next prefetch could not
+// reuse it and the loop latch had to move freshly loaded registers, which
+// waited for every load of the pass (the prefetch depth collapsed to zero).
+__device__ __forceinline__ float fresh(float x) {
+    float r;
+    asm("v_mov_b32 %0, %1" : "=&v"(r) : "v"(x));
+    return r;
+}
 }  // namespace
 
 template <int T, int PF, bool PML>
@@ -68,6 +80,7 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
     const bool zown = zin && L >= FUSED_HALO && L < T - FUSED_HALO;  // rows this workgroup stores
     const bool vlane = L >= 4 && L < T - 4;    // velocity needed (z taps of owned stress rows)
     const bool zupd = z >= NOP && z < nzp - NOP;
+    const int x0 = C.x0, x1 = C.x1;
 
     __shared__ float lds[2][4][T];
 
@@ -130,6 +143,10 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
             o.vxx = at(split(a.A, 0), c);
             o.vzz = at(split(a.A, 3), c);
         }
+        // receiver map of column j, prefetched like the operands (a load at
+        // the store would wait for every prefetch issued before it)
+        o.rec = -1;
+        if (z == S.igz && j >= x0 && j < x1) o.rec = rmap[S.rmap_off + j];
         o.dxs = o.dzs = o.sxx = o.szz = o.sxz = 0.f;
         if (PML && supd && damp_cell_f(S, cs, z)) {
             const uint32_t c = zo4 + (uint32_t)cs * colb;
@@ -147,7 +164,6 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
 
     const float c1 = DS2D_C1, c2 = DS2D_C2, c3 = DS2D_C3, c4 = DS2D_C4;
     float qxx[9], qxz[9], qzz[5], qvx[9], qvz[8];
-    const int x0 = C.x0, x1 = C.x1;
     const int j0 = x0 - 4;
     // prefill: qxx/qxz[1..8] = old stress of columns j0-4 .. j0+3; qzz[1..4] = j0-4 .. j0-1
 #pragma unroll
@@ -169,6 +185,8 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
         return d == 1 ? c1 : d == 2 ? c2 : d == 3 ? c3 : d == 4 ? c4 :
                d == 0 ? -c1 : d == -1 ? -c2 : d == -2 ? -c3 : d == -3 ? -c4 : 0.f;
     };
+    // z-tap weight of this lane's row in the source column (once per lane)
+    const float wz = w * tap(S.isz - z);
 
     // One column: velocity of column j and stress of column j - 4, from the
     // operands o of column j; p selects the LDS buffer of this phase.
@@ -183,9 +201,9 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
         for (int k = 0; k < 4; ++k) qzz[k] = qzz[k + 1];
 #pragma unroll
         for (int k = 0; k < 7; ++k) qvz[k] = qvz[k + 1];
-        qxx[8] = o.txx4;
-        qxz[8] = o.txz4;
-        qzz[4] = o.tzz;
+        qxx[8] = fresh(o.txx4);
+        qxz[8] = fresh(o.txz4);
+        qzz[4] = fresh(o.tzz);
         // ---- z taps through LDS: txz, tzz of column j; new vx, vz of column j-4
         lds[p][0][L] = qxz[4];
         lds[p][1][L] = qzz[4];
@@ -203,8 +221,9 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
                        c4 * (L0[L + 3] - L0[L - 4]);
             float cz = c1 * (L1[L + 1] - L1[L]) + c2 * (L1[L + 2] - L1[L - 1]) + c3 * (L1[L + 3] - L1[L - 2]) +
                        c4 * (L1[L + 4] - L1[L - 3]);
-            if (z == S.isz) ax += w * tap(S.isx - j);
-            if (j == S.isx) cz += w * tap(S.isz - z);
+            // the source reaches the x taps of columns isx-4 .. isx+3 (a uniform test)
+            if ((unsigned)(S.isx - j + 3) < 8u && z == S.isz) ax += w * tap(S.isx - j);
+            if (j == S.isx) cz += wz;
             const float bx = o.bx;
             // A wave holding a damped cell runs the split update on all its lanes (in the
             // undamped ones the damping and the loaded split are zero, so it is the unsplit
@@ -236,13 +255,10 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
             const uint32_t c = zo4 + (uint32_t)j * colb;
             st(a.B, c, nvx);
             st(a.B + n, c, nvz);
-            if (z == S.igz) {
-                const int r = rmap[S.rmap_off + j];
-                if (r >= 0) {
-                    const size_t q = (size_t)S.rec_off + (size_t)r * nt + it;
-                    a.rec_z[q] = nvz;
-                    a.rec_x[q] = nvx;
-                }
+            if (o.rec >= 0) {
+                const size_t q = (size_t)S.rec_off + (size_t)o.rec * nt + it;
+                a.rec_z[q] = nvz;
+                a.rec_x[q] = nvx;
             }
         }
         // ---- stress of column cs = j - 4 (owned columns only)

@@ -95,10 +95,12 @@ void launch_setup_acoustic(hipStream_t st, const BatchShape& b, const SetupArgs&
     hipLaunchKernelGGL(k_setup, grid, dim3(BZ, BX), 0, st, a, m, bu, kk, refl, b.slots);
 }
 
-// glob[k][gx][z] += sum_{slots s covering gx, in slot order} win[k][s][gx-col0_s][z]
+// glob[k][gx][z] += sum_{slots s covering gx, in slot order} win[k][s][gx-col0_s][z];
+// slots flagged in skip (may be null) are left out.
 __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ win, long long wstride, float* glob,
                                                 long long gstride, int nimg, int allnx, int allnz,
-                                                const SlotDesc* __restrict__ slots, int nslots) {
+                                                const SlotDesc* __restrict__ slots, int nslots,
+                                                const int* __restrict__ skip) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)allnx * allnz) return;
     const int gx = (int)(t / allnz), z = (int)(t % allnz);
@@ -107,7 +109,7 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ win, l
         bool any = false;
         for (int s = 0; s < nslots; ++s) {
             const int c0 = slots[s].col0, nx = slots[s].nx;
-            if (gx < c0 || gx >= c0 + nx) continue;
+            if (gx < c0 || gx >= c0 + nx || (skip && skip[s])) continue;
             const float v = win[k * wstride + slots[s].win_off + (long long)(gx - c0) * allnz + z];
             acc = any ? acc + v : v;
             any = true;
@@ -117,10 +119,32 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ win, l
 }
 
 void launch_gather(hipStream_t st, const BatchShape& b, const float* win, long long wstride, float* glob,
-                   long long gstride, int nimg, int allnx, int allnz) {
+                   long long gstride, int nimg, int allnx, int allnz, const int* skip) {
     const long long n = (long long)allnx * allnz;
     hipLaunchKernelGGL(k_gather, dim3(ceil_div(n, 256)), dim3(256), 0, st, win, wstride, glob, gstride, nimg, allnx,
-                       allnz, b.slots, b.nslots);
+                       allnz, b.slots, b.nslots, skip);
+}
+
+// flags[s] = 1 when any of the narr arrays (stride astride) holds a NaN or an
+// infinity in slot s's records (windows = 0: ntr*nt samples from rec_off) or
+// image window (windows = 1: nx*nz cells from win_off); flags are only set.
+__global__ __launch_bounds__(256) void k_slot_nonfinite(const float* __restrict__ a, long long astride, int narr,
+                                                        const SlotDesc* __restrict__ slots, int windows, int nt,
+                                                        int* flags) {
+    const SlotDesc S = slots[blockIdx.y];
+    const long long len = windows ? (long long)S.nx * S.nz : (long long)S.ntr * nt;
+    const long long base = windows ? S.win_off : S.rec_off;
+    bool bad = false;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < len;
+         i += (long long)gridDim.x * blockDim.x)
+        for (int k = 0; k < narr; ++k) bad |= !(fabsf(a[k * astride + base + i]) <= 3.402823466e38f);
+    if (__any(bad) && (threadIdx.x & 63) == 0) flags[blockIdx.y] = 1;
+}
+
+void launch_slot_nonfinite(hipStream_t st, const BatchShape& b, const float* a, long long astride, int narr,
+                           int windows, int* flags) {
+    hipLaunchKernelGGL(k_slot_nonfinite, dim3(64, b.nslots), dim3(256), 0, st, a, astride, narr, b.slots, windows,
+                       b.nt, flags);
 }
 
 // One workgroup per (trace, slot): FP64 sums over time of the NCC terms

@@ -107,7 +107,9 @@ void write_bin(const std::string& fn, const float* v, size_t n) {
 
 // ---- SU (SEG-Y trace header, subfun/segy.h:8-309): 240 bytes, native endian
 constexpr int SU_HDR = 240;
-constexpr int OFF_TRACL = 0, OFF_FLDR = 8, OFF_TRACF = 12, OFF_SX = 72, OFF_GX = 80, OFF_NS = 114, OFF_DT = 116;
+constexpr int OFF_TRACL = 0, OFF_FLDR = 8, OFF_TRACF = 12, OFF_EP = 16, OFF_CDP = 20, OFF_TRID = 28, OFF_OFFSET = 36,
+              OFF_SCALEL = 68, OFF_SCALCO = 70, OFF_SX = 72, OFF_SY = 76, OFF_GX = 80, OFF_GY = 84, OFF_NS = 114,
+              OFF_DT = 116;
 
 template <typename T>
 T hget(const unsigned char* h, int off) {
@@ -170,16 +172,29 @@ void read_gather(const std::string& fn, long long pos, int ntr, int nt, float* d
     std::fclose(fp);
 }
 
-void write_su(FILE* fp, int tracl, int fldr, int tracf, int sx, int gx, int nt, float dt, const float* tr) {
+// One trace of shot j (0-based), receiver k of the shot, with the header
+// fields write_shot_gather_su_3d sets (read_write_su.c:43-62, ny = 1 so the
+// in-line index i is 0): tracl = i, fldr = ep = j + 1, tracf = cdp = k,
+// offset = gx - sx, trid = scalel = scalco = 1, sy = gy = 0, and the sample
+// interval truncated to microseconds in float arithmetic (hdt = dt*1000000).
+void write_su(FILE* fp, int j, int k, int sx, int gx, int nt, float dt, const float* tr) {
     unsigned char h[SU_HDR];
     std::memset(h, 0, sizeof h);
-    hset<int>(h, OFF_TRACL, tracl);
-    hset<int>(h, OFF_FLDR, fldr);
-    hset<int>(h, OFF_TRACF, tracf);
+    hset<int>(h, OFF_TRACL, 0);
+    hset<int>(h, OFF_FLDR, j + 1);
+    hset<int>(h, OFF_TRACF, k);
+    hset<int>(h, OFF_EP, j + 1);
+    hset<int>(h, OFF_CDP, k);
+    hset<short>(h, OFF_TRID, 1);
+    hset<int>(h, OFF_OFFSET, gx - sx);
+    hset<short>(h, OFF_SCALEL, 1);
+    hset<short>(h, OFF_SCALCO, 1);
     hset<int>(h, OFF_SX, sx);
+    hset<int>(h, OFF_SY, 0);
     hset<int>(h, OFF_GX, gx);
+    hset<int>(h, OFF_GY, 0);
     hset<unsigned short>(h, OFF_NS, (unsigned short)nt);
-    hset<unsigned short>(h, OFF_DT, (unsigned short)std::lround(dt * 1e6));
+    hset<unsigned short>(h, OFF_DT, (unsigned short)(short)(dt * 1000000));
     std::fwrite(h, 1, SU_HDR, fp);
     std::fwrite(tr, 4, nt, fp);
 }
@@ -240,15 +255,13 @@ int run_synth(const Params& p) {
     FILE* fz = std::fopen(p.fn1.c_str(), "wb");
     FILE* fx = std::fopen(p.fn2.c_str(), "wb");
     if (!fz || !fx) die("cannot write SU files");
-    int tracl = 0;
     for (size_t s = 0; s < shots.size(); ++s) {
         const int sx = (int)std::lround((shots[s].isx - p.pml) * p.dx);
         for (int r = 0; r < p.allnx; ++r) {
             const size_t o = (s * p.allnx + r) * (size_t)nt;
             const int gx = (int)std::lround(r * p.dx);
-            ++tracl;
-            write_su(fz, tracl, (int)s + 1, r + 1, sx, gx, nt, dt, &rz[o]);
-            write_su(fx, tracl, (int)s + 1, r + 1, sx, gx, nt, dt, &rx[o]);
+            write_su(fz, (int)s, r, sx, gx, nt, dt, &rz[o]);
+            write_su(fx, (int)s, r, sx, gx, nt, dt, &rx[o]);
         }
     }
     std::fclose(fz);
@@ -380,7 +393,17 @@ int run_lsrtm(const Params& p) {
         size_t roff = 0;
         for (size_t s0 = 0; s0 < L.shots.size(); s0 += batch) {
             const int nb = (int)std::min<size_t>(batch, L.shots.size() - s0);
-            check(ds2d_batch_gradient(h, nb, &L.shots[s0], &L.oz[roff], &L.ox[roff], iter, &misfit), "gradient");
+            const int grc = ds2d_batch_gradient(h, nb, &L.shots[s0], &L.oz[roff], &L.ox[roff], iter, &misfit);
+            if (grc == DS2D_ESHOT) {  // the batch completed without the failed shots: report them, go on
+                std::vector<int> st(nb);
+                check(ds2d_shot_status(h, nb, st.data()), "shot status");
+                for (int k = 0; k < nb; ++k)
+                    if (st[k])
+                        std::fprintf(stderr, "ds2d_lsrtm: iteration %d: shot %d left out (non-finite residual or "
+                                     "gradient)\n", iter, L.shots[s0 + k].shot_id);
+            } else {
+                check(grc, "gradient");
+            }
             for (int k = 0; k < nb; ++k) roff += (size_t)L.shots[s0 + k].ntr * R;
         }
         check(ds2d_take_gradients(h, gpp.data(), gps.data(), il.data(), bpp.data(), bps.data(), b2pp.data(),

@@ -16,7 +16,12 @@ _lib = None
 
 
 class DS2DError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+ESHOT = -5  # some shots of a batch failed (non-finite); see Solver.shot_status
 
 
 class Geom(C.Structure):
@@ -69,6 +74,10 @@ SIGNATURES = {
     "ds2d_shot_gradient": (C.c_int, [_H, C.c_int, _D]),
     "ds2d_batch_gradient": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.c_int, _D]),
     "ds2d_take_gradients": (C.c_int, [_H, _F, _F, _F, _F, _F, _F, _F]),
+    "ds2d_shot_status": (C.c_int, [_H, C.c_int, C.POINTER(C.c_int)]),
+    "ds2d_set_time_block": (C.c_int, [_H, C.c_int]),
+    "ds2d_set_stop": (C.c_int, [_H, C.c_double]),
+    "ds2d_get_time_block": (C.c_int, [_H, C.POINTER(C.c_int)]),
     "ds2d_forward": (C.c_int, [_H, _F, _F]),
     "ds2d_batch_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.c_int, C.POINTER(Stats)]),
     "ds2d_set_chunk": (C.c_int, [_H, C.c_int, C.c_int, C.c_int]),
@@ -127,7 +136,7 @@ def load(path=LIB_PATH):
 
 def _check(rc):
     if rc != 0:
-        raise DS2DError(f"ds2d error {rc}: {_lib.ds2d_last_error().decode()}")
+        raise DS2DError(f"ds2d error {rc}: {_lib.ds2d_last_error().decode()}", rc)
 
 
 def fptr(a):
@@ -235,6 +244,15 @@ class Solver:
         """Fused-kernel tile: x-chunk width, workgroup lanes, prefetch depth (0 = defaults)."""
         _check(self.lib.ds2d_set_chunk(self.h, int(width), int(threads), int(depth)))
 
+    def set_time_block(self, steps):
+        """Fused elastic forward: 1 or 2 time steps per launch (temporal blocking)."""
+        _check(self.lib.ds2d_set_time_block(self.h, int(steps)))
+
+    def time_block(self):
+        v = C.c_int(0)
+        _check(self.lib.ds2d_get_time_block(self.h, C.byref(v)))
+        return v.value
+
     def set_graph(self, on):
         _check(self.lib.ds2d_set_graph(self.h, 1 if on else 0))
 
@@ -262,6 +280,12 @@ class Solver:
         ox = np.ascontiguousarray(obs_x, np.float32)
         _check(self.lib.ds2d_batch_gradient(self.h, len(shots), arr, fptr(oz), fptr(ox), it, C.byref(mf)))
         return mf.value
+
+    def shot_status(self, nshots):
+        """Per shot of the last reference-compat gradient: 0 included, 1 left out (non-finite)."""
+        st = (C.c_int * nshots)()
+        _check(self.lib.ds2d_shot_status(self.h, nshots, st))
+        return list(st)
 
     def set_precision(self, source_fp64):
         """Reference-compat gradient: background and first-order Born field in FP64 (1) or FP32 (0)."""
@@ -374,8 +398,12 @@ class Solver:
 
     def set_fused(self, mode):
         """Fused Born/adjoint steps of the resident-history passes: -1 auto (timed), 0 never,
-        1 always (x width 64), k in [8, 4096] always with x width k."""
+        1 always (x width 64), k in [8, 1024] always with x width k."""
         _check(self.lib.ds2d_set_fused(self.h, int(mode)))
+
+    def set_stop(self, target):
+        """CG solvers stop once the relative data residual is at most target (0: never)."""
+        _check(self.lib.ds2d_set_stop(self.h, float(target)))
 
     def set_batch(self, shots_per_launch):
         """Shots per time loop for the plain Born/adjoint/CG entries (0 = all)."""

@@ -49,6 +49,8 @@ extern "C" {
 #define DS2D_EDEVICE (-2)  /* HIP runtime error or no usable gfx950 device */
 #define DS2D_ENOMEM (-3)   /* device allocation failed                     */
 #define DS2D_ESTATE (-4)   /* call out of order (no model, no shot, ...)    */
+#define DS2D_ESHOT (-5)    /* some shots of a batch failed (non-finite); the
+                              others completed: see ds2d_shot_status        */
 
 typedef struct ds2d_ctx ds2d_t;
 
@@ -113,9 +115,16 @@ int ds2d_set_shot(ds2d_t* h, const ds2d_shot* s, const float* rec_z, const float
 /* Reference per-shot LSRTM gradient (compat mode: Born + 2nd-order Born,
  * NCC residual, three-term imaging).  Accumulates into the handle's images. */
 int ds2d_shot_gradient(ds2d_t* h, int iter, double* misfit);
-/* Same for many shots in one batched launch per step (obs_* concatenated). */
+/* Same for many shots in one batched launch per step (obs_* concatenated).
+ * A shot whose NCC residual or adjoint images hold a NaN or an infinity is
+ * left out of the misfit and of the accumulated images, the other shots
+ * complete, and the call returns DS2D_ESHOT (no reference counterpart: the
+ * reference has no per-shot failure handling). */
 int ds2d_batch_gradient(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* obs_z,
                         const float* obs_x, int iter, double* misfit);
+/* Per-shot status of the last ds2d_batch_gradient / ds2d_shot_gradient
+ * (nshots must match it): 0 = included, 1 = left out (non-finite). */
+int ds2d_shot_status(ds2d_t* h, int nshots, int* status);
 int ds2d_take_gradients(ds2d_t* h, float* pp, float* ps, float* illum, float* b_pp, float* b_ps,
                         float* b2_pp, float* b2_ps);
 /* Precision of the reference-compat gradient's two source-side wavefields
@@ -148,11 +157,18 @@ int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec
 /* Tile of the fused kernel: x-chunk width (0 = default 64), workgroup size
  * (256, 512, 704 or 768 lanes, one z row each, owning threads-16 rows; 0 =
  * default 256) and prefetch depth (columns of operands in flight, 2 or 4;
- * 0 = 2); and an autotuner that times nsteps per candidate on this batch --
- * the fused kernel at several tiles and the two-kernel step (reported as
+ * 0 = 2; with two steps per launch 1 or 2); and an autotuner that times
+ * nsteps per candidate on this batch -- the fused kernel at several tiles,
+ * one or two time steps per launch, and the two-kernel step (reported as
  * width 0) -- and keeps the fastest for mode 2.  No reference counterpart
  * (new capability). */
 int ds2d_set_chunk(ds2d_t* h, int width, int threads, int depth);
+/* Time steps per launch of the fused elastic forward (mode 2): 1, or 2 =
+ * temporal blocking (steps it and it+1 in one pass over HBM; workgroups of
+ * 512, 704 or 768 lanes own threads-32 rows).  Set by the autotuner too.
+ * No reference counterpart (new capability). */
+int ds2d_set_time_block(ds2d_t* h, int steps);
+int ds2d_get_time_block(ds2d_t* h, int* steps);
 /* Capture each time loop into one hipGraph (1) or launch step by step (0). */
 int ds2d_set_graph(ds2d_t* h, int on);
 int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nsteps, int* best_width,
@@ -268,9 +284,15 @@ int ds2d_set_batch(ds2d_t* h, int shots_per_launch);
 /* Resident-history Born and adjoint passes as one fused launch per step
  * (x-marching workgroups): -1 = auto (default: the per-cell kernels and the
  * fused step at several x widths are timed once per batch geometry), 0 =
- * never (per-cell kernels), 1 = always (x width 64), k in [8, 4096] = always,
+ * never (per-cell kernels), 1 = always (x width 64), k in [8, 1024] = always,
  * with x width k. */
 int ds2d_set_fused(ds2d_t* h, int mode);
+/* Stop rule of the CG solvers (ds2d_lsrtm_cg, ds2d_lsrtm_learned,
+ * ds2d_elsrtm_cg, ds2d_elsrtm_learned): return after the first iteration
+ * whose relative data residual |d - Lm|/|d| is at most target (the rest of
+ * resid repeats it); 0 = run all niter iterations (default).  No reference
+ * counterpart (the reference runs a fixed iteration count). */
+int ds2d_set_stop(ds2d_t* h, double target);
 
 /* Host-side nonlinear CG update of the reference (sjcgsolver, main.cpp:1232-1303):
  * cg = -g1 + beta cg, beta = max(0, min(HS, DY)) (0 at iter 0); g0 <- g1;

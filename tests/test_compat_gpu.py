@@ -167,3 +167,30 @@ def test_autotuned_tile_matches_oracle():
         oz, ox = orc.elastic_forward(_osh(sh, allnx, allnz, pml, nt), vp, vs, rho, wav)
         assert orc.rel_l2(rz[off:off + n], oz) <= TOL and orc.rel_l2(rx[off:off + n], ox) <= TOL
         off += n
+
+
+def test_failed_shot_is_left_out_and_reported():
+    """A NaN in shot 1's observed records makes its NCC residual and adjoint
+    images non-finite.  The batched gradient leaves that shot out of the
+    misfit and the images, completes shot 0 (images equal to shot 0's alone),
+    returns DS2D_ESHOT and reports the shot through ds2d_shot_status.  The
+    failure is forced from the host: every kernel runs to its normal end."""
+    from ds2d_amd.lib import DS2DError, ESHOT
+    allnx, allnz, pml, nt, vp, vs, wav, img_pp, img_ps, shots = _case(0, nt=200)
+    oz, ox = _observed(shots, allnx, allnz, pml, nt, vp, vs, wav)
+    bad_z = oz[1].copy()
+    bad_z[5] = np.nan
+    s = _solver(allnx, allnz, pml, nt, 40, wav, rho_const=2000.0)
+    s.set_model(vp, vs, None)
+    s.set_image(img_pp, img_ps)
+    mf0 = s.batch_gradient(shots[:1], oz[0], ox[0], 1)
+    ref = s.take_gradients()
+    assert s.shot_status(1) == [0]
+    with pytest.raises(DS2DError) as e:
+        s.batch_gradient(shots, np.concatenate([oz[0], bad_z]), np.concatenate([ox[0], ox[1]]), 1)
+    assert e.value.code == ESHOT
+    assert s.shot_status(2) == [0, 1]
+    got = s.take_gradients()
+    for k in range(7):
+        assert np.all(np.isfinite(got[k])) and np.array_equal(got[k], ref[k]), k
+    print(f"shot 0 alone misfit {mf0:.6f}; failed shot left out, images equal shot 0's")

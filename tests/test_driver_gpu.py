@@ -47,6 +47,8 @@ def _run(*args):
 def test_driver_reference_mode(tmp_path, opt):
     par, img = _params(tmp_path, "ref", {"opt": opt, "batch": "2"})
     _run("synth", par)
+    for name in ("vz.su", "vx.su"):
+        _check_su_headers(tmp_path / name, 3, NX, 400, 0.001, [16, 48, 80], 10.0)
     _run(par)
     last = np.fromfile(str(img / f"sum_image_pp_1_{NX}_{NZ}.dat"), np.float32)
     grad = np.fromfile(str(img / f"all_grad_pp_0_{NX}_{NZ}.dat"), np.float32)
@@ -58,6 +60,31 @@ def test_driver_reference_mode(tmp_path, opt):
     assert float(np.dot(first.astype(np.float64), grad)) < 0
     lines = (img / "misfit.txt").read_text().strip().split("\n")
     assert len(lines) == 2
+
+
+def _check_su_headers(path, nshots, ntr, nt, dt, sxs, dx):
+    """Every 240-byte header as write_shot_gather_su_3d writes it
+    (read_write_su.c:43-62, one in-line index i = 0): tracl = 0,
+    fldr = ep = shot + 1, tracf = cdp = receiver k, sx, sy = 0, gx, gy = 0,
+    offset = gx - sx, ns, dt = (short)(dt * 1000000), trid = scalel =
+    scalco = 1; every other byte zero."""
+    raw = np.fromfile(str(path), np.uint8).reshape(nshots * ntr, 240 + 4 * nt)
+    hdt = int(np.float32(dt) * np.float32(1000000))
+    for t in range(nshots * ntr):
+        j, k = divmod(t, ntr)
+        sx, gx = int(round(sxs[j] * dx)), int(round(k * dx))
+        want = np.zeros(240, np.uint8)
+
+        def put(off, val, fmt):
+            want[off:off + np.dtype(fmt).itemsize] = np.frombuffer(np.array([val], fmt).tobytes(), np.uint8)
+        for off, val in ((0, 0), (8, j + 1), (12, k), (16, j + 1), (20, k), (36, gx - sx), (72, sx), (76, 0),
+                         (80, gx), (84, 0)):
+            put(off, val, "<i4")
+        for off, val in ((28, 1), (68, 1), (70, 1)):
+            put(off, val, "<i2")
+        put(114, nt, "<u2")
+        put(116, hdt, "<u2")
+        assert np.array_equal(raw[t, :240], want), (t, np.nonzero(raw[t, :240] != want))
 
 
 def _read_su(path, ntr, nt):

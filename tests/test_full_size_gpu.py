@@ -39,14 +39,17 @@ def test_c1_full_size_matches_oracle():
         assert ez <= TOL and ex <= TOL
 
 
-def test_c3_window_at_full_size_matches_oracle():
-    """The whole 128-window C3 batch runs on the GPU with the 240x704x4 tile (512
-    chunks: two rounds of 256 CUs); its narrowest window is checked against the
-    FP64 oracle over all 3000 steps."""
+@pytest.mark.parametrize("tblock,depth", [(1, 4), (2, 1)])
+def test_c3_window_at_full_size_matches_oracle(tblock, depth):
+    """The whole 128-window C3 batch runs on the GPU with 240-column, 704-lane
+    tiles (512 chunks: two rounds of 256 CUs), one time step per launch or two
+    (temporal blocking); its narrowest window is checked against the FP64
+    oracle over all 3000 steps."""
     cfg = models.c3_batched_elastic()
     s, wav = _solver(cfg, pml_rr=1e-3)
     s.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
-    s.set_chunk(240, 704, 4)
+    s.set_chunk(240, 704, depth)
+    s.set_time_block(tblock)
     rz, rx, _ = s.batch_forward(cfg["shots"], mode=2)
     k = int(np.argmin([x.nx for x in cfg["shots"]]))
     off = sum(x.ntr for x in cfg["shots"][:k]) * cfg["nt"]
@@ -54,7 +57,7 @@ def test_c3_window_at_full_size_matches_oracle():
     o = _oshot(cfg, cfg["shots"][k])
     oz, ox = orc.elastic_forward(o, cfg["vp"], cfg["vs"], cfg["rho"], wav)
     ez, ex = orc.rel_l2(rz[off:off + n], oz), orc.rel_l2(rx[off:off + n], ox)
-    print(f"C3 window {k} ({cfg['shots'][k].nx} columns): rel z {ez:.2e} x {ex:.2e}")
+    print(f"C3 window {k} ({cfg['shots'][k].nx} columns), {tblock} step(s) per launch: rel z {ez:.2e} x {ex:.2e}")
     assert np.abs(oz).max() > 0
     assert ez <= TOL and ex <= TOL
 
