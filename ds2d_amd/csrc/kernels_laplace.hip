@@ -12,6 +12,7 @@
 // for x86-64 without FMA.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <vector>
 
 #include "../../include/ds2d.h"
@@ -116,7 +117,55 @@ struct Scoped {
 
 }  // namespace
 
+// Scratch of the allocation-free band filter: the two intermediate images,
+// at most LAP_SCRATCH_CELLS cells each (2 x 16 MiB).  Every GPU the library
+// runs on holds its own copy until the process ends.
+constexpr long long LAP_SCRATCH_CELLS = 1LL << 22;
+static __device__ __align__(256) float scratch[2 * LAP_SCRATCH_CELLS];
+
+namespace {
+
+// Device address of `scratch` on the current device, looked up once per
+// device (the array's host-side name is not a device address).
+constexpr int LAP_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[LAP_MAX_DEVICES] = {};
+
+int scratch_on_current_device(float** out) {
+    int dev = -1;
+    LAP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= LAP_MAX_DEVICES) return ds2d_internal_fail(DS2D_EDEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        LAP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+}  // namespace
+
 extern "C" {
+
+int ds2d_laplace_band_noalloc(int nz, int nx, const float* in, float* out, void* stream) {
+    if (!in || !out || nz <= 0 || nx <= 0) return ds2d_internal_fail(DS2D_EINVAL, "bad laplace arguments");
+    const long long n = (long long)nx * nz;
+    if (n > LAP_SCRATCH_CELLS) {
+        char msg[160];
+        snprintf(msg, sizeof msg, "image of %lld cells exceeds the %lld-cell limit of the allocation-free filter", n,
+                 LAP_SCRATCH_CELLS);
+        return ds2d_internal_fail(DS2D_EINVAL, msg);
+    }
+    float* s = nullptr;
+    int rc;
+    if ((rc = scratch_on_current_device(&s))) return rc;
+    // every pass writes all cells of its output, so the scratch needs no clearing
+    launch_band(static_cast<hipStream_t>(stream), in, out, s, s + LAP_SCRATCH_CELLS, nx, nz);
+    LAP_TRY(hipGetLastError());
+    return 0;
+}
 
 int ds2d_laplace_filter(int nz, int nx, int order, float dx, float* dat) {
     (void)order;  // laplace.cpp:82 takes it and never reads it

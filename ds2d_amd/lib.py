@@ -114,6 +114,7 @@ SIGNATURES = {
     "ds2d_sjcg_update": (C.c_int, [C.c_int, _F, _F, _F, _F, C.c_int, _F, _F]),
     "ds2d_laplace_filter": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_float, _F]),
     "ds2d_laplace_band": (C.c_int, [C.c_int, C.c_int, _F]),
+    "ds2d_laplace_band_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ds2d_ricker": (C.c_int, [C.c_int, C.c_float, C.c_float, _F]),
 }
 

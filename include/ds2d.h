@@ -302,6 +302,16 @@ int ds2d_sjcg_update(int n, float* m, float* cg, const float* g1, float* g0, int
 
 int ds2d_laplace_filter(int nz, int nx, int order, float dx, float* dat);
 int ds2d_laplace_band(int nz, int nx, float* dat);
+/* ds2d_laplace_band on device buffers that allocates nothing: no counterpart
+ * in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  `in` and `out` are device
+ * pointers of nx*nz floats (they may be equal); the work is queued on `stream`
+ * (a hipStream_t, or null for the default stream) and the call returns
+ * without waiting.  The two intermediate images live in a library-owned
+ * device array of 2 x 4194304 floats per GPU (held until the process ends):
+ * images above 4194304 cells are rejected with DS2D_EINVAL.  Calls that
+ * overlap on one GPU share that array and race. */
+int ds2d_laplace_band_noalloc(int nz, int nx, const float* in, float* out, void* stream);
 int ds2d_ricker(int nt, float f0, float dt, float* out);
 
 #ifdef __cplusplus
