@@ -98,8 +98,7 @@ def compare(solver, unet, prob, niter, target=1e-2, log=None):
         k = first_hit(res, target)
         if log:
             log(f"seed {prob['seed']} {name}: residual {res[-1]:.4f} after {niter} iterations, {wall:.1f} s")
-        out[name] = dict(resid=[float(r) for r in res], wall_s=wall, iters_to_target=k,
-                         time_to_target_s=None if k is None else wall * k / max(niter, 1), image_rel_err=err,
+        out[name] = dict(resid=[float(r) for r in res], wall_s=wall, iters_to_target=k, image_rel_err=err,
                          learned_steps=acc)
     return out
 
@@ -174,7 +173,6 @@ def compare_elastic(solver, unet, prob, niter, target=1e-2, log=None):
         k = first_hit(res, target)
         if log:
             log(f"seed {prob['seed']} elastic {name}: residual {res[-1]:.4f} after {niter} iterations, {wall:.1f} s")
-        out[name] = dict(resid=[float(r) for r in res], wall_s=wall, iters_to_target=k,
-                         time_to_target_s=None if k is None else wall * k / max(niter, 1), image_rel_err=err,
+        out[name] = dict(resid=[float(r) for r in res], wall_s=wall, iters_to_target=k, image_rel_err=err,
                          learned_steps=acc)
     return out
