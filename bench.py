@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "unet", "compat"])
     ap.add_argument("--c5-iter", type=int, default=2, help="c5: CG iterations on the full configuration")
+    ap.add_argument("--train-elastic", type=int, default=0,
+                    help="c5: train the U-Net on elastic PP/PS pairs of this many seeded 512x256 models first")
     ap.add_argument("--train-models", type=int, default=64, help="c4: training models (seeds 1000..)")
     ap.add_argument("--eval-models", type=int, default=4, help="c4: held-out models (seeds eval-start..)")
     ap.add_argument("--eval-start", type=int, default=2000, help="c4: seed of the first held-out model")
@@ -65,6 +67,8 @@ def parse():
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "k_fused_step_c3_traffic.json"),
                     help="rocprof FETCH/WRITE summary (JSON) to embed as roofline.traffic when its kernel and "
                          "workload match the run")
+    ap.add_argument("--random-boundary", action="store_true",
+                    help="c3: the same windows with the random boundary (no PML strips)")
     ap.add_argument("--time-block", type=int, default=0, choices=[0, 1, 2],
                     help="mode 2: time steps per launch (0 = what the autotuner picked, or 1)")
     ap.add_argument("--tile", default=None,
@@ -306,6 +310,12 @@ def run_learned(a):
         u.set_params(np.load(a.load_unet))
         t_gen = t_train = 0.0
         loss, tst = np.zeros(20, np.float32), {"ms_per_step": 0.0, "flops_per_step": 0.0}
+        side = a.load_unet + ".json"
+        if os.path.exists(side):  # the cost of the network being evaluated, recorded when it was trained
+            info = json.load(open(side))
+            t_gen, t_train = info["pair_generation_s"], info["training_s"]
+            loss = np.array([info["train_loss_first_last"][0]] * 20 + [info["train_loss_first_last"][1]] * 20,
+                            np.float32)
     else:
         X, T, info, s = LR.generate_pairs(range(1000, 1000 + a.train_models), niter=20, log=log, **kw)
         t_gen = time.perf_counter() - t0
@@ -315,11 +325,17 @@ def run_learned(a):
         log(f"training: loss {loss[:20].mean():.4f} -> {loss[-20:].mean():.4f}, {tst['ms_per_step']:.2f} ms/step")
         if a.save_unet:
             np.save(a.save_unet, u.get_params())
+            with open(a.save_unet + ".json", "w") as f:  # what the network cost, for later evaluations
+                json.dump({"pair_generation_s": t_gen, "training_s": t_train, "train_models": a.train_models,
+                           "train_steps": a.train_steps, "train_loss_first_last":
+                           [float(loss[:20].mean()), float(loss[-20:].mean())]}, f)
     seeds = list(range(a.eval_start, a.eval_start + a.eval_models))
     prob = LR.acoustic_problem(seeds[0], **kw)
     # the reference's scheme on its own handle: one shot per launch, no resident history
     sr = LR.make_solver(prob, bg_mode=2)
     sr.set_batch(1)
+    for h in (s, sr):  # progress lines from the solvers themselves
+        h.set_log(25)
     # untimed warm-up of both handles (per-geometry tuning, workspace allocation)
     for h in (s, sr):
         h.set_model(prob["vp"], None, prob["rho"])
@@ -436,9 +452,31 @@ def run_c5(a):
     ref = np.concatenate([cfg["refl_pp"].ravel(), cfg["refl_ps"].ravel()])
     runs = {}
     solvers = [("plain", None)]
-    if a.load_unet:
+    train = None
+    params = None
+    if a.train_elastic > 0:
+        # elastic PP/PS pairs (RTM gradient -> 20-iteration PP/PS CG image) from the plain method on
+        # seeded 512x256 elastic models with C5's physics (dx 10 m, dt 0.8 ms, random boundary 40)
+        from ds2d_amd import learned as LR
+        t1 = time.perf_counter()
+        X, T, _, se = LR.generate_elastic_pairs(range(3000, 3000 + a.train_elastic), niter=20, log=log)
+        se.close()
+        t_gen = time.perf_counter() - t1
+        ut, loss, tst = LR.train_unet(X, T, steps=a.train_steps, batch=8, lr=1e-3, seed=0)
+        params = ut.get_params()
+        ut.close()
+        if a.save_unet:
+            np.save(a.save_unet, params)
+        train = {"elastic_models": a.train_elastic, "samples": int(len(X)), "pair_generation_s": round(t_gen, 1),
+                 "train_steps": a.train_steps, "train_s": round(tst["wall_s"], 1),
+                 "train_loss_first_last": [float(loss[:20].mean()), float(loss[-20:].mean())]}
+        log(f"elastic U-Net: {len(X)} samples in {t_gen:.1f} s, loss {loss[:20].mean():.4f} -> "
+            f"{loss[-20:].mean():.4f}")
+    elif a.load_unet:
+        params = np.load(a.load_unet)
+    if params is not None:
         u = Unet(levels=4, base=32, max_batch=1, max_h=cfg["nx"], max_w=cfg["nz"], seed=0)
-        u.set_params(np.load(a.load_unet))
+        u.set_params(params)
         solvers.append(("learned", u))
     for name, u in solvers:
         t0 = time.perf_counter()
@@ -466,7 +504,7 @@ def run_c5(a):
                                    f"{cfg['nt']} steps, PP/PS elastic Born CG LSRTM",
                        "padded_cells_per_wavefield": int(sum((x.nx + 2 * cfg["pml"]) * (cfg["nz"] + 2 * cfg["pml"])
                                                              for x in shots)),
-                       "born_data_s": round(t_fwd, 2), "solvers": runs,
+                       "born_data_s": round(t_fwd, 2), "solvers": runs, "elastic_unet_training": train,
                        "note": "a step = one CG iteration over all shots; 1e-2 is not reached in this many "
                                "iterations (see resid)"}}
 
@@ -573,8 +611,13 @@ def main():
     if a.workload == "c3":
         cfg = models.c3_batched_elastic(nshots=a.nshots or 128, nt=a.nt or 3000)
         cfg["seed"] = 3
+        edge = f"pml {cfg['pml']}"
+        if a.random_boundary:  # the same windows with the random boundary instead of the PML
+            for sh in cfg["shots"]:
+                sh.boundary = 1
+            edge = f"random boundary {cfg['pml']}"
         cfg["desc"] = (f"c3_batched_elastic: {cfg['nx']}x{cfg['nz']} model, dx=dz={cfg['dx']} m, {len(cfg['shots'])} "
-                       f"uneven windows (400-1200 cols), pml {cfg['pml']}, dt {cfg['dt'] * 1e3:.1f} ms, {cfg['nt']} steps")
+                       f"uneven windows (400-1200 cols), {edge}, dt {cfg['dt'] * 1e3:.1f} ms, {cfg['nt']} steps")
         out = run_forward(a, cfg)
     elif a.workload == "c1":
         cfg = models.c1_elastic_forward()
@@ -608,7 +651,8 @@ def main():
                tile.get("time_steps_per_launch", 1)]
         # per-launch traffic depends on the batch and the tile, not on the number of steps
         if (tr.get("workload_prefix") and str(cfgd.get("workload", "")).startswith(tr["workload_prefix"])
-                and tr.get("shots") == cfgd.get("shots")):
+                and tr.get("shots") == cfgd.get("shots")
+                and bool(tr.get("random_boundary", False)) == bool(getattr(a, "random_boundary", False))):
             for e in tr.get("entries", [tr]):
                 t = list(e.get("tile") or [])
                 if len(t) == 3:  # entries without a time block are one step per launch

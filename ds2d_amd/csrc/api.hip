@@ -80,6 +80,7 @@ struct ds2d_ctx {
     int chunk_d = 0;    // fused-kernel prefetch depth in columns (2 or 4; 0: default 2)
     int tblock = 1;     // fused elastic forward: time steps per launch (1 or 2)
     double stop_resid = 0;  // CG solvers stop once the relative data residual is at most this (0: never)
+    int log_every = 0;      // CG solvers print the residual to stderr every this many iterations (0: never)
     int use_graph = 0;  // capture time loops into one hipGraph per loop
     int per_launch = 0; // plain Born/adjoint: shots per time loop (0: all)
     int fwd_two = 0;    // autotuned: mode 2 runs the two-kernel step instead of the fused one
@@ -1456,6 +1457,8 @@ long long padded_cells(const Batch& B) {
 // after iteration k is at most the target, the remaining entries of resid
 // repeat it and the solver returns.
 bool reached(const ds2d_ctx* h, double rr, double dd, double* resid, int k, int niter) {
+    if (h->log_every > 0 && ((k + 1) % h->log_every == 0 || k + 1 == niter))
+        std::fprintf(stderr, "ds2d: iteration %d, relative data residual %.6e\n", k + 1, std::sqrt(rr / dd));
     if (!(h->stop_resid > 0) || std::sqrt(rr / dd) > h->stop_resid) return false;
     if (resid)
         for (int j = k + 1; j < niter; ++j) resid[j + 1] = resid[k + 1];
@@ -1625,6 +1628,13 @@ int ds2d_set_stop(ds2d_t* h, double target) {
     if (!h) return fail(DS2D_EINVAL, "null handle");
     if (!(target >= 0) || target >= 1) return fail(DS2D_EINVAL, "stop target must be in [0, 1), got %g", target);
     h->stop_resid = target;
+    return 0;
+}
+
+int ds2d_set_log(ds2d_t* h, int every) {
+    if (!h) return fail(DS2D_EINVAL, "null handle");
+    if (every < 0) return fail(DS2D_EINVAL, "log interval must be >= 0");
+    h->log_every = every;
     return 0;
 }
 

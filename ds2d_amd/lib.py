@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libds2d.so")
+# DS2D_LIB selects another build of the same library (e.g. a kernel variant under test)
+LIB_PATH = os.environ.get("DS2D_LIB") or os.path.join(HERE, "libds2d.so")
 
 _lib = None
 
@@ -77,6 +78,7 @@ SIGNATURES = {
     "ds2d_shot_status": (C.c_int, [_H, C.c_int, C.POINTER(C.c_int)]),
     "ds2d_set_time_block": (C.c_int, [_H, C.c_int]),
     "ds2d_set_stop": (C.c_int, [_H, C.c_double]),
+    "ds2d_set_log": (C.c_int, [_H, C.c_int]),
     "ds2d_get_time_block": (C.c_int, [_H, C.POINTER(C.c_int)]),
     "ds2d_forward": (C.c_int, [_H, _F, _F]),
     "ds2d_batch_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.c_int, C.POINTER(Stats)]),
@@ -405,6 +407,10 @@ class Solver:
     def set_stop(self, target):
         """CG solvers stop once the relative data residual is at most target (0: never)."""
         _check(self.lib.ds2d_set_stop(self.h, float(target)))
+
+    def set_log(self, every):
+        """CG solvers print their residual to stderr every `every` iterations (0: silent)."""
+        _check(self.lib.ds2d_set_log(self.h, int(every)))
 
     def set_batch(self, shots_per_launch):
         """Shots per time loop for the plain Born/adjoint/CG entries (0 = all)."""

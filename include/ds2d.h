@@ -293,6 +293,10 @@ int ds2d_set_fused(ds2d_t* h, int mode);
  * resid repeats it); 0 = run all niter iterations (default).  No reference
  * counterpart (the reference runs a fixed iteration count). */
 int ds2d_set_stop(ds2d_t* h, double target);
+/* The CG solvers print "ds2d: iteration k, relative data residual r" to
+ * stderr every `every` iterations and at the last one (0 = silent, the
+ * default), as the reference prints its misfit per iteration. */
+int ds2d_set_log(ds2d_t* h, int every);
 
 /* Host-side nonlinear CG update of the reference (sjcgsolver, main.cpp:1232-1303):
  * cg = -g1 + beta cg, beta = max(0, min(HS, DY)) (0 at iter 0); g0 <- g1;
