@@ -5,7 +5,9 @@ Default workload (c3, the N=1 line): a "step" is one pass of the hot path
 over one batch = batched elastic forward modelling of 128 uneven-aperture
 shots (windows U{400..1200} columns) on the seeded 2000x600 layered+faulted
 model, dx=dz=12.5 m, 20-cell PML, dt = 1 ms, 3000 time steps, vx/vz recorded
-at every window surface node.  All shots run in one launch per time step.
+at every window surface node.  All shots run in one launch per time step
+(or per two time steps, when the autotuner picks temporal blocking), and the
+whole time loop is captured in one hipGraph (--no-graph: direct launches).
 The model and every wavefield are resident in HBM before timing; records
 stay in HBM (fetched once, untimed).
 
@@ -61,7 +63,9 @@ def parse():
     ap.add_argument("--nt", type=int, default=None)
     ap.add_argument("--niter", type=int, default=10, help="c2: CG iterations")
     ap.add_argument("--mode", type=int, default=2, help="2 fused kernel (default), 1 two fast kernels, 0 reference layout")
-    ap.add_argument("--graph", action="store_true", help="capture each time loop in one hipGraph")
+    ap.add_argument("--graph", action="store_true",
+                    help="capture each time loop in one hipGraph (always on for c3, whose configuration names it)")
+    ap.add_argument("--no-graph", action="store_true", help="c3: launch the time loop's kernels directly")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=5.0, help="CPU baseline: seconds per timed run (3 runs)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "k_fused_step_c3_traffic.json"),
@@ -75,7 +79,9 @@ def parse():
                     help="mode 2: fixed fused tile 'width,threads,depth' instead of the autotuner")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (calibrates rocprof FETCH/WRITE_SIZE)")
-    return ap.parse_args()
+    a = ap.parse_args()
+    a.graph = (a.graph or a.workload == "c3") and not a.no_graph
+    return a
 
 
 def _orc():

@@ -73,8 +73,22 @@ __global__ __launch_bounds__(T) void k_fused_step(FusedArgs a, const ChunkDesc* 
     if (ch >= nchunks) return;
     const ChunkDesc C = chunks[ch];
     const SlotDesc S = slots[C.slot];
-    const int L = threadIdx.x;                 // lane = LDS index
-    const int z = C.z0 - FUSED_HALO + L;       // padded row of this lane
+    // Lane -> local row of the block (LDS index).  With damping strips the
+    // rows of the top and bottom strips (and those outside the window) go to
+    // the first lanes, so one wave instead of several takes the damped-row
+    // branch at every column; the other waves hold contiguous interior rows.
+    // The map is a bijection on [0, T): every row still has exactly one lane.
+    const int zb = C.z0 - FUSED_HALO;
+    int rr = threadIdx.x;
+    if (PML && S.boundary == 0) {
+        const int a0 = min(max(S.pml - zb, 0), T);
+        const int b0 = min(max(S.nzp - S.pml + 1 - zb, a0), T);
+        const int nb = T - b0;
+        const int l = threadIdx.x;
+        rr = l < a0 ? l : (l < a0 + nb ? b0 + (l - a0) : a0 + (l - a0 - nb));
+    }
+    const int L = rr;                          // local row = LDS index
+    const int z = zb + L;                      // padded row of this lane
     const int nzp = S.nzp, nxp = S.nxp;
     const bool zin = z >= 0 && z < nzp;
     const bool zown = zin && L >= FUSED_HALO && L < T - FUSED_HALO;  // rows this workgroup stores

@@ -31,8 +31,8 @@ namespace ds2d {
 
 namespace {
 
-__device__ __forceinline__ bool damp_cell2(const SlotDesc& S, int ix, int iz) {
-    return S.boundary == 0 && (ix < S.pml || ix > S.nxp - S.pml || iz < S.pml || iz > S.nzp - S.pml);
+__device__ __forceinline__ bool damp_col2(const SlotDesc& S, int ix) {
+    return S.boundary == 0 && (ix < S.pml || ix > S.nxp - S.pml);
 }
 
 // A copy in a register distinct from the source (early clobber): loaded
@@ -77,15 +77,29 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
     if (ch >= nchunks) return;
     const ChunkDesc C = chunks[ch];
     const SlotDesc S = slots[C.slot];
-    const int L = threadIdx.x;
-    const int z = C.z0 - FUSED2_HALO + L;
     const int nzp = S.nzp, nxp = S.nxp;
+    // Lane -> local row r of the block (LDS index).  With damping strips the
+    // rows of the z strips (and those outside the window) go to the first
+    // lanes, so one wave instead of three carries the damped rows of the
+    // top and bottom strips; the other waves hold contiguous interior rows.
+    const int zb = C.z0 - FUSED2_HALO;
+    int r = threadIdx.x;
+    if (PML && S.boundary == 0) {
+        const int a0 = min(max(S.pml - zb, 0), T);                // rows with z < pml
+        const int b0 = min(max(nzp - S.pml + 1 - zb, a0), T);      // first row with z > nzp - pml
+        const int nb = T - b0;
+        const int l = threadIdx.x;
+        r = l < a0 ? l : (l < a0 + nb ? b0 + (l - a0) : a0 + (l - a0 - nb));
+    }
+    const int L = r;
+    const int z = zb + r;
     const bool zin = z >= 0 && z < nzp;
     const bool zupd = z >= NOP && z < nzp - NOP;
-    const bool l1v = L >= 4 && L < T - 4;     // stage 1 velocity (its z taps read lanes L-4..L+4)
+    const bool l1v = L >= 4 && L < T - 4;     // stage 1 velocity (its z taps read rows L-4..L+4)
     const bool l1s = L >= 8 && L < T - 8;     // stage 1 stress
     const bool l2v = L >= 12 && L < T - 12;   // stage 2 velocity
     const bool zown = zin && L >= FUSED2_HALO && L < T - FUSED2_HALO;  // stage 2 stress: rows stored
+    const bool zdamp = PML && S.boundary == 0 && (z < S.pml || z > nzp - S.pml);
     const int x0 = C.x0, x1 = C.x1;
 
     __shared__ float lds[2][8][T];
@@ -147,7 +161,7 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
         o.dxv = o.dzv = o.vxx = o.vzz = 0.f;
         o.dxs = o.dzs = o.sxx = o.szz = o.sxz = 0.f;
         if (PML && zupd) {
-            if (inx(j) && damp_cell2(S, j, z)) {
+            if (inx(j) && (zdamp || damp_col2(S, j))) {
                 const uint32_t c = zo4 + (uint32_t)j * colb;
                 o.dxv = at(a.dampx, c);
                 o.dzv = at(a.dampz, c);
@@ -155,7 +169,7 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
                 o.vzz = at(splitA(3), c);
             }
             const int cs = j - 4;
-            if (inx(cs) && damp_cell2(S, cs, z)) {
+            if (inx(cs) && (zdamp || damp_col2(S, cs))) {
                 const uint32_t c = zo4 + (uint32_t)cs * colb;
                 o.dxs = at(a.dampx, c);
                 o.dzs = at(a.dampz, c);
@@ -174,7 +188,7 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
         q.dxs2 = q.dzs2 = q.sxx2 = q.szz2 = q.sxz2 = 0.f;
         if (PML && zupd) {
             const int c2 = j - 8;
-            if (l2v && inx(c2) && c2 >= x0 - 4 && c2 < x1 + 4 && damp_cell2(S, c2, z)) {
+            if (l2v && inx(c2) && c2 >= x0 - 4 && c2 < x1 + 4 && (zdamp || damp_col2(S, c2))) {
                 const uint32_t c = zo4 + (uint32_t)c2 * colb;
                 q.dxv2 = at(a.dampx, c);
                 q.dzv2 = at(a.dampz, c);
@@ -182,7 +196,7 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
                 q.vzz2 = at(scr(1), c);
             }
             const int c3 = j - 12;
-            if (zown && inx(c3) && c3 >= x0 && c3 < x1 && damp_cell2(S, c3, z)) {
+            if (zown && inx(c3) && c3 >= x0 && c3 < x1 && (zdamp || damp_col2(S, c3))) {
                 const uint32_t c = zo4 + (uint32_t)c3 * colb;
                 q.dxs2 = at(a.dampx, c);
                 q.dzs2 = at(a.dampz, c);
@@ -325,7 +339,7 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
             float ax = dxp(qxx), ex = dxm(qxz), bz = dzm(lds[p][0]), cz = dzp(lds[p][1]);
             if ((unsigned)(S.isx - j + 3) < 8u && z == S.isz) ax += w1 * tap(S.isx - j);
             if (j == S.isx) cz += w1 * tz;
-            const bool dv = PML && damp_cell2(S, j, z);
+            const bool dv = PML && (zdamp || damp_col2(S, j));
             const V2 r = vel(o.vx, o.vz, ax, ex, bz, cz, o.bx, o.dxv, o.dzv, o.vxx, o.vzz, dv);
             nvx = r.vx;
             nvz = r.vz;
@@ -352,7 +366,7 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
                 const float dvzx = dxp(qvz) * idx;
                 const float dvz = dzm(lds[p][3]) * idz;
                 const float dvxz = dzp(lds[p][2]) * idz;
-                const bool ds = PML && damp_cell2(S, cs, z);
+                const bool ds = PML && (zdamp || damp_col2(S, cs));
                 const S3 r = str(txx, tzz, txz, dvx, dvzx, dvz, dvxz, o.lam, o.mu, o.dxs, o.dzs, o.sxx, o.szz, o.sxz,
                                  ds);
                 txx = r.xx;
@@ -378,7 +392,7 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
                 float ax = dxp(sxx), ex = dxm(sxz), bz = dzm(lds[p][4]), cz = dzp(lds[p][5]);
                 if ((unsigned)(S.isx - c2 + 3) < 8u && z == S.isz) ax += w2 * tap(S.isx - c2);
                 if (c2 == S.isx) cz += w2 * tz;
-                const bool dv = PML && damp_cell2(S, c2, z);
+                const bool dv = PML && (zdamp || damp_col2(S, c2));
                 const V2 r = vel(mvx, mvz, ax, ex, bz, cz, o.bx2, q.dxv2, q.dzv2, q.vxx2, q.vzz2, dv);
                 mvx = r.vx;
                 mvz = r.vz;
@@ -412,7 +426,7 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
                     const float dvzx = dxp(wvz) * idx;
                     const float dvz = dzm(lds[p][7]) * idz;
                     const float dvxz = dzp(lds[p][6]) * idz;
-                    const bool ds = PML && damp_cell2(S, c3, z);
+                    const bool ds = PML && (zdamp || damp_col2(S, c3));
                     const S3 r = str(txx, tzz, txz, dvx, dvzx, dvz, dvxz, o.lam2, o.mu2, q.dxs2, q.dzs2, q.sxx2,
                                      q.szz2, q.sxz2, ds);
                     txx = r.xx;

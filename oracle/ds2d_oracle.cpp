@@ -1,6 +1,7 @@
 // DS2D CPU oracle — TEST INFRASTRUCTURE ONLY (see ds2d_oracle.hpp header).
 // extern "C" surface used by tests/ (ctypes) and bench.py's cpu_baseline leg.
 #include "ds2d_oracle.hpp"
+#include "ds2d_oracle_rot.hpp"
 
 #include <cstdio>
 #include <omp.h>
@@ -961,6 +962,237 @@ void adam_update(int n, int iter, const R* g, const R* bg, const R* bg2, R* s0, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// P/S-decoupled (rotation) propagation and decomposed imaging
+// ---------------------------------------------------------------------------
+
+// Rotation forward modelling, the forward loop of ROT_EXP_IMAGE_SINGLESHOT
+// (FD2DGPU.cu:3544-3551): theta += s[it] (elastic_rotation_addS_for injects
+// into txx, which is theta, FD2DGPU.cu:2046-2049 and 1407), rot_v, rot_p;
+// then vz, vx at the receivers (the elastic recorder, generalised to a
+// receiver list).  The medium needs no density: the scheme carries vp^2 and vs^2.
+// One source-side step of the rotation scheme: forward (dir 1) theta += w,
+// rot_v, rot_p; reversed (dir -1) rot_p(-1), theta += w, rot_v(-1), which
+// undoes the forward step exactly when there is no damping (rot_p rewrites
+// theta from its parts, so the injected sample must be back in theta when
+// rot_v(-1) reads it).
+template <typename R>
+void rot_source_step(const Setup<R>& S, Rot<R>& f, size_t src, R w, int dir) {
+    const Medium<R>& m = S.m;
+    if (dir > 0) {
+        f.th[src] += w;
+        rot_v(S.g, f, m.vp.data(), m.vs.data(), m.dampx.data(), m.dampz.data(), 1);
+        rot_p(S.g, f, m.dampx.data(), m.dampz.data(), 1);
+    } else {
+        rot_p(S.g, f, m.dampx.data(), m.dampz.data(), -1);
+        f.th[src] += w;
+        rot_v(S.g, f, m.vp.data(), m.vs.data(), m.dampx.data(), m.dampz.data(), -1);
+    }
+}
+
+template <typename R>
+void copy_rot4(const Rot<R>& f, size_t n, R* out4) {
+    const std::vector<R>* out[4] = {&f.th, &f.om, &f.vx, &f.vz};
+    for (int k = 0; k < 4; ++k) std::copy(out[k]->begin(), out[k]->end(), out4 + k * n);
+}
+
+template <typename R>
+void rot_forward(const orc_shot* s, const R* vp, const R* vs, const R* wav, R* rz, R* rx, R* final4) {
+    Setup<R> S = make_setup<R>(s, vp, vs, nullptr);
+    Rot<R> f;
+    f.init(S.g.n());
+    const size_t src = S.g.at(s->isx, s->isz);
+    for (int it = 0; it < s->nt; ++it) {
+        rot_source_step(S, f, src, wav[it], 1);
+        record_pair(S.g, s, f.vz, f.vx, it, rz, rx);
+    }
+    if (final4) copy_rot4(f, S.g.n(), final4);
+}
+
+// The RTM's source side after reversing the last nback of nt steps: the
+// state of step nt-1-nback (theta, omega, vx, vz), for the reversal pin.
+template <typename R>
+void rot_reconstruct(const orc_shot* s, const R* vp, const R* vs, const R* wav, int nback, R* out4) {
+    Setup<R> S = make_setup<R>(s, vp, vs, nullptr);
+    Rot<R> f;
+    f.init(S.g.n());
+    const size_t src = S.g.at(s->isx, s->isz);
+    for (int it = 0; it < s->nt; ++it) rot_source_step(S, f, src, wav[it], 1);
+    for (int it = s->nt - 1; it >= s->nt - nback; --it) rot_source_step(S, f, src, wav[it], -1);
+    copy_rot4(f, S.g.n(), out4);
+}
+
+// P/S-decomposed RTM with the rotation propagator (ROT_EXP_IMAGE_SINGLESHOT,
+// FD2DGPU.cu:3505-3585, commented out in the reference; restated):
+//   forward as rot_forward (random boundary: no damping);
+//   for it = nt-1 .. 0:
+//     source side reversed: rot_p(-1), theta += s[it], rot_v(-1).  The
+//       reference re-injects the source after rot_v(-1) (FD2DGPU.cu:3562-3564),
+//       where rot_p overwrites theta before anything reads it, so its reversal
+//       injects the source a second time instead of removing it; injecting
+//       before rot_v(-1) makes the reversal exact (documented deviation);
+//     receiver side: rot_v(+1), vz += rec_z[tr][it] and vx += rec_x[tr][it]
+//       at the receivers (GPUapplysource_backward_from_record with stack,
+//       FD2DGPU.cu:3306-3327), rot_p(+1);
+//     image on the physical window (doimage, FD2DGPU.cu:3218-3233):
+//       PP += (vpx vpx' + vpz vpz') / (vpx^2 + vpz^2 + eps),
+//       PS += (vpx vsx' + vpz vsz') / (vpx^2 + vpz^2 + eps)   (' receiver side),
+//     stacked at global column ngx_min + ix (stack_image, FD2DGPU.cu:3082-3091).
+template <typename R>
+void rot_rtm(const orc_shot* s, const R* vp, const R* vs, const R* wav, const R* oz, const R* ox, R* img_pp,
+             R* img_ps) {
+    Setup<R> S = make_setup<R>(s, vp, vs, nullptr);
+    const Grid<R>& g = S.g;
+    const Medium<R>& m = S.m;
+    const R *P = m.vp.data(), *Sv = m.vs.data(), *dxp = m.dampx.data(), *dzp = m.dampz.data();
+    Rot<R> f, b;
+    f.init(g.n());
+    b.init(g.n());
+    const size_t src = g.at(s->isx, s->isz);
+    for (int it = 0; it < s->nt; ++it) rot_source_step(S, f, src, wav[it], 1);
+    const int pml = g.pml, nx = S.nx, nz = S.nz;
+    const R eps = image_eps<R>();
+    for (int it = s->nt - 1; it >= 0; --it) {
+        rot_source_step(S, f, src, wav[it], -1);
+        rot_v(g, b, P, Sv, dxp, dzp, 1);
+        for (int r = 0; r < s->ntr; ++r) {
+            const size_t c = g.at(s->rec_ix[r], s->igz);
+            b.vz[c] += oz[(size_t)r * s->nt + it];
+            b.vx[c] += ox[(size_t)r * s->nt + it];
+        }
+        rot_p(g, b, dxp, dzp, 1);
+#pragma omp parallel for schedule(static)
+        for (int ix = 0; ix < nx; ++ix)
+            for (int iz = 0; iz < nz; ++iz) {
+                const size_t c = g.at(ix + pml, iz + pml);
+                const R sp = f.vpx[c] * f.vpx[c] + f.vpz[c] * f.vpz[c];
+                const R pp = f.vpx[c] * b.vpx[c] + f.vpz[c] * b.vpz[c];
+                const R ps = f.vpx[c] * b.vsx[c] + f.vpz[c] * b.vsz[c];
+                const size_t o = (size_t)(s->ngx_min + ix) * nz + iz;
+                img_pp[o] += pp / (sp + eps);
+                img_ps[o] += ps / (sp + eps);
+            }
+    }
+}
+
+// Floats per cell and step of the imaging quantity of each decomposed image
+// kind: 0 PP with the Poynting up/down split of vector P (up xz, down xz);
+// 1 PP, 2 PS, 3 SS vector images; 4 PP, 5 PS, 6 SS scalar images.
+inline int decomp_floats(int kind) { return kind == 0 ? 4 : kind <= 3 ? 2 : 1; }
+
+// The imaging quantity of one side at cell c (side 0 source, 1 receiver):
+// PS images the source's P against the receiver's S.
+template <typename R>
+void decomp_quantity(int kind, int side, const Grid<R>& g, size_t c, const R* VP, const R* VS,
+                     const Elastic<R>& f, R* q) {
+    const bool use_s = kind == 3 || kind == 6 || ((kind == 2 || kind == 5) && side == 1);
+    if (kind == 0) {
+        R vec[2];
+        vector_at(g, c, VP, VS, vec, (R*)nullptr);
+        poynting_split_at(f.txz[c], f.tzz[c], f.vx[c], f.vz[c], vec, q, q + 2);
+    } else if (kind <= 3) {
+        if (use_s)
+            vector_at(g, c, VP, VS, (R*)nullptr, q);
+        else
+            vector_at(g, c, VP, VS, q, (R*)nullptr);
+    } else {
+        q[0] = use_s ? VS[c] : VP[c];
+    }
+}
+
+template <typename R>
+R decomp_image(int kind, const R* h, const R* q) {
+    if (kind == 0) return image_poynting(h, h + 2, q, q + 2);
+    if (kind <= 3) return image_vector(h, q);
+    return image_scalar(h[0], q[0]);
+}
+
+// Elastic RTM with decomposed imaging (FD_ELASTIC, FD2DGPU.cu:2241-2517,
+// which no longer compiles against FD2DGPU.cuh; restated):
+//   forward, per step: txx, tzz += s[it] at the source (the live order,
+//   FD2DGPU.cu:2046), update_v, Helmholtz VP/VS (FD2DGPU.cu:1963-1986), the
+//   source quantity of `kind` on the physical window, kept per step (the
+//   reference writes 100-step blocks to disk, FD2DGPU.cu:2330-2336), update_s;
+//   backward, for it = nt-1 .. 0: txx = tzz = rec_z[tr][it] at each receiver
+//   (GPUapplysource_backward assigns, FD2DGPU.cu:2128-2134), update_v,
+//   Helmholtz, the receiver quantity, the imaging condition against the
+//   source quantity of step it, update_s.  Step nt-1 adds nothing: its
+//   backward slot is never written (FD2DGPU.cu:2420) and stays zero.
+// The image is stacked at global column ngx_min + ix.
+template <typename R>
+void decomp_rtm(const orc_shot* s, const R* vp, const R* vs, const R* rho, const R* wav, int kind, const R* oz,
+                R* img) {
+    Setup<R> S = make_setup<R>(s, vp, vs, rho);
+    const Grid<R>& g = S.g;
+    const int pml = g.pml, nx = S.nx, nz = S.nz, K = decomp_floats(kind);
+    const size_t n = g.n(), win = (size_t)nx * nz;
+    std::vector<R> hist((size_t)s->nt * win * K, R(0));
+    std::vector<R> VP(n, R(0)), VS(n, R(0));
+    Elastic<R> f;
+    f.init(n);
+    const size_t src = g.at(s->isx, s->isz);
+    for (int it = 0; it < s->nt; ++it) {
+        f.txx[src] += wav[it];
+        f.tzz[src] += wav[it];
+        update_v(g, f, S.m, 1);
+        helmholtz(g, f.vx.data(), f.vz.data(), VP.data(), VS.data());
+        R* h = hist.data() + (size_t)it * win * K;
+#pragma omp parallel for schedule(static)
+        for (int ix = 0; ix < nx; ++ix)
+            for (int iz = 0; iz < nz; ++iz)
+                decomp_quantity(kind, 0, g, g.at(ix + pml, iz + pml), VP.data(), VS.data(), f,
+                                h + ((size_t)ix * nz + iz) * K);
+        update_s(g, f, S.m, 1);
+    }
+    f.init(n);
+    std::fill(VP.begin(), VP.end(), R(0));
+    std::fill(VS.begin(), VS.end(), R(0));
+    for (int it = s->nt - 1; it >= 0; --it) {
+        for (int r = 0; r < s->ntr; ++r) {
+            const size_t c = g.at(s->rec_ix[r], s->igz);
+            f.txx[c] = oz[(size_t)r * s->nt + it];
+            f.tzz[c] = oz[(size_t)r * s->nt + it];
+        }
+        update_v(g, f, S.m, 1);
+        helmholtz(g, f.vx.data(), f.vz.data(), VP.data(), VS.data());
+        if (it < s->nt - 1) {
+            const R* h = hist.data() + (size_t)it * win * K;
+#pragma omp parallel for schedule(static)
+            for (int ix = 0; ix < nx; ++ix)
+                for (int iz = 0; iz < nz; ++iz) {
+                    R q[4];
+                    decomp_quantity(kind, 1, g, g.at(ix + pml, iz + pml), VP.data(), VS.data(), f, q);
+                    img[(size_t)(s->ngx_min + ix) * nz + iz] +=
+                        decomp_image(kind, h + ((size_t)ix * nz + iz) * K, q);
+                }
+        }
+        update_s(g, f, S.m, 1);
+    }
+}
+
+// nsteps rotation steps on a bare grid (dir 1: rot_v then rot_p; dir -1 the
+// reversal rot_p(-1) then rot_v(-1)); f12 holds the 12 fields of Rot in order.
+template <typename R>
+void rot_run(int nxp, int nzp, R dx, R dz, R dt, const R* vp, const R* vs, const R* dampx, const R* dampz,
+             int nsteps, int dir, R* f12) {
+    Grid<R> g{nxp, nzp, 0, dx, dz, dt};
+    const size_t n = g.n();
+    Rot<R> f;
+    f.init(n);
+    auto all = f.all();
+    for (size_t a = 0; a < all.size(); ++a) std::copy(f12 + a * n, f12 + (a + 1) * n, all[a]->begin());
+    for (int k = 0; k < nsteps; ++k) {
+        if (dir > 0) {
+            rot_v(g, f, vp, vs, dampx, dampz, 1);
+            rot_p(g, f, dampx, dampz, 1);
+        } else {
+            rot_p(g, f, dampx, dampz, -1);
+            rot_v(g, f, vp, vs, dampx, dampz, -1);
+        }
+    }
+    for (size_t a = 0; a < all.size(); ++a) std::copy(all[a]->begin(), all[a]->end(), f12 + a * n);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1031,7 +1263,43 @@ int orc_threads(int n) {
     int orc_ncc_residual_##SUF(int ntr, int nt, int iter, R* sz, R* sx, const R* oz, const R* ox, double* mf) { \
         ncc_residual<R>(ntr, nt, iter, sz, sx, oz, ox, mf); return 0; }                                        \
     int orc_adam_update_##SUF(int n, int iter, const R* g, const R* bg, const R* bg2, R* s0, R* r0, R* img) { \
-        adam_update<R>(n, iter, g, bg, bg2, s0, r0, img); return 0; }
+        adam_update<R>(n, iter, g, bg, bg2, s0, r0, img); return 0; }                                           \
+    int orc_rot_forward_##SUF(const orc_shot* s, const R* vp, const R* vs, const R* wav, R* rz, R* rx,        \
+                              R* final4) {                                                                     \
+        rot_forward<R>(s, vp, vs, wav, rz, rx, final4); return 0; }                                            \
+    int orc_rot_reconstruct_##SUF(const orc_shot* s, const R* vp, const R* vs, const R* wav, int nback,       \
+                                  R* out4) {                                                                   \
+        rot_reconstruct<R>(s, vp, vs, wav, nback, out4); return 0; }                                           \
+    int orc_rot_rtm_##SUF(const orc_shot* s, const R* vp, const R* vs, const R* wav, const R* oz, const R* ox, \
+                          R* ipp, R* ips) {                                                                    \
+        rot_rtm<R>(s, vp, vs, wav, oz, ox, ipp, ips); return 0; }                                              \
+    int orc_decomp_rtm_##SUF(const orc_shot* s, const R* vp, const R* vs, const R* rho, const R* wav,         \
+                             int kind, const R* oz, R* img) {                                                  \
+        if (kind < 0 || kind > 6) return -1;                                                                   \
+        decomp_rtm<R>(s, vp, vs, rho, wav, kind, oz, img); return 0; }                                         \
+    int orc_rot_run_##SUF(int nxp, int nzp, double dx, double dz, double dt, const R* vp, const R* vs,        \
+                          const R* dampx, const R* dampz, int nsteps, int dir, R* f12) {                       \
+        rot_run<R>(nxp, nzp, (R)dx, (R)dz, (R)dt, vp, vs, dampx, dampz, nsteps, dir, f12); return 0; }         \
+    int orc_helmholtz_##SUF(int nxp, int nzp, double dx, double dz, const R* vx, const R* vz, R* VP, R* VS) { \
+        Grid<R> g{nxp, nzp, 0, (R)dx, (R)dz, 1}; helmholtz<R>(g, vx, vz, VP, VS); return 0; }                 \
+    int orc_vector_##SUF(int nxp, int nzp, double dx, double dz, const R* VP, const R* VS, R* vecp, R* vecs) { \
+        Grid<R> g{nxp, nzp, 0, (R)dx, (R)dz, 1};                                                              \
+        for (int ix = NOP; ix <= nxp - NOP; ++ix)                                                              \
+            for (int iz = NOP; iz <= nzp - NOP; ++iz) {                                                        \
+                const size_t c = g.at(ix, iz);                                                                 \
+                vector_at<R>(g, c, VP, VS, vecp + 2 * c, vecs + 2 * c);                                        \
+            }                                                                                                  \
+        return 0; }                                                                                            \
+    int orc_poynting_split_##SUF(long long n, const R* txz, const R* tzz, const R* vx, const R* vz,           \
+                                 const R* vec2, R* up2, R* down2) {                                            \
+        for (long long c = 0; c < n; ++c)                                                                      \
+            poynting_split_at<R>(txz[c], tzz[c], vx[c], vz[c], vec2 + 2 * c, up2 + 2 * c, down2 + 2 * c);     \
+        return 0; }                                                                                            \
+    int orc_image_cond_##SUF(int kind, long long n, const R* fw, const R* bw, R* out) {                       \
+        if (kind < 0 || kind > 6) return -1;                                                                   \
+        const int K = decomp_floats(kind);                                                                     \
+        for (long long c = 0; c < n; ++c) out[c] = decomp_image<R>(kind, fw + c * K, bw + c * K);              \
+        return 0; }
 
 DEF_T(double, d)
 DEF_T(float, f)

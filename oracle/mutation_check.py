@@ -3,7 +3,8 @@
 Plants one-line mistakes (a dropped term, a flipped stagger, an off-by-one
 bound, a wrong grid step) in a copy of the oracle under /tmp/mut, rebuilds
 it, and runs the CPU oracle pins against each copy: every mutation must make
-at least one pin fail.  Usage: python oracle/mutation_check.py
+at least one pin fail.  Usage: python oracle/mutation_check.py [substring]
+(the optional substring keeps the mutations whose description contains it).
 """
 import os, shutil, subprocess, sys
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -37,18 +38,42 @@ MUTS = [
  ("cpp", "vz[c] = (d1 * vz[c] - ((R)1 / rr[c]) * (dt / dx) * tp) / d2;", "vz[c] = (d1 * vz[c] - ((R)1 / rr[c]) * (dt / dz) * tp) / d2;", "acoustic ref: vz dt/dz (the reference uses dt/dx)"),
  ("cpp", "    for (int ix = NOP - 1; ix <= nxp - NOP - 1; ++ix)\n        for (int iz = NOP - 1; iz <= nzp - NOP - 1; ++iz) {\n            size_t c = g.at(ix, iz);\n            R d1 = 1 - dt * damp[c] / 2, d2 = 1 + dt * damp[c] / 2;\n            R tvx",
          "    for (int ix = NOP; ix <= nxp - NOP - 1; ++ix)\n        for (int iz = NOP - 1; iz <= nzp - NOP - 1; ++iz) {\n            size_t c = g.at(ix, iz);\n            R d1 = 1 - dt * damp[c] / 2, d2 = 1 + dt * damp[c] / 2;\n            R tvx", "acoustic ref: p lower bound"),
+ ("rot", "            f.vsx[c] = (d1 * f.vsx[c] + dir * s2 * (dt / dz) * oz) / d2;",
+         "            f.vsx[c] = (d1 * f.vsx[c] + dir * s2 * (dt / dx) * oz) / d2;", "rot_v: dx for dz in vsx"),
+ ("rot", "tx += (R)C8[i] * (f.th[c + (size_t)(i - 1) * nzp] - f.th[c - (size_t)i * nzp]);",
+         "tx += (R)C8[i] * (f.th[c + (size_t)i * nzp] - f.th[c - (size_t)(i - 1) * nzp]);", "rot_v: flipped theta x stagger"),
+ ("rot", "            f.vpz[c] = (d3 * f.vpz[c] + dir * p2 * (dt / dz) * tz) / d4;",
+         "            f.vpz[c] = (d3 * f.vpz[c] + dir * s2 * (dt / dz) * tz) / d4;", "rot_v: vs^2 for vp^2"),
+ ("rot", "            f.om_x[c] = (d1 * f.om_x[c] - dir * (dt / dx) * zx) / d2;",
+         "            f.om_x[c] = (d1 * f.om_x[c] + dir * (dt / dx) * zx) / d2;", "rot_p: omega_x sign"),
+ ("rot", "            VS[c] = xz / g.dz - zx / g.dx;", "            VS[c] = xz / g.dz + zx / g.dx;", "helmholtz: curl sign"),
+ ("rot", "                xz += (R)C8[i] * (vx[c + i - 1] - vx[c - i]);",
+         "                xz += (R)C8[i] * (vx[c + i] - vx[c - i + 1]);", "helmholtz: flipped vx z stagger"),
+ ("rot", "    if (sxz) { sxz[0] = -sz / g.dz; sxz[1] = sx / g.dx; }",
+         "    if (sxz) { sxz[0] = sz / g.dz; sxz[1] = sx / g.dx; }", "vector: S sign"),
+ ("rot", "    const bool dn = p >= 0;", "    const bool dn = p <= 0;", "poynting: direction flipped"),
+ ("rot", "    const R cc = (fu[0] * bd[0] + fu[1] * bd[1]) + (fd[0] * bu[0] + fd[1] * bu[1]);",
+         "    const R cc = (fu[0] * bu[0] + fu[1] * bu[1]) + (fd[0] * bd[0] + fd[1] * bd[1]);", "poynting image: same-direction pairs"),
+ ("cpp", "        rot_p(S.g, f, m.dampx.data(), m.dampz.data(), -1);\n        f.th[src] += w;\n        rot_v(S.g, f, m.vp.data(), m.vs.data(), m.dampx.data(), m.dampz.data(), -1);",
+         "        rot_p(S.g, f, m.dampx.data(), m.dampz.data(), -1);\n        rot_v(S.g, f, m.vp.data(), m.vs.data(), m.dampx.data(), m.dampz.data(), -1);\n        f.th[src] += w;",
+         "rot reversal: the reference's injection order"),
 ]
-TESTS = ["tests/test_oracle_kernel_pins.py", "tests/test_oracle_pins.py", "tests/test_oracle_physics.py", "tests/test_oracle_eborn.py"]
+FILES = {"hpp": "ds2d_oracle.hpp", "cpp": "ds2d_oracle.cpp", "rot": "ds2d_oracle_rot.hpp"}
+TESTS = ["tests/test_oracle_kernel_pins.py", "tests/test_oracle_pins.py", "tests/test_oracle_physics.py",
+         "tests/test_oracle_eborn.py", "tests/test_oracle_rot.py"]
 root = "/tmp/mut"
 res = []
+only = sys.argv[1] if len(sys.argv) > 1 else None  # optional substring filter on the descriptions
 for k, (which, old, new, desc) in enumerate(MUTS):
+    if only and only not in desc:
+        continue
     d = f"{root}/{k}"
     shutil.rmtree(d, ignore_errors=True)
     os.makedirs(d)
     for sub in ("tests", "oracle", "ds2d_amd"):
         shutil.copytree(f"{REPO}/{sub}", f"{d}/{sub}", ignore=shutil.ignore_patterns("_build", "bin", "*.so", "__pycache__", "_ref"))
     shutil.copy(f"{REPO}/pytest.ini", d)
-    f = f"{d}/oracle/ds2d_oracle.{which}"
+    f = f"{d}/oracle/{FILES[which]}"
     src = open(f).read()
     assert src.count(old) == 1, (desc, src.count(old))
     open(f, "w").write(src.replace(old, new))

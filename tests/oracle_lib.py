@@ -324,3 +324,107 @@ def elsrtm_born(shot, vp, vs, rho, wav, img_pp, img_ps, col0, order, prec="d"):
     args = [_c(a, prec) for a in (vp, vs, rho, wav, img_pp, img_ps)]
     getattr(load(), f"orc_elsrtm_born_{prec}")(C.byref(shot), *[P(a) for a in args], col0, order, P(rz), P(rx))
     return rz, rx
+
+
+# --- P/S-decoupled rotation propagator and decomposed imaging ----------------
+
+ROT_FIELDS = ("th", "om", "th_x", "th_z", "om_x", "om_z", "vx", "vz", "vpx", "vpz", "vsx", "vsz")
+DECOMP_FLOATS = {0: 4, 1: 2, 2: 2, 3: 2, 4: 1, 5: 1, 6: 1}
+
+
+def rot_forward(shot, vp, vs, wav, prec="d", final=False):
+    """Rotation forward modelling: (rec_z, rec_x[, final theta, omega, vx, vz])."""
+    lib = load()
+    t = dtype_of(prec)
+    rz = np.zeros(shot.ntr * shot.nt, t)
+    rx = np.zeros_like(rz)
+    n = (shot.nx + 2 * shot.pml) * (shot.allnz + 2 * shot.pml)
+    f4 = np.zeros(4 * n, t) if final else None
+    vp, vs, wav = _c(vp, prec), _c(vs, prec), _c(wav, prec)
+    getattr(lib, f"orc_rot_forward_{prec}")(C.byref(shot), P(vp), P(vs), P(wav), P(rz), P(rx), P(f4))
+    return (rz, rx, f4) if final else (rz, rx)
+
+
+def rot_reconstruct(shot, vp, vs, wav, nback, prec="d"):
+    """Source side of the rotation RTM after reversing the last nback steps:
+    [4][nxp*nzp] theta, omega, vx, vz."""
+    lib = load()
+    n = (shot.nx + 2 * shot.pml) * (shot.allnz + 2 * shot.pml)
+    out = np.zeros(4 * n, dtype_of(prec))
+    vp, vs, wav = _c(vp, prec), _c(vs, prec), _c(wav, prec)
+    getattr(lib, f"orc_rot_reconstruct_{prec}")(C.byref(shot), P(vp), P(vs), P(wav), int(nback), P(out))
+    return out
+
+
+def rot_rtm(shot, vp, vs, wav, obs_z, obs_x, img_pp=None, img_ps=None, prec="d"):
+    """P/S images of one rotation RTM shot, stacked into global [allnx][allnz] images."""
+    lib = load()
+    t = dtype_of(prec)
+    A = shot.allnx * shot.allnz
+    img_pp = np.zeros(A, t) if img_pp is None else img_pp
+    img_ps = np.zeros(A, t) if img_ps is None else img_ps
+    args = [_c(a, prec) for a in (vp, vs, wav, obs_z, obs_x)]
+    getattr(lib, f"orc_rot_rtm_{prec}")(C.byref(shot), *[P(a) for a in args], P(img_pp), P(img_ps))
+    return img_pp, img_ps
+
+
+def decomp_rtm(shot, vp, vs, rho, wav, kind, obs_z, img=None, prec="d"):
+    """Decomposed elastic RTM image of one shot (kind 0..6), stacked into a global image."""
+    lib = load()
+    t = dtype_of(prec)
+    img = np.zeros(shot.allnx * shot.allnz, t) if img is None else img
+    args = [_c(a, prec) for a in (vp, vs, rho, wav)]
+    oz = _c(obs_z, prec)
+    rc = getattr(lib, f"orc_decomp_rtm_{prec}")(C.byref(shot), *[P(a) for a in args], int(kind), P(oz), P(img))
+    if rc:
+        raise ValueError(f"bad image kind {kind}")
+    return img
+
+
+def rot_run(fields12, vp, vs, dampx, dampz, nsteps, direction, dx, dz, dt, prec="d"):
+    """nsteps rotation steps on a bare [nxp][nzp] grid; fields12 [12][nxp][nzp] in ROT_FIELDS order."""
+    lib = load()
+    f = np.ascontiguousarray(fields12, dtype_of(prec)).copy()
+    nxp, nzp = f.shape[1], f.shape[2]
+    args = [_c(a, prec) for a in (vp, vs, dampx, dampz)]
+    getattr(lib, f"orc_rot_run_{prec}")(nxp, nzp, C.c_double(dx), C.c_double(dz), C.c_double(dt),
+                                        *[P(a) for a in args], int(nsteps), int(direction), P(f))
+    return f
+
+
+def helmholtz(vx, vz, dx, dz, prec="d"):
+    lib = load()
+    vx, vz = _c(vx, prec), _c(vz, prec)
+    VP, VS = np.zeros_like(vx), np.zeros_like(vx)
+    getattr(lib, f"orc_helmholtz_{prec}")(vx.shape[0], vx.shape[1], C.c_double(dx), C.c_double(dz), P(vx), P(vz),
+                                          P(VP), P(VS))
+    return VP, VS
+
+
+def vector(VP, VS, dx, dz, prec="d"):
+    """(vecP, vecS), each [nxp][nzp][2] (x, z components)."""
+    lib = load()
+    VP, VS = _c(VP, prec), _c(VS, prec)
+    vp2, vs2 = np.zeros(VP.shape + (2,), VP.dtype), np.zeros(VP.shape + (2,), VP.dtype)
+    getattr(lib, f"orc_vector_{prec}")(VP.shape[0], VP.shape[1], C.c_double(dx), C.c_double(dz), P(VP), P(VS),
+                                       P(vp2), P(vs2))
+    return vp2, vs2
+
+
+def poynting_split(txz, tzz, vx, vz, vec2, prec="d"):
+    lib = load()
+    a = [_c(x, prec) for x in (txz, tzz, vx, vz, vec2)]
+    up, down = np.zeros_like(a[4]), np.zeros_like(a[4])
+    getattr(lib, f"orc_poynting_split_{prec}")(C.c_longlong(a[0].size), *[P(x) for x in a], P(up), P(down))
+    return up, down
+
+
+def image_cond(kind, fw, bw, prec="d"):
+    lib = load()
+    fw, bw = _c(fw, prec), _c(bw, prec)
+    n = fw.size // DECOMP_FLOATS[kind]
+    out = np.zeros(n, fw.dtype)
+    rc = getattr(lib, f"orc_image_cond_{prec}")(int(kind), C.c_longlong(n), P(fw), P(bw), P(out))
+    if rc:
+        raise ValueError(f"bad image kind {kind}")
+    return out
