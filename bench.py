@@ -18,7 +18,10 @@ preconditioner: training pairs from the plain method, U-Net training, and
 plain vs learned LSRTM time to the residual target on held-out models;
 c5 = PP/PS elastic Born CG LSRTM of 256 uneven windows on 4000x1000 (a step
 = one CG iteration); unet = U-Net training throughput; compat = the
-reference-compat ELSRTM gradient main.cpp runs (26 shots on the C1 model).
+reference-compat ELSRTM gradient main.cpp runs (26 shots on the C1 model);
+rot = the C3 windows with the P/S-decoupled theta/omega scheme; decomp = the
+elastic RTM with a decomposed (Poynting / vector / scalar) imaging condition,
+26 shots on the C1 model.
 
 Prints one JSON line: value = Gcell-updates/s (one cell-update = one padded
 cell advanced one step for one wavefield); roofline of the dominant kernel
@@ -45,7 +48,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "unet", "compat"])
+    ap.add_argument("--workload", default="c3",
+                    choices=["c1", "c2", "c3", "c4", "c5", "unet", "compat", "rot", "decomp"])
+    ap.add_argument("--kind", type=int, default=0, help="decomp: image kind (0 Poynting PP, 1-3 vector, 4-6 scalar)")
     ap.add_argument("--c5-iter", type=int, default=2, help="c5: CG iterations on the full configuration")
     ap.add_argument("--train-elastic", type=int, default=0,
                     help="c5: train the U-Net on elastic PP/PS pairs of this many seeded 512x256 models first")
@@ -226,9 +231,112 @@ def run_forward(a, cfg):
             "hipgraph": a.graph},
         "roofline": roofline(stats),
     }
+    if tblock == 2 and out["roofline"]["kernel_avg_ms"] > 0:
+        # a launch advances two steps and moves the state once: also give the bandwidth a one-step-per-launch
+        # kernel would need for the same step rate (52 B per padded cell + 48 B per PML cell and step)
+        pml, nzp = cfg["pml"], cfg["nz"] + 2 * cfg["pml"]
+        cells = sum(((sh.nx + 2 * pml) * nzp + 63) // 64 * 64 for sh in cfg["shots"])
+        pml_cells = sum((sh.nx + 2 * pml - 8) * (nzp - 8) - (sh.nx + 1) * (nzp - 2 * pml + 1)
+                        for sh in cfg["shots"] if sh.boundary == 0 and pml >= 4)
+        one = float(cells * 52 + pml_cells * 48)
+        eq = 2 * one / (out["roofline"]["kernel_avg_ms"] * 1e-3) / 1e9
+        out["roofline"]["per_step_equivalent"] = {"single_step_alg_bytes": one, "achieved": round(eq, 1),
+                                                  "frac": round(eq / HBM_PEAK_GBS, 4)}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_elastic(cfg, a.cpu_seconds, wav)
     return out
+
+
+def run_rot(a, cfg):
+    """P/S-decoupled (theta/omega) forward modelling of the C3 windows: two
+    per-cell kernels per step (velocity parts, then dilatation/rotation)."""
+    from ds2d_amd import Solver
+    from ds2d_amd.lib import ricker
+    wav = ricker(cfg["nt"], cfg["f0"], cfg["dt"])
+    nx_max = max(s.nx for s in cfg["shots"])
+    s = Solver(dx=cfg["dx"], dz=cfg["dz"], dt=cfg["dt"], nt=cfg["nt"], allnx=cfg["nx"], allnz=cfg["nz"],
+               pml=cfg["pml"], nx_max=nx_max, wavelet=wav, pml_rr=1e-3)
+    s.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
+    for _ in range(a.warmup):
+        s.rot_forward(cfg["shots"], fetch=False)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        _, _, st = s.rot_forward(cfg["shots"], fetch=False)
+        stats.append(st)
+    elapsed = time.perf_counter() - t0
+    rz, rx, _ = s.rot_forward(cfg["shots"][:2], fetch=True)  # untimed validation
+    assert np.isfinite(rz).all() and np.isfinite(rx).all() and np.abs(rz).max() > 0
+    cu = stats[0]["cell_updates"]
+    out = {
+        "metric": "Gcell-updates/s/GPU", "value": round(cu * a.steps / elapsed / 1e9, 3), "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(elapsed / a.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic: seeded layered+faulted elastic model (seed {cfg['seed']}), vs = vp/1.73",
+        "config": {"workload": cfg["desc"], "shots": len(cfg["shots"]), "nt": cfg["nt"], "cell_updates_per_step": cu,
+                   "kernels": "k_rot_v + k_rot_p per step (per-cell, reference layout of the rotation scheme)",
+                   "parallelism": "shots batched on one GPU"},
+        "roofline": roofline(stats),
+    }
+    if not a.no_cpu_baseline:
+        orc = _orc()
+        threads = orc.threads(0)
+        shot = cfg["shots"][0]
+        nxp, nzp = shot.nx + 2 * cfg["pml"], cfg["nz"] + 2 * cfg["pml"]
+
+        def run(nsteps):
+            o = orc.make_shot(shot, cfg["nx"], cfg["nz"], cfg["pml"], nsteps, cfg["dx"], cfg["dz"], cfg["dt"])
+            t = time.perf_counter()
+            orc.rot_forward(o, cfg["vp"], cfg["vs"], wav[:nsteps], prec="f")
+            return time.perf_counter() - t
+
+        steps, t = _cpu_sample(run, cfg["nt"], min(40, cfg["nt"]), a.cpu_seconds)
+        out["cpu_baseline"] = _cpu_record(
+            round(nxp * nzp * steps / t / 1e9, 4), threads,
+            f"FP32 oracle rotation forward, shot 0 only ({shot.nx}-column window, {nxp}x{nzp} padded), {steps} of "
+            f"{cfg['nt']} steps, best of 3 runs: {t:.2f} s, OpenMP threads={threads}")
+    return out
+
+
+def run_decomp(a):
+    """Elastic RTM with a decomposed imaging condition (default: the Poynting
+    up/down PP image) for 26 shots 10 nodes apart on the 251x101 C1 model
+    (pml 20, 2500 steps), on records of the library's own forward modelling."""
+    from ds2d_amd import Solver, models
+    from ds2d_amd.lib import ricker
+    cfg = models.c1_elastic_forward()
+    nt = a.nt or cfg["nt"]
+    wav = ricker(nt, cfg["f0"], cfg["dt"])
+    nx, nz, pml = cfg["nx"], cfg["nz"], cfg["pml"]
+    from ds2d_amd.lib import ShotSpec
+    shots = [ShotSpec(0, nx, pml + x, pml + 2, pml, pml + np.arange(nx, dtype=np.int32), shot_id=i)
+             for i, x in enumerate(range(0, nx, 10))][:a.nshots or 26]
+    s = Solver(dx=cfg["dx"], dz=cfg["dz"], dt=cfg["dt"], nt=nt, allnx=nx, allnz=nz, pml=pml, nx_max=nx, wavelet=wav)
+    s.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
+    rz, _, _ = s.batch_forward(shots, mode=2)
+    for _ in range(a.warmup):
+        s.decomp_rtm(shots, a.kind, rz)
+    times = []
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        img = s.decomp_rtm(shots, a.kind, rz)
+        times.append(time.perf_counter() - t0)
+    assert np.isfinite(img).all() and np.abs(img).max() > 0
+    elapsed = float(np.mean(times))
+    cu = len(shots) * (nx + 2 * pml) * (nz + 2 * pml) * nt * 2  # source and receiver wavefields
+    names = {0: "PP, Poynting up/down split of vector P", 1: "PP vector", 2: "PS vector", 3: "SS vector",
+             4: "PP scalar", 5: "PS scalar", 6: "SS scalar"}
+    return {
+        "metric": "Gcell-updates/s/GPU", "value": round(cu / elapsed / 1e9, 3), "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(elapsed * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded layered+faulted elastic model (seed 1), records of the library's forward modelling",
+        "config": {"workload": f"decomp_rtm: 251x101, dx=dz=10 m, pml 20, dt 0.8 ms, {nt} steps, {len(shots)} shots "
+                               f"10 nodes apart, image {names[a.kind]}",
+                   "kernels": "k_ref_v/k_ref_s + k_helmholtz + k_decomp per step, source quantity resident in HBM",
+                   "parallelism": "shots batched on one GPU"},
+        "roofline": None,
+    }
 
 
 def run_lsrtm(a, cfg):
@@ -387,6 +495,7 @@ def run_learned(a):
             row["time_plain_reference_scheme_s"] = round(tr, 2)
             row["iters_plain_reference_scheme"] = LR.first_hit(rr_, a.target)
         rows.append(row)
+        log("row " + json.dumps(row))  # each model's record survives a run that stops early
         log(f"seed {seed}: learned {kl} it {tl:.1f} s, plain {kp} it {tp:.1f} s, "
             f"reference scheme {row.get('time_plain_reference_scheme_s', '-')} s "
             f"(extrapolated {row.get('time_plain_reference_scheme_extrapolated_s', '-')} s)")
@@ -625,6 +734,14 @@ def main():
         cfg["desc"] = (f"c3_batched_elastic: {cfg['nx']}x{cfg['nz']} model, dx=dz={cfg['dx']} m, {len(cfg['shots'])} "
                        f"uneven windows (400-1200 cols), {edge}, dt {cfg['dt'] * 1e3:.1f} ms, {cfg['nt']} steps")
         out = run_forward(a, cfg)
+    elif a.workload == "rot":
+        cfg = models.c3_batched_elastic(nshots=a.nshots or 128, nt=a.nt or 3000)
+        cfg["seed"] = 3
+        cfg["desc"] = (f"rot_batched: the {len(cfg['shots'])} C3 windows ({cfg['nx']}x{cfg['nz']} model, dx=dz="
+                       f"{cfg['dx']} m, pml {cfg['pml']}, {cfg['nt']} steps) with the P/S-decoupled theta/omega scheme")
+        out = run_rot(a, cfg)
+    elif a.workload == "decomp":
+        out = run_decomp(a)
     elif a.workload == "c1":
         cfg = models.c1_elastic_forward()
         if a.nt:

@@ -839,7 +839,7 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
             // temporal blocking of depth 2: steps (it, it+1) per launch; an odd
             // last step runs the single-step kernel on its own chunk list
             const int T2 = fused2_threads_ok(T) ? T : 704;
-            const int D2 = D >= 2 ? 2 : 1;
+            const int D2 = 2;  // two columns in flight
             std::vector<ChunkDesc> ch2;
             const int rows2 = T2 - 2 * FUSED2_HALO;
             for (int s = 0; s < nshots; ++s)
@@ -997,7 +997,7 @@ int ds2d_set_chunk(ds2d_t* h, int width, int threads, int depth) {
     if (width < 0 || (width > 0 && width < 8)) return fail(DS2D_EINVAL, "chunk width must be 0 or >= 8");
     if (threads != 0 && !fused_threads_ok(threads))
         return fail(DS2D_EINVAL, "threads must be 0, 256, 512, 704 or 768");
-    if (depth != 0 && depth != 1 && depth != 2 && depth != 4) return fail(DS2D_EINVAL, "depth must be 0, 1, 2 or 4");
+    if (depth != 0 && depth != 2 && depth != 4) return fail(DS2D_EINVAL, "depth must be 0, 2 or 4");
     h->chunk_w = width;
     h->chunk_t = threads;
     h->chunk_d = depth;
@@ -1018,7 +1018,7 @@ int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nst
     static const Cand cand[] = {{0, 0, 0, 1},     {32, 256, 2, 1},  {64, 256, 2, 1},  {64, 256, 4, 1},
                                 {128, 256, 4, 1}, {64, 512, 4, 1},  {64, 768, 2, 1},  {64, 768, 4, 1},
                                 {128, 768, 4, 1}, {64, 704, 4, 1},  {112, 704, 4, 1}, {160, 704, 4, 1},
-                                {64, 704, 1, 2},  {128, 704, 1, 2}, {128, 704, 2, 2}};
+                                {64, 704, 2, 2},  {128, 704, 2, 2}};
     std::vector<Cand> cands(std::begin(cand), std::end(cand));
     // 704-lane tiles whose chunk count just fills R rounds of one workgroup per CU
     // (R = 2, 3, 4): the step time moves with the number of rounds, not with the width alone
@@ -1046,7 +1046,6 @@ int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nst
             for (const Cand& c : cands) have = have || (c.w == lo && c.t == 704 && c.d == 4);
             if (!have) cands.push_back(Cand{lo, 704, 4, 1});
             // the same widths two steps per launch (704 lanes own 672 rows; 640-row windows stay one block)
-            cands.push_back(Cand{lo, 704, 1, 2});
             cands.push_back(Cand{lo, 704, 2, 2});
         }
     }
@@ -2267,6 +2266,209 @@ int ds2d_sjcg_update(int n, float* m, float* cg, const float* g1, float* g0, int
     if (alpha_out) *alpha_out = alpha;
     if (beta_out) *beta_out = beta;
     return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// P/S-decoupled (theta/omega rotation) modelling and RTM; elastic RTM with
+// decomposed (vector P/S, Poynting up/down) imaging
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+int alloc_rot(ds2d_ctx* h, const char* tag, long long n, RotFieldT<T>& f) {
+    T** ptrs[12] = {&f.th, &f.om, &f.th_x, &f.th_z, &f.om_x, &f.om_z, &f.vx, &f.vz, &f.vpx, &f.vpz, &f.vsx, &f.vsz};
+    T* base;
+    std::string name = std::string("rot_") + tag;
+    int rc = h->fbuf(name.c_str(), (size_t)n * 12, &base);
+    if (rc) return rc;
+    for (int k = 0; k < 12; ++k) *ptrs[k] = base + (size_t)k * n;
+    HIP_TRY(hipMemsetAsync(base, 0, (size_t)n * 12 * sizeof(T), h->st));
+    return 0;
+}
+
+// Entry checks, batch and padded medium of the rotation and decomposed paths.
+int rot_setup(ds2d_ctx* h, int nshots, const ds2d_shot* shots, Batch& B, Med& m) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!h->have_model) return fail(DS2D_ESTATE, "no model");
+    if (h->g.physics != 0) return fail(DS2D_EINVAL, "needs physics = 0 (elastic)");
+    if (h->g.pml < NOP) return fail(DS2D_EINVAL, "needs a frame of at least %d cells", NOP);
+    if ((rc = build_batch(h, nshots, shots, B))) return rc;
+    if ((rc = alloc_med(h, B.total_cells, m))) return rc;
+    launch_setup(h->st, B.shape, setup_args(h, false), m);
+    return 0;
+}
+
+// Records concatenated in shot order ([ntr_s][nt] each) into a device arena.
+int upload_records(ds2d_ctx* h, const Batch& B, const char* name, const float* host, float** dev) {
+    int rc;
+    if ((rc = h->fbuf(name, std::max<long long>(B.total_rec, 1), dev))) return rc;
+    if (B.total_rec > 0) HIP_TRY(hipMemcpyAsync(*dev, host, B.total_rec * 4, hipMemcpyHostToDevice, h->st));
+    return 0;
+}
+
+// Window images (nimg x total_win) -> global images (nimg x allnx*allnz) -> host.
+int gather_out(ds2d_ctx* h, const Batch& B, const float* win, int nimg, float* const* out) {
+    int rc;
+    const long long A = (long long)h->g.allnx * h->g.allnz;
+    float* g;
+    if ((rc = h->fbuf("rot_glob", (size_t)A * nimg, &g))) return rc;
+    HIP_TRY(hipMemsetAsync(g, 0, (size_t)A * nimg * 4, h->st));
+    launch_gather(h->st, B.shape, win, B.total_win, g, A, nimg, h->g.allnx, h->g.allnz);
+    for (int k = 0; k < nimg; ++k)
+        if (out[k]) HIP_TRY(hipMemcpyAsync(out[k], g + (size_t)k * A, A * 4, hipMemcpyDeviceToHost, h->st));
+    HIP_TRY(hipStreamSynchronize(h->st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ds2d_rot_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec_z, float* rec_x,
+                     ds2d_stats* stats) {
+    Batch B;
+    Med m;
+    int rc;
+    if ((rc = rot_setup(h, nshots, shots, B, m))) return rc;
+    const BatchShape& sh = B.shape;
+    hipStream_t st = h->st;
+    const int nt = h->g.nt;
+    RotField f;
+    if ((rc = alloc_rot(h, "f", B.total_cells, f))) return rc;
+    const size_t R = std::max<long long>(B.total_rec, 1);
+    float* rs;
+    if ((rc = h->fbuf("frec", R * 2, &rs))) return rc;
+    StepSampler smp;
+    if ((rc = smp.init(nt))) return rc;
+    stats_begin(h);
+    stats_loop_begin(h);
+    for (int it = 0; it < nt; ++it) {
+        const bool tk = smp.take(it);
+        launch_rot_src(st, sh, f.th, h->d_wav, it);
+        if (tk) smp.mark(st, 0);
+        launch_rot_v(st, sh, f, m, 1.f);
+        if (tk) smp.mark(st, 1);
+        launch_rot_p(st, sh, f, m, 1.f);
+        if (tk) smp.mark(st, 2);
+        launch_record(st, sh, f.vz, f.vx, rs, rs + R, nullptr, nullptr, nullptr, nullptr, it);
+    }
+    stats_loop_end(h);
+    if ((rc = stats_end(h, stats, padded_cells(B) * nt, nt, 0))) return rc;
+    double kv, kp;
+    smp.averages(&kv, &kp);
+    // per padded cell: rot_v reads theta, omega, 4 velocity parts, vp, vs, dampx, dampz and writes
+    // 4 parts + vx, vz (16 floats); rot_p reads vx, vz, 4 splits, dampx, dampz and writes 4 splits
+    // + theta, omega (14 floats)
+    const double bv = (double)B.total_cells * 16 * 4, bp = (double)B.total_cells * 14 * 4;
+    const bool p_dom = kp >= kv;
+    h->last.kernel_avg_ms = p_dom ? kp : kv;
+    h->last.alg_bytes = p_dom ? bp : bv;
+    h->last.aux_kernel_avg_ms = p_dom ? kv : kp;
+    h->last.aux_alg_bytes = p_dom ? bv : bp;
+    h->last.kernel_samples = smp.n;
+    if (stats) *stats = h->last;
+    if (rec_z) HIP_TRY(hipMemcpyAsync(rec_z, rs, B.total_rec * 4, hipMemcpyDeviceToHost, st));
+    if (rec_x) HIP_TRY(hipMemcpyAsync(rec_x, rs + R, B.total_rec * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int ds2d_rot_rtm(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* rec_z, const float* rec_x,
+                 float* img_pp, float* img_ps) {
+    if (!rec_z || !rec_x) return fail(DS2D_EINVAL, "records required");
+    for (int s = 0; s < nshots && shots; ++s)
+        if (shots[s].boundary != 1)
+            return fail(DS2D_EINVAL,
+                        "shot %d: the rotation RTM rebuilds its source wavefield by reversal, which needs the "
+                        "random boundary (boundary = 1)", s);
+    Batch B;
+    Med m;
+    int rc;
+    if ((rc = rot_setup(h, nshots, shots, B, m))) return rc;
+    const BatchShape& sh = B.shape;
+    hipStream_t st = h->st;
+    const int nt = h->g.nt;
+    RotFieldD f;  // source side: rebuilt by reversal and imaged normalised by its own energy
+    RotField b;
+    if ((rc = alloc_rot(h, "src", B.total_cells, f)) || (rc = alloc_rot(h, "rcv", B.total_cells, b))) return rc;
+    float *dz, *dx, *w2;
+    if ((rc = upload_records(h, B, "obs_z", rec_z, &dz)) || (rc = upload_records(h, B, "obs_x", rec_x, &dx)))
+        return rc;
+    if ((rc = h->fbuf("rot_win", (size_t)B.total_win * 2, &w2))) return rc;
+    HIP_TRY(hipMemsetAsync(w2, 0, (size_t)B.total_win * 2 * 4, st));
+    stats_begin(h);
+    stats_loop_begin(h);
+    for (int it = 0; it < nt; ++it) {
+        launch_rot_src(st, sh, f.th, h->d_wav, it);
+        launch_rot_v(st, sh, f, m, 1.f);
+        launch_rot_p(st, sh, f, m, 1.f);
+    }
+    for (int it = nt - 1; it >= 0; --it) {
+        // reversal of step it: the sample is back in theta before rot_v(-1) reads it
+        launch_rot_p(st, sh, f, m, -1.f);
+        launch_rot_src(st, sh, f.th, h->d_wav, it);
+        launch_rot_v(st, sh, f, m, -1.f);
+        launch_rot_v(st, sh, b, m, 1.f);
+        launch_inject_res(st, sh, b.vz, b.vx, dz, dx, it);
+        launch_rot_p(st, sh, b, m, 1.f);
+        launch_rot_image(st, sh, f, b, w2, B.total_win);
+    }
+    stats_loop_end(h);
+    if ((rc = stats_end(h, nullptr, padded_cells(B) * nt * 3, 0, 0))) return rc;
+    float* out[2] = {img_pp, img_ps};
+    return gather_out(h, B, w2, 2, out);
+}
+
+int ds2d_decomp_rtm(ds2d_t* h, int nshots, const ds2d_shot* shots, int kind, const float* rec_z, float* image) {
+    if (kind < 0 || kind > 6) return fail(DS2D_EINVAL, "image kind must be 0..6");
+    if (!rec_z) return fail(DS2D_EINVAL, "records required");
+    Batch B;
+    Med m;
+    int rc;
+    if ((rc = rot_setup(h, nshots, shots, B, m))) return rc;
+    const BatchShape& sh = B.shape;
+    hipStream_t st = h->st;
+    const int nt = h->g.nt;
+    const long long n = B.total_cells, W = B.total_win, K = decomp_floats(kind);
+    EField f;
+    if ((rc = alloc_efield(h, "bg", n, f))) return rc;
+    float *vps, *hist, *win, *dz;
+    if ((rc = h->fbuf("helm", (size_t)n * 2, &vps))) return rc;
+    if ((rc = h->fbuf("decomp_hist", (size_t)nt * W * K, &hist))) return rc;
+    if ((rc = h->fbuf("rot_win", (size_t)W, &win))) return rc;
+    if ((rc = upload_records(h, B, "obs_z", rec_z, &dz))) return rc;
+    float *VP = vps, *VS = vps + n;
+    HIP_TRY(hipMemsetAsync(vps, 0, (size_t)n * 2 * 4, st));
+    HIP_TRY(hipMemsetAsync(win, 0, (size_t)W * 4, st));
+    const Strain none{nullptr, nullptr, nullptr, nullptr};
+    stats_begin(h);
+    stats_loop_begin(h);
+    for (int it = 0; it < nt; ++it) {
+        launch_inject_src(st, sh, f.txx, f.tzz, h->d_wav, it);
+        launch_ref_v(st, sh, f, m, 1.f, 0);
+        launch_helmholtz(st, sh, f.vx, f.vz, VP, VS);
+        launch_decomp(st, sh, kind, VP, VS, f, hist + (size_t)it * W * K, nullptr, 1);
+        launch_ref_s(st, sh, f, m, none, 1.f, 0);
+    }
+    // receiver side from rest; step nt-1 is not imaged (its backward slot is never written)
+    HIP_TRY(hipMemsetAsync(f.vx, 0, (size_t)n * 15 * 4, st));
+    HIP_TRY(hipMemsetAsync(vps, 0, (size_t)n * 2 * 4, st));
+    for (int it = nt - 1; it >= 0; --it) {
+        launch_assign_rec(st, sh, f.txx, f.tzz, dz, it);
+        launch_ref_v(st, sh, f, m, 1.f, 0);
+        launch_helmholtz(st, sh, f.vx, f.vz, VP, VS);
+        if (it < nt - 1) launch_decomp(st, sh, kind, VP, VS, f, hist + (size_t)it * W * K, win, 0);
+        launch_ref_s(st, sh, f, m, none, 1.f, 0);
+    }
+    stats_loop_end(h);
+    if ((rc = stats_end(h, nullptr, padded_cells(B) * nt * 2, 0, 0))) return rc;
+    float* out[1] = {image};
+    return gather_out(h, B, win, 1, out);
 }
 
 // main.cpp:461-473 ricker1.

@@ -185,6 +185,37 @@ void launch_eb_adj_v(hipStream_t st, const BatchShape& b, EbField lm, EbAux x, E
 void launch_eb_adj_s(hipStream_t st, const BatchShape& b, EbField lm, EbAux x, EbField bg, EbMed m,
                      const float* hist, long long hstride, float* gpp, float* gps, int it, int undo);
 
+// --- P/S-decoupled rotation scheme and decomposed imaging (kernels_rot.hip) --
+// The rotation wavefield (the reference aliases it onto the split elastic
+// arrays, FD2DGPU.cu:1402-1412): dilatation theta and rotation omega with
+// their x/z splits, the velocity and its P and S parts.
+template <typename T>
+struct RotFieldT {
+    T *th, *om, *th_x, *th_z, *om_x, *om_z, *vx, *vz, *vpx, *vpz, *vsx, *vsz;
+};
+using RotField = RotFieldT<float>;
+using RotFieldD = RotFieldT<double>;
+// Velocity parts from theta/omega, then theta/omega from the velocity (uses
+// Med vp, vs, dampx, dampz).  dir = -1 with rot_p first reverses a step.
+void launch_rot_v(hipStream_t st, const BatchShape& b, RotField f, Med m, float dir);
+void launch_rot_v(hipStream_t st, const BatchShape& b, RotFieldD f, Med m, float dir);
+void launch_rot_p(hipStream_t st, const BatchShape& b, RotField f, Med m, float dir);
+void launch_rot_p(hipStream_t st, const BatchShape& b, RotFieldD f, Med m, float dir);
+void launch_rot_src(hipStream_t st, const BatchShape& b, float* th, const float* wav, int it);
+void launch_rot_src(hipStream_t st, const BatchShape& b, double* th, const float* wav, int it);
+// PP and PS of the rotation RTM into the window images w2 (2 x wstride).
+void launch_rot_image(hipStream_t st, const BatchShape& b, RotFieldD src, RotField rcv, float* w2, long long wstride);
+void launch_helmholtz(hipStream_t st, const BatchShape& b, const float* vx, const float* vz, float* VP, float* VS);
+// Floats per window cell and step of a decomposed image kind (0..6).
+int decomp_floats(int kind);
+// store = 1: the source quantity into hist (this step's block of
+// decomp_floats(kind) x total_win floats); store = 0: the receiver quantity
+// imaged against hist into the window image win.  Needs pml >= NOP.
+void launch_decomp(hipStream_t st, const BatchShape& b, int kind, const float* VP, const float* VS, EField f,
+                   float* hist, float* win, int store);
+// txx = tzz = rec_z[tr][it] at each receiver (assignment).
+void launch_assign_rec(hipStream_t st, const BatchShape& b, float* txx, float* tzz, const float* rz, int it);
+
 // --- fused single-kernel elastic step (kernels_fused.hip) -------------------
 struct ChunkDesc {
     int slot, x0, x1, z0;  // columns [x0, x1) of slot, rows from z0 (block of threads - 2*FUSED_HALO rows)
@@ -219,7 +250,7 @@ void launch_fused_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, 
 // Two time steps per launch (steps it and it+1, it+1 < nt): workgroups own
 // threads - 2*FUSED2_HALO rows.  scratch: 5 fields of n floats (the damped
 // cells' intermediate splits; unused when pml is false).  depth: columns of
-// operands in flight per lane (1 or 2).
+// operands in flight per lane (2).
 constexpr int FUSED2_HALO = 16;
 bool fused2_threads_ok(int threads);
 void launch_fused2_step(hipStream_t st, const BatchShape& b, const FusedArgs& a, float* scratch,

@@ -447,22 +447,16 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
         }
     };
 
-    // march columns [x0-12, x1+12) in passes of PF columns
-    const int jend = j0 + (x1 + 12 - j0 + PF - 1) / PF * PF;
-    if (PF == 1) {
-        Col2 cA = fetch(j0);
-        for (int j = j0; j < jend; ++j) {
-            column(j, cA, j & 1);
-            cA = fetch(j + 1);
-        }
-    } else {
-        Col2 cA = fetch(j0), cB = fetch(j0 + 1);
-        for (int j = j0; j < jend; j += 2) {
-            column(j, cA, 0);
-            cA = fetch(j + 2);
-            column(j + 1, cB, 1);
-            cB = fetch(j + 3);
-        }
+    // march columns [x0-12, x1+12) in passes of two columns, the operands of
+    // the next two in flight
+    static_assert(PF == 2, "two steps per launch march with two columns in flight");
+    const int jend = j0 + (x1 + 12 - j0 + 1) / 2 * 2;
+    Col2 cA = fetch(j0), cB = fetch(j0 + 1);
+    for (int j = j0; j < jend; j += 2) {
+        column(j, cA, 0);
+        cA = fetch(j + 2);
+        column(j + 1, cB, 1);
+        cB = fetch(j + 3);
     }
 }
 
@@ -484,18 +478,11 @@ void launch_fused2_step(hipStream_t st, const BatchShape& b, const FusedArgs& a,
         DS2D_F2_LAUNCH(704, PP, MM);    \
     else                                \
         DS2D_F2_LAUNCH(768, PP, MM);
+    (void)depth;  // two columns in flight (the only depth of this kernel)
     if (pml) {
-        if (depth == 2) {
-            DS2D_F2_THREADS(2, true)
-        } else {
-            DS2D_F2_THREADS(1, true)
-        }
+        DS2D_F2_THREADS(2, true)
     } else {
-        if (depth == 2) {
-            DS2D_F2_THREADS(2, false)
-        } else {
-            DS2D_F2_THREADS(1, false)
-        }
+        DS2D_F2_THREADS(2, false)
     }
 #undef DS2D_F2_THREADS
 #undef DS2D_F2_LAUNCH

@@ -118,6 +118,9 @@ SIGNATURES = {
     "ds2d_laplace_band": (C.c_int, [C.c_int, C.c_int, _F]),
     "ds2d_laplace_band_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ds2d_ricker": (C.c_int, [C.c_int, C.c_float, C.c_float, _F]),
+    "ds2d_rot_forward": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, C.POINTER(Stats)]),
+    "ds2d_rot_rtm": (C.c_int, [_H, C.c_int, C.POINTER(Shot), _F, _F, _F, _F]),
+    "ds2d_decomp_rtm": (C.c_int, [_H, C.c_int, C.POINTER(Shot), C.c_int, _F, _F]),
 }
 
 
@@ -275,6 +278,31 @@ class Solver:
         arr = _shots_array(shots)
         _check(self.lib.ds2d_batch_forward(self.h, len(shots), arr, fptr(rz), fptr(rx), mode, C.byref(st)))
         return rz, rx, st.as_dict()
+
+    def rot_forward(self, shots, fetch=True):
+        """P/S-decoupled (theta/omega) forward modelling: (rec_z, rec_x, stats)."""
+        rz, rx = (self._rec(shots), self._rec(shots)) if fetch else (None, None)
+        st = Stats()
+        _check(self.lib.ds2d_rot_forward(self.h, len(shots), _shots_array(shots), fptr(rz), fptr(rx), C.byref(st)))
+        return rz, rx, st.as_dict()
+
+    def rot_rtm(self, shots, rec_z, rec_x):
+        """P/S-decomposed rotation RTM (random-boundary shots): (PP, PS) images [allnx][allnz]."""
+        rz = np.ascontiguousarray(rec_z, np.float32)
+        rx = np.ascontiguousarray(rec_x, np.float32)
+        pp = np.zeros((self.allnx, self.allnz), np.float32)
+        ps = np.zeros_like(pp)
+        _check(self.lib.ds2d_rot_rtm(self.h, len(shots), _shots_array(shots), fptr(rz), fptr(rx), fptr(pp),
+                                     fptr(ps)))
+        return pp, ps
+
+    def decomp_rtm(self, shots, kind, rec_z):
+        """Elastic RTM with a decomposed imaging condition (kind 0 Poynting PP; 1-3 vector PP/PS/SS;
+        4-6 scalar PP/PS/SS): image [allnx][allnz]."""
+        rz = np.ascontiguousarray(rec_z, np.float32)
+        img = np.zeros((self.allnx, self.allnz), np.float32)
+        _check(self.lib.ds2d_decomp_rtm(self.h, len(shots), _shots_array(shots), int(kind), fptr(rz), fptr(img)))
+        return img
 
     def batch_gradient(self, shots, obs_z, obs_x, it):
         mf = C.c_double(0.0)

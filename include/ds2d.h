@@ -36,6 +36,10 @@
  *   ds2d_laplace_band      laplace_filter_2d(nz,nx,dat)          laplace.cpp:3-57
  *   ds2d_ricker            ricker1                               main.cpp:461-473
  *   ds2d_sjcg_update       sjcgbeta + sjcglength + sjcgsolver    main.cpp:1232-1303
+ *   ds2d_rot_forward       rot_exp_calculate_v/_p + elastic_rotation_addS_for
+ *                                                            FD2DGPU.cu:3387-3424, 2046-2049
+ *   ds2d_rot_rtm           ROT_EXP_IMAGE_SINGLESHOT (commented out) FD2DGPU.cu:3505-3585
+ *   ds2d_decomp_rtm        FD_ELASTIC with VECTORIMG / POYNTING  FD2DGPU.cu:2241-2517
  */
 #ifndef DS2D_H
 #define DS2D_H
@@ -157,7 +161,7 @@ int ds2d_batch_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec
 /* Tile of the fused kernel: x-chunk width (0 = default 64), workgroup size
  * (256, 512, 704 or 768 lanes, one z row each, owning threads-16 rows; 0 =
  * default 256) and prefetch depth (columns of operands in flight, 2 or 4;
- * 0 = 2; with two steps per launch 1 or 2); and an autotuner that times
+ * 0 = 2; two steps per launch always run with 2); and an autotuner that times
  * nsteps per candidate on this batch -- the fused kernel at several tiles,
  * one or two time steps per launch, and the two-kernel step (reported as
  * width 0) -- and keeps the fastest for mode 2.  No reference counterpart
@@ -317,6 +321,43 @@ int ds2d_laplace_band(int nz, int nx, float* dat);
  * overlap on one GPU share that array and race. */
 int ds2d_laplace_band_noalloc(int nz, int nx, const float* in, float* out, void* stream);
 int ds2d_ricker(int nt, float f0, float dt, float* out);
+
+/* P/S-decoupled (theta/omega rotation) elastic modelling of a batch: per step
+ * theta += s[it] at the source (elastic_rotation_addS_for, FD2DGPU.cu:2046-2049,
+ * theta aliased to txx at 1407), GPUcalculate_rotation_v then _p
+ * (FD2DGPU.cu:3005-3080; undamped forms 2545-2609), then vz, vx at
+ * (rec_ix, igz).  PML or random boundary per shot.  The medium is vp and vs
+ * (the scheme carries vp^2 and vs^2, no density).  Records concatenated in
+ * shot order, each [ntr_s][nt].  stats may be null. */
+int ds2d_rot_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec_z, float* rec_x,
+                     ds2d_stats* stats);
+/* P/S-decomposed RTM with the rotation scheme (ROT_EXP_IMAGE_SINGLESHOT,
+ * FD2DGPU.cu:3505-3585): the source wavefield is rebuilt by reversal through
+ * the random boundary (every shot needs boundary = 1); the receiver side
+ * adds rec_z / rec_x to vz / vx at the receivers (FD2DGPU.cu:3306-3327); at
+ * every step PP += (vpx vpx' + vpz vpz') / (vpx^2 + vpz^2 + 1e-6) and PS the
+ * same with the receiver's S parts (doimage, FD2DGPU.cu:3218-3233), stacked
+ * over shots into allnx*allnz images (stack_image, FD2DGPU.cu:3082-3091).
+ * Deviation: the reversal re-injects each source sample before the velocity
+ * update, which makes it exact (the reference's order injects it twice). */
+int ds2d_rot_rtm(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* rec_z, const float* rec_x,
+                 float* img_pp, float* img_ps);
+/* Elastic RTM with decomposed imaging (FD_ELASTIC, FD2DGPU.cu:2241-2517):
+ * source side propagated forward with the live elastic step, its imaging
+ * quantity kept per step in device memory (the reference stages it on disk);
+ * receiver side propagated from rest with txx = tzz = rec_z[tr][it] assigned
+ * at the receivers (GPUapplysource_backward, FD2DGPU.cu:2128-2134) for
+ * it = nt-1 .. 0, imaged at every step but the first.  Per step, on the
+ * physical window: Helmholtz VP/VS (FD2DGPU.cu:1963-1986), vector VP/VS
+ * (1988-2017), and by kind
+ *   0  PP, vector P split up/down by the Poynting flux (2223-2236),
+ *      (fu.bd + fd.bu) / (fu.fu + fd.fd + 1e-6)  (doimage_poynting, 2164-2196)
+ *   1  PP, 2 PS, 3 SS vector images: f.b / (f.f + 1e-6)   (doimage, 2141-2162)
+ *   4  PP, 5 PS, 6 SS scalar images: f b / (f^2 + 1e-6)
+ * (PS: the source's P against the receiver's S).  Stacked over shots into
+ * one allnx*allnz image.  Device memory: nt x window cells x (4, 2 or 1)
+ * floats for the source side. */
+int ds2d_decomp_rtm(ds2d_t* h, int nshots, const ds2d_shot* shots, int kind, const float* rec_z, float* image);
 
 #ifdef __cplusplus
 }

@@ -85,7 +85,7 @@ def test_fused_tiles_match_oracle(width, threads, depth):
         off += n
 
 
-@pytest.mark.parametrize("width,threads,depth,nt", [(64, 704, 1, 300), (128, 704, 2, 301), (40, 512, 1, 300),
+@pytest.mark.parametrize("width,threads,depth,nt", [(64, 704, 2, 300), (128, 704, 2, 301), (40, 512, 4, 301),
                                                     (16, 768, 2, 300), (200, 512, 2, 301)])
 def test_two_step_kernel_matches_oracle(width, threads, depth, nt):
     """Temporal blocking (two time steps per launch) on a window deep enough

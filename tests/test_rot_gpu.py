@@ -1,0 +1,111 @@
+"""P/S-decoupled (theta/omega rotation) modelling and RTM, and the elastic
+RTM with decomposed (vector P/S, Poynting up/down) imaging, on the GPU vs
+the FP64 CPU oracle.
+
+Tolerance: relative L2 <= 1e-5 on records and images.  The
+oracle's own FP32 restatements sit near 1e-6 on these cases; the rotation
+RTM's source side runs in FP64 on the GPU (its energy-normalised image
+magnifies FP32 reversal noise wherever the source field is small)."""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from cases import DT, DX, DZ, elastic_model, wavelet
+from ds2d_amd import Solver
+from ds2d_amd.lib import DS2DError, ShotSpec
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _case(nt, boundaries=(0, 1)):
+    """Two uneven windows on a 72 x 40 layered elastic model, frame 20."""
+    allnx, allnz, pml = 72, 40, 20
+    vp, vs, rho = elastic_model(allnx, allnz, 29)
+    wav = wavelet(nt)
+    shots = []
+    for i, ((x0, w), bnd) in enumerate(zip(((0, 52), (24, 48)), boundaries)):
+        shots.append(ShotSpec(x0, w, pml + w // 2, pml + 3, pml + 1, pml + np.arange(0, w, 2, dtype=np.int32),
+                              shot_id=i, boundary=bnd))
+    return allnx, allnz, pml, nt, vp, vs, rho, wav, shots
+
+
+def _solver(allnx, allnz, pml, nt, wav):
+    return Solver(dx=DX, dz=DZ, dt=DT, nt=nt, allnx=allnx, allnz=allnz, pml=pml, nx_max=allnx, wavelet=wav)
+
+
+def _oshot(sh, allnx, allnz, pml, nt):
+    return orc.make_shot(sh, allnx, allnz, pml, nt, DX, DZ, DT)
+
+
+def test_rot_forward_matches_oracle():
+    """A PML window and a random-boundary window in one batch: vz, vx records."""
+    allnx, allnz, pml, nt, vp, vs, rho, wav, shots = _case(400)
+    s = _solver(allnx, allnz, pml, nt, wav)
+    s.set_model(vp, vs, rho)
+    rz, rx, st = s.rot_forward(shots)
+    assert st["kernel_avg_ms"] > 0
+    off = 0
+    for sh in shots:
+        n = sh.ntr * nt
+        oz, ox = orc.rot_forward(_oshot(sh, allnx, allnz, pml, nt), vp, vs, wav)
+        ez, ex = orc.rel_l2(rz[off:off + n], oz), orc.rel_l2(rx[off:off + n], ox)
+        print(f"shot {sh.shot_id} (boundary {sh.boundary}): records rel z {ez:.2e} x {ex:.2e}")
+        assert np.abs(oz).max() > 0
+        assert ez <= TOL and ex <= TOL
+        off += n
+
+
+def test_rot_rtm_matches_oracle():
+    """Two random-boundary windows: PP and PS images of the rotation RTM on
+    the oracle's own records, stacked over both shots."""
+    allnx, allnz, pml, nt, vp, vs, rho, wav, shots = _case(360, boundaries=(1, 1))
+    s = _solver(allnx, allnz, pml, nt, wav)
+    s.set_model(vp, vs, rho)
+    recs = [orc.rot_forward(_oshot(sh, allnx, allnz, pml, nt), vp, vs, wav) for sh in shots]
+    rz = np.concatenate([r[0] for r in recs]).astype(np.float32)
+    rx = np.concatenate([r[1] for r in recs]).astype(np.float32)
+    pp, ps = s.rot_rtm(shots, rz, rx)
+    opp, ops = None, None
+    off = 0
+    for sh in shots:
+        n = sh.ntr * nt
+        opp, ops = orc.rot_rtm(_oshot(sh, allnx, allnz, pml, nt), vp, vs, wav, rz[off:off + n], rx[off:off + n],
+                               opp, ops)
+        off += n
+    epp, eps_ = orc.rel_l2(pp.ravel(), opp), orc.rel_l2(ps.ravel(), ops)
+    print(f"rotation RTM: PP rel {epp:.2e}, PS rel {eps_:.2e}")
+    assert np.abs(opp).max() > 0 and np.abs(ops).max() > 0
+    assert epp <= TOL and eps_ <= TOL
+
+
+def test_rot_rtm_rejects_damped_windows():
+    allnx, allnz, pml, nt, vp, vs, rho, wav, shots = _case(50)
+    s = _solver(allnx, allnz, pml, nt, wav)
+    s.set_model(vp, vs, rho)
+    z = np.zeros(sum(sh.ntr for sh in shots) * nt, np.float32)
+    with pytest.raises(DS2DError, match="random boundary"):
+        s.rot_rtm(shots, z, z)
+
+
+@pytest.mark.parametrize("kind", range(7))
+def test_decomp_rtm_matches_oracle(kind):
+    """A PML window and a random-boundary window: the decomposed image of
+    each kind (0 Poynting PP, 1-3 vector PP/PS/SS, 4-6 scalar PP/PS/SS) on
+    the oracle's own elastic records."""
+    allnx, allnz, pml, nt, vp, vs, rho, wav, shots = _case(320)
+    s = _solver(allnx, allnz, pml, nt, wav)
+    s.set_model(vp, vs, rho)
+    oshots = [_oshot(sh, allnx, allnz, pml, nt) for sh in shots]
+    rz = np.concatenate([orc.elastic_forward(o, vp, vs, rho, wav)[0] for o in oshots]).astype(np.float32)
+    img = s.decomp_rtm(shots, kind, rz)
+    oimg = None
+    off = 0
+    for sh, o in zip(shots, oshots):
+        n = sh.ntr * nt
+        oimg = orc.decomp_rtm(o, vp, vs, rho, wav, kind, rz[off:off + n], oimg)
+        off += n
+    e = orc.rel_l2(img.ravel(), oimg)
+    print(f"decomposed RTM kind {kind}: rel {e:.2e}")
+    assert np.abs(oimg).max() > 0
+    assert e <= TOL
