@@ -560,6 +560,7 @@ def run_c5(a):
     s = Solver(dx=cfg["dx"], dz=cfg["dz"], dt=cfg["dt"], nt=cfg["nt"], allnx=cfg["nx"], allnz=cfg["nz"],
                pml=cfg["pml"], nx_max=max(x.nx for x in shots), wavelet=wav, physics=0)
     s.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
+    s.set_log(1)  # a C5 iteration takes minutes: the solver prints each one
     t0 = time.perf_counter()
     dz, dx, stf = s.eborn_forward(shots, cfg["refl_pp"], cfg["refl_ps"])
     t_fwd = time.perf_counter() - t0
