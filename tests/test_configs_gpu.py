@@ -146,6 +146,42 @@ def test_c4_held_out_model_learned_vs_plain():
         np.testing.assert_allclose(rc, rl[a:], rtol=1e-4)
 
 
+def test_c4_held_out_plain_cg_matches_fp64_oracle():
+    """The plain solver of the C4 comparison on held-out seed 2000 at the
+    full C4 geometry (512x256, random boundary 40, 1500 steps; shots 0, 5,
+    10 and 15 of the 16): two CG iterations match CGLS on the FP64 oracle
+    pair, image and residuals within 1e-5, so the plain GPU image the learned
+    solver is compared with is the FP64 oracle's to that bar."""
+    from ds2d_amd import learned as LR
+    prob = LR.acoustic_problem(2000)
+    shots = [prob["shots"][k] for k in (0, 5, 10, 15)]
+    nx, nz, nt = prob["nx"], prob["nz"], prob["nt"]
+    wav = ricker(nt, prob["f0"], prob["dt"])
+    s = LR.make_solver(prob)
+    s.set_model(prob["vp"], None, prob["rho"])
+    _, d, _ = s.born_forward(shots, prob["refl"])
+    img, res, _ = s.lsrtm_cg(shots, d, 2)
+    osh = [orc.make_shot(x, nx, nz, prob["pml"], nt, prob["dx"], prob["dz"], prob["dt"]) for x in shots]
+    n = [x.ntr * nt for x in shots]
+
+    def L(m):
+        return np.concatenate([orc.acou_forward(o, prob["vp"], prob["rho"], m.reshape(nx, nz), wav)[1] for o in osh])
+
+    def LT(r):
+        g, off = np.zeros(nx * nz), 0
+        for o, k in zip(osh, n):
+            g += orc.acou_adjoint(o, prob["vp"], prob["rho"], wav, r[off:off + k])
+            off += k
+        return g
+
+    x, hist = _cgls(L, LT, d.astype(np.float64), 2, nx * nz)
+    e = orc.rel_l2(img.ravel(), x)
+    print(f"C4 held-out seed 2000, 4 shots, 2 CG iterations: image rel {e:.2e}; residuals gpu {list(res)} "
+          f"oracle {hist}")
+    assert e <= TOL
+    np.testing.assert_allclose(res, hist, rtol=TOL)
+
+
 def test_c5_window_pp_ps_cg_matches_fp64_oracle():
     """PP/PS CG on a 500x250 window of the C5 model (2 full-window shots,
     800 steps, random boundary 40) against CGLS on the FP64 oracle's elastic
