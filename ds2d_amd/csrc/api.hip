@@ -297,6 +297,7 @@ int stats_end(ds2d_ctx* h, ds2d_stats* out, long long cell_updates, long long la
     h->last.stencil_launches = launches;
     h->last.alg_bytes = alg_bytes;
     h->last.aux_alg_bytes = h->last.aux_kernel_avg_ms = 0;
+    h->last.blocked_bytes = 0;
     if (out) *out = h->last;
     return 0;
 }
@@ -771,7 +772,7 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
     stats_begin(h);
     launch_setup(st, sh, setup_args(h, false), m);
     HIP_TRY(hipMemsetAsync(f.vx, 0, (size_t)n * 15 * 4, st));
-    double bytes_v = 0, bytes_s = 0;
+    double bytes_v = 0, bytes_s = 0, blocked = 0;
     // cells of the update region that carry split PML memory (PML windows only)
     double pml_cells = 0;
     for (auto& S : B.slots)
@@ -877,10 +878,12 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
             }
             stats_loop_end(h);
             launches = (long long)(nt / 2);
-            // per padded cell and launch (two steps): the unsplit state and parameters once (13 floats);
-            // each PML cell reads dampx, dampz twice and moves 4 x 5 splits (A read, scratch write + read,
-            // B write) = 24 floats
-            bytes_v = (double)B.total_cells * 13 * 4 + pml_cells * 24 * 4;
+            // algorithmic bytes of the cell-updates one launch performs: two steps at the one-step figure
+            // (52 B per padded cell, +48 B per PML cell).  What this kernel itself must move is less: the
+            // state and parameters once per launch (13 floats per cell) and per PML cell dampx, dampz twice
+            // and 4 x 5 splits (A read, scratch write + read, B write) = 24 floats.
+            bytes_v = 2.0 * ((double)B.total_cells * 13 * 4 + pml_cells * 12 * 4);
+            blocked = (double)B.total_cells * 13 * 4 + pml_cells * 24 * 4;
             bytes_s = 0;
         } else {
         stats_loop_begin(h);
@@ -949,7 +952,8 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
         h->last.aux_kernel_avg_ms = bytes_s > 0 && bytes_v > 0 ? (s_dom ? kv : ks) : 0;
         h->last.aux_alg_bytes = bytes_s > 0 && bytes_v > 0 ? (s_dom ? bytes_v : bytes_s) : 0;
         h->last.kernel_samples = smp.n;
-        if (smp.n == 0 && launches > 0) {  // graph mode: one kernel per step, no gaps
+        h->last.blocked_bytes = blocked;
+        if (smp.n == 0 && launches > 0) {  // graph mode: one kernel per launch, no gaps
             h->last.kernel_avg_ms = h->last.stencil_ms / (double)launches;
             h->last.aux_kernel_avg_ms = h->last.aux_alg_bytes = 0;
             h->last.kernel_samples = launches;

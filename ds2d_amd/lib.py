@@ -41,7 +41,8 @@ class Shot(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("elapsed_ms", C.c_double), ("stencil_ms", C.c_double), ("cell_updates", C.c_longlong),
                 ("stencil_launches", C.c_longlong), ("alg_bytes", C.c_double), ("kernel_avg_ms", C.c_double),
-                ("kernel_samples", C.c_longlong), ("aux_alg_bytes", C.c_double), ("aux_kernel_avg_ms", C.c_double)]
+                ("kernel_samples", C.c_longlong), ("aux_alg_bytes", C.c_double), ("aux_kernel_avg_ms", C.c_double),
+                ("blocked_bytes", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -104,6 +104,9 @@ typedef struct {
     long long kernel_samples;  /* launches that were bracketed by events           */
     double aux_alg_bytes;      /* the other kernel of a two-kernel step, per launch */
     double aux_kernel_avg_ms;  /* (0 when the step is one kernel)                  */
+    double blocked_bytes;      /* several time steps per launch: the bytes one launch
+                                  itself must move (alg_bytes counts each step's
+                                  cell-updates at the one-step figure); else 0     */
 } ds2d_stats;
 
 const char* ds2d_last_error(void);
