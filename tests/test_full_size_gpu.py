@@ -39,7 +39,7 @@ def test_c1_full_size_matches_oracle():
         assert ez <= TOL and ex <= TOL
 
 
-@pytest.mark.parametrize("tblock,depth", [(1, 4), (2, 1)])
+@pytest.mark.parametrize("tblock,depth", [(1, 4), (2, 2)])
 def test_c3_window_at_full_size_matches_oracle(tblock, depth):
     """The whole 128-window C3 batch runs on the GPU with 240-column, 704-lane
     tiles (512 chunks: two rounds of 256 CUs), one time step per launch or two
