@@ -15,7 +15,8 @@
 //                 stage 2: velocity of column j-8, stress of column j-12 (step it+1)
 // Stage 2 reads the step-it+1 values stage 1 left in register queues (x taps)
 // and in an LDS column (z taps), so the intermediate step never leaves the
-// CU.  Stage 1 is therefore computed on 16 halo rows and columns each side
+// CU; its parameters (dt/rho, lambda dt, mu dt) come from a per-lane LDS ring
+// of what stage 1 loaded eight columns earlier, not from memory again.  Stage 1 is therefore computed on 16 halo rows and columns each side
 // of the owned block (velocity on 12, stress on 8): the owned block needs
 // nothing of its neighbours' intermediate step.
 //
@@ -50,7 +51,6 @@ struct Col2 {
     float txx4, txz4;        // step-it stress of column j+4 (x-tap queues)
     float tzz, vx, vz, bx;   // step-it tzz, velocity and dt/rho of column j
     float lam, mu;           // moduli (x dt) of column j-4
-    float bx2, lam2, mu2;    // dt/rho of column j-8, moduli of column j-12 (stage 2)
     int rec, rec2;           // traces recorded at column j (step it) and j-8 (step it+1)
     // damping lanes only
     float dxv, dzv, vxx, vzz;        // stage 1 velocity: damping, old vx_x, vz_z (column j)
@@ -103,6 +103,10 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
     const int x0 = C.x0, x1 = C.x1;
 
     __shared__ float lds[2][8][T];
+    // per-lane ring of stage 1's parameters (dt/rho of column j, moduli of
+    // column j-4): stage 2 reads them eight columns later instead of loading
+    // them again from memory
+    __shared__ float pring[8][3][T];
 
     const long long n = a.n;
     const float* Avx = a.A;
@@ -150,9 +154,6 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
         o.bx = ldc(Pbx, j);
         o.lam = ldc(Plam, j - 4);
         o.mu = ldc(Pmu, j - 4);
-        o.bx2 = ldc(Pbx, j - 8);
-        o.lam2 = ldc(Plam, j - 12);
-        o.mu2 = ldc(Pmu, j - 12);
         o.rec = o.rec2 = -1;
         if (z == S.igz) {
             if (j >= x0 && j < x1) o.rec = rmap[S.rmap_off + j];
@@ -215,6 +216,8 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
     // stage 2 queues: step-it+1 stress s*[k] = column j-12+k, step-it+2 velocity w*[k] = column j-16+k
     float sxx[9], sxz[9], szz[9], wvx[9], wvz[9];
     const int j0 = x0 - 12;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) pring[k][0][L] = pring[k][1][L] = pring[k][2][L] = 0.f;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         qxx[k + 1] = ld(Atxx, j0 - 4 + k);
@@ -305,6 +308,11 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
 
     auto column = [&](const int j, const Col2& o, const int p) {
         const Late2 q = fetch_late(j);
+        const int slot = j & 7;  // (j - 8) & 7 as well
+        const float bx2 = pring[slot][0][L], lam2 = pring[slot][1][L], mu2 = pring[slot][2][L];
+        pring[slot][0][L] = o.bx;
+        pring[slot][1][L] = o.lam;
+        pring[slot][2][L] = o.mu;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             qxx[k] = qxx[k + 1];
@@ -393,7 +401,7 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
                 if ((unsigned)(S.isx - c2 + 3) < 8u && z == S.isz) ax += w2 * tap(S.isx - c2);
                 if (c2 == S.isx) cz += w2 * tz;
                 const bool dv = PML && (zdamp || damp_col2(S, c2));
-                const V2 r = vel(mvx, mvz, ax, ex, bz, cz, o.bx2, q.dxv2, q.dzv2, q.vxx2, q.vzz2, dv);
+                const V2 r = vel(mvx, mvz, ax, ex, bz, cz, bx2, q.dxv2, q.dzv2, q.vxx2, q.vzz2, dv);
                 mvx = r.vx;
                 mvz = r.vz;
                 if (PML && dv && zown && c2 >= x0 && c2 < x1) {
@@ -427,7 +435,7 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
                     const float dvz = dzm(lds[p][7]) * idz;
                     const float dvxz = dzp(lds[p][6]) * idz;
                     const bool ds = PML && (zdamp || damp_col2(S, c3));
-                    const S3 r = str(txx, tzz, txz, dvx, dvzx, dvz, dvxz, o.lam2, o.mu2, q.dxs2, q.dzs2, q.sxx2,
+                    const S3 r = str(txx, tzz, txz, dvx, dvzx, dvz, dvxz, lam2, mu2, q.dxs2, q.dzs2, q.sxx2,
                                      q.szz2, q.sxz2, ds);
                     txx = r.xx;
                     tzz = r.zz;
