@@ -109,3 +109,44 @@ def test_decomp_rtm_matches_oracle(kind):
     print(f"decomposed RTM kind {kind}: rel {e:.2e}")
     assert np.abs(oimg).max() > 0
     assert e <= TOL
+
+
+def test_rotation_and_decomposed_rtm_at_c1_size():
+    """The C1 geometry at full size (251x101, 20-cell frame, 2500 steps,
+    dt 0.8 ms) with the C1 model: rotation forward records of a PML shot,
+    the rotation RTM of a random-boundary shot, and the Poynting (kind 0) and
+    PS vector (kind 2) decomposed images of a PML shot, each within 1e-5 of
+    the FP64 oracle."""
+    from ds2d_amd import models
+    from ds2d_amd.lib import ricker
+    cfg = models.c1_elastic_forward()
+    nx, nz, pml, nt = cfg["nx"], cfg["nz"], cfg["pml"], cfg["nt"]
+    vp, vs, rho = cfg["vp"], cfg["vs"], cfg["rho"]
+    wav = ricker(nt, cfg["f0"], cfg["dt"])
+    rec = pml + np.arange(nx, dtype=np.int32)
+    pml_shot = ShotSpec(0, nx, pml + 125, pml + 2, pml, rec, shot_id=0, boundary=0)
+    rb_shot = ShotSpec(0, nx, pml + 60, pml + 2, pml, rec, shot_id=1, boundary=1)
+    s = Solver(dx=cfg["dx"], dz=cfg["dz"], dt=cfg["dt"], nt=nt, allnx=nx, allnz=nz, pml=pml, nx_max=nx, wavelet=wav)
+    s.set_model(vp, vs, rho)
+
+    def oshot(sh):
+        return orc.make_shot(sh, nx, nz, pml, nt, cfg["dx"], cfg["dz"], cfg["dt"])
+
+    rz, rx, _ = s.rot_forward([pml_shot])
+    oz, ox = orc.rot_forward(oshot(pml_shot), vp, vs, wav)
+    ez, ex = orc.rel_l2(rz, oz), orc.rel_l2(rx, ox)
+    print(f"C1 size rotation forward: records rel z {ez:.2e} x {ex:.2e}")
+    assert ez <= TOL and ex <= TOL
+    bz, bx = orc.rot_forward(oshot(rb_shot), vp, vs, wav)
+    pp, ps = s.rot_rtm([rb_shot], bz.astype(np.float32), bx.astype(np.float32))
+    opp, ops = orc.rot_rtm(oshot(rb_shot), vp, vs, wav, bz.astype(np.float32), bx.astype(np.float32))
+    epp, eps_ = orc.rel_l2(pp.ravel(), opp), orc.rel_l2(ps.ravel(), ops)
+    print(f"C1 size rotation RTM: PP rel {epp:.2e}, PS rel {eps_:.2e}")
+    assert epp <= TOL and eps_ <= TOL
+    ez32 = orc.elastic_forward(oshot(pml_shot), vp, vs, rho, wav)[0].astype(np.float32)
+    for kind in (0, 2):
+        img = s.decomp_rtm([pml_shot], kind, ez32)
+        oimg = orc.decomp_rtm(oshot(pml_shot), vp, vs, rho, wav, kind, ez32)
+        e = orc.rel_l2(img.ravel(), oimg)
+        print(f"C1 size decomposed RTM kind {kind}: rel {e:.2e}")
+        assert np.abs(oimg).max() > 0 and e <= TOL
