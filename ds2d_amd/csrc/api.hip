@@ -2397,8 +2397,9 @@ int ds2d_rot_rtm(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* rec
     const BatchShape& sh = B.shape;
     hipStream_t st = h->st;
     const int nt = h->g.nt;
-    RotFieldD f;  // source side: rebuilt by reversal and imaged normalised by its own energy
-    RotField b;
+    // both sides in FP64: the image divides by the source's own P energy, which magnifies the rounding of
+    // either wavefield wherever the source field is small
+    RotFieldD f, b;
     if ((rc = alloc_rot(h, "src", B.total_cells, f)) || (rc = alloc_rot(h, "rcv", B.total_cells, b))) return rc;
     float *dz, *dx, *w2;
     if ((rc = upload_records(h, B, "obs_z", rec_z, &dz)) || (rc = upload_records(h, B, "obs_x", rec_x, &dx)))
@@ -2418,7 +2419,7 @@ int ds2d_rot_rtm(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* rec
         launch_rot_src(st, sh, f.th, h->d_wav, it);
         launch_rot_v(st, sh, f, m, -1.f);
         launch_rot_v(st, sh, b, m, 1.f);
-        launch_inject_res(st, sh, b.vz, b.vx, dz, dx, it);
+        launch_rot_inject(st, sh, b.vz, b.vx, dz, dx, it);
         launch_rot_p(st, sh, b, m, 1.f);
         launch_rot_image(st, sh, f, b, w2, B.total_win);
     }

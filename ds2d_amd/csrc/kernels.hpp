@@ -204,7 +204,10 @@ void launch_rot_p(hipStream_t st, const BatchShape& b, RotFieldD f, Med m, float
 void launch_rot_src(hipStream_t st, const BatchShape& b, float* th, const float* wav, int it);
 void launch_rot_src(hipStream_t st, const BatchShape& b, double* th, const float* wav, int it);
 // PP and PS of the rotation RTM into the window images w2 (2 x wstride).
-void launch_rot_image(hipStream_t st, const BatchShape& b, RotFieldD src, RotField rcv, float* w2, long long wstride);
+void launch_rot_image(hipStream_t st, const BatchShape& b, RotFieldD src, RotFieldD rcv, float* w2, long long wstride);
+// vz += rec_z, vx += rec_x at the receivers (FP64 receiver field of the rotation RTM).
+void launch_rot_inject(hipStream_t st, const BatchShape& b, double* vz, double* vx, const float* rz, const float* rx,
+                       int it);
 void launch_helmholtz(hipStream_t st, const BatchShape& b, const float* vx, const float* vz, float* VP, float* VS);
 // Floats per window cell and step of a decomposed image kind (0..6).
 int decomp_floats(int kind);

@@ -56,9 +56,10 @@ struct Col2 {
     float dxv, dzv, vxx, vzz;        // stage 1 velocity: damping, old vx_x, vz_z (column j)
     float dxs, dzs, sxx, szz, sxz;   // stage 1 stress: damping, old txx_x, tzz_z, txz_x (column j-4)
 };
-// Damping operands of stage 2 (loaded at the top of the column, in flight
-// while stage 1 computes: prefetching them a column ahead would not fit the
-// register budget of 704-lane workgroups).
+// Damping operands of stage 2 (loaded at the end of the previous column, in
+// flight across the barrier and stage 1: prefetching them two columns ahead
+// with the other operands would not fit the register budget of 704-lane
+// workgroups).
 struct Late2 {
     float dxv2, dzv2, vxx2, vzz2;        // stage 2 velocity (column j-8), splits from the scratch
     float dxs2, dzs2, sxx2, szz2, sxz2;  // stage 2 stress (column j-12), splits from the scratch
@@ -306,8 +307,7 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
         return r;
     };
 
-    auto column = [&](const int j, const Col2& o, const int p) {
-        const Late2 q = fetch_late(j);
+    auto column = [&](const int j, const Col2& o, const int p, Late2& q) {
         const int slot = j & 7;  // (j - 8) & 7 as well
         const float bx2 = pring[slot][0][L], lam2 = pring[slot][1][L], mu2 = pring[slot][2][L];
         pring[slot][0][L] = o.bx;
@@ -453,6 +453,8 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
                 st(a.B + 4 * n, c, txz);
             }
         }
+        // stage 2 damping operands of the next column, in flight across the barrier
+        q = fetch_late(j + 1);
     };
 
     // march columns [x0-12, x1+12) in passes of two columns, the operands of
@@ -460,10 +462,11 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
     static_assert(PF == 2, "two steps per launch march with two columns in flight");
     const int jend = j0 + (x1 + 12 - j0 + 1) / 2 * 2;
     Col2 cA = fetch(j0), cB = fetch(j0 + 1);
+    Late2 q = fetch_late(j0);
     for (int j = j0; j < jend; j += 2) {
-        column(j, cA, 0);
+        column(j, cA, 0, q);
         cA = fetch(j + 2);
-        column(j + 1, cB, 1);
+        column(j + 1, cB, 1, q);
         cB = fetch(j + 3);
     }
 }

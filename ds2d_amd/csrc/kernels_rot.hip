@@ -20,9 +20,10 @@
 //                       images against it at once.
 //
 // Wavefields of the rotation kernels are FP32 or FP64 (T): the rotation RTM
-// rebuilds its source side by reversal and images it normalised by its own
-// energy, which amplifies the reversal's rounding wherever the source field
-// is small, so that side runs in FP64.
+// images the source field against the receiver field normalised by the
+// source's own energy, which amplifies the rounding of either wavefield
+// wherever the source field is small, so the RTM runs both in FP64 (the
+// forward modelling runs in FP32).
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -128,10 +129,10 @@ __global__ void k_rot_src(T* th, const float* __restrict__ wav, const SlotDesc* 
 // velocity against the receiver's P (PP) and S (PS) velocity parts,
 // normalised by the source P energy + eps (float eps = 1e-6), accumulated in
 // the slot's window images w2 = [pp, ps].
-template <typename TS>
+template <typename TS, typename TR>
 __global__ __launch_bounds__(256) void k_rot_image(const TS* __restrict__ spx, const TS* __restrict__ spz,
-                                                   const float* __restrict__ rpx, const float* __restrict__ rpz,
-                                                   const float* __restrict__ rsx, const float* __restrict__ rsz,
+                                                   const TR* __restrict__ rpx, const TR* __restrict__ rpz,
+                                                   const TR* __restrict__ rsx, const TR* __restrict__ rsz,
                                                    float* w2, long long wstride, const SlotDesc* __restrict__ slots) {
     const SlotDesc S = slots[blockIdx.z];
     const int iz = blockIdx.x * BZ + threadIdx.x;
@@ -146,6 +147,20 @@ __global__ __launch_bounds__(256) void k_rot_image(const TS* __restrict__ spx, c
     const TS ps = ax * (TS)rsx[c] + az * (TS)rsz[c];
     w2[o] += (float)(pp / den);
     w2[o + wstride] += (float)(ps / den);
+}
+
+// GPUapplysource_backward_from_record with stack (FD2DGPU.cu:3306-3327):
+// vz += rec_z[tr][it], vx += rec_x[tr][it] at each receiver.
+template <typename T>
+__global__ void k_rot_inject(T* vz, T* vx, const float* __restrict__ rz, const float* __restrict__ rx,
+                             const int* __restrict__ rix, const SlotDesc* __restrict__ slots, int nt, int it) {
+    const SlotDesc S = slots[blockIdx.y];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= S.ntr) return;
+    const size_t c = (size_t)S.off + (size_t)rix[S.rix_off + r] * S.nzp + S.igz;
+    const size_t o = (size_t)S.rec_off + (size_t)r * nt + it;
+    vz[c] += (T)rz[o];
+    vx[c] += (T)rx[o];
 }
 
 // FD2DGPU.cu:1963-1986: VP = div v, VS = d(vx)/dz - d(vz)/dx on the
@@ -295,9 +310,15 @@ void launch_rot_src(hipStream_t st, const BatchShape& b, double* th, const float
     hipLaunchKernelGGL(k_rot_src<double>, dim3(ceil_div(b.nslots, 64)), dim3(64), 0, st, th, wav, b.slots, b.nslots,
                        it);
 }
-void launch_rot_image(hipStream_t st, const BatchShape& b, RotFieldD src, RotField rcv, float* w2, long long wstride) {
-    hipLaunchKernelGGL(k_rot_image<double>, win_grid(b), dim3(BZ, BX), 0, st, src.vpx, src.vpz, rcv.vpx, rcv.vpz,
-                       rcv.vsx, rcv.vsz, w2, wstride, b.slots);
+void launch_rot_image(hipStream_t st, const BatchShape& b, RotFieldD src, RotFieldD rcv, float* w2,
+                      long long wstride) {
+    hipLaunchKernelGGL((k_rot_image<double, double>), win_grid(b), dim3(BZ, BX), 0, st, src.vpx, src.vpz, rcv.vpx,
+                       rcv.vpz, rcv.vsx, rcv.vsz, w2, wstride, b.slots);
+}
+void launch_rot_inject(hipStream_t st, const BatchShape& b, double* vz, double* vx, const float* rz, const float* rx,
+                       int it) {
+    hipLaunchKernelGGL(k_rot_inject<double>, dim3(ceil_div(b.max_ntr, 64), b.nslots), dim3(64), 0, st, vz, vx, rz, rx,
+                       b.rix, b.slots, b.nt, it);
 }
 void launch_helmholtz(hipStream_t st, const BatchShape& b, const float* vx, const float* vz, float* VP, float* VS) {
     hipLaunchKernelGGL(k_helmholtz, rot_grid(b), dim3(BZ, BX), 0, st, vx, vz, VP, VS, b.slots, b.dx, b.dz);

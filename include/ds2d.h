@@ -342,7 +342,9 @@ int ds2d_rot_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec_z
  * same with the receiver's S parts (doimage, FD2DGPU.cu:3218-3233), stacked
  * over shots into allnx*allnz images (stack_image, FD2DGPU.cu:3082-3091).
  * Deviation: the reversal re-injects each source sample before the velocity
- * update, which makes it exact (the reference's order injects it twice). */
+ * update, which makes it exact (the reference's order injects it twice).
+ * Both wavefields run in FP64 (the image divides by the source's own energy,
+ * which magnifies FP32 rounding where the source field is small). */
 int ds2d_rot_rtm(ds2d_t* h, int nshots, const ds2d_shot* shots, const float* rec_z, const float* rec_x,
                  float* img_pp, float* img_ps);
 /* Elastic RTM with decomposed imaging (FD_ELASTIC, FD2DGPU.cu:2241-2517):

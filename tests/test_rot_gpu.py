@@ -4,8 +4,9 @@ the FP64 CPU oracle.
 
 Tolerance: relative L2 <= 1e-5 on records and images.  The
 oracle's own FP32 restatements sit near 1e-6 on these cases; the rotation
-RTM's source side runs in FP64 on the GPU (its energy-normalised image
-magnifies FP32 reversal noise wherever the source field is small)."""
+RTM runs both wavefields in FP64 on the GPU (its image divides by the source's
+own energy, which magnifies FP32 rounding wherever the source field is small:
+with an FP32 receiver side the C1-size image was 1.6e-4 off)."""
 import numpy as np
 import pytest
 
