@@ -160,6 +160,9 @@ inline double rb_uniform(uint64_t seed, int shot, int ix, int iz) {
 // every cell of the pml-wide frame gets v = v_edge*(1 + amp*u), u ~ U[-1,1)
 // from the counter hash; v_edge is the edge-replicated value already in the
 // padded array.  vs keeps vs/vp of the replicated cell; rho is left replicated.
+// The reference writes the frame into its float vp / vs arrays (random_vel,
+// FD2DGPU.cu:3181), so the new velocities are float products and quotients
+// whatever R is: the FP64 oracle then steps with the values the model holds.
 template <typename R>
 void random_boundary(const Grid<R>& g, uint64_t seed, int shot, R amp, R* vp, R* vs) {
     const int pml = g.pml;
@@ -170,11 +173,11 @@ void random_boundary(const Grid<R>& g, uint64_t seed, int shot, R amp, R* vp, R*
             size_t c = g.at(ix, iz);
             double u = rb_uniform(seed, shot, ix, iz);
             float fac = (float)(1.0 + (double)(float)amp * u);
-            R v0 = vp[c];
-            R ratio = v0 > 0 ? vs[c] / v0 : R(0);
-            R vnew = v0 * (R)fac;
-            vp[c] = vnew;
-            vs[c] = ratio * vnew;
+            const float v0 = (float)vp[c];
+            const float ratio = v0 > 0.f ? (float)vs[c] / v0 : 0.f;
+            const float vnew = v0 * fac;
+            vp[c] = (R)vnew;
+            vs[c] = (R)(ratio * vnew);
         }
 }
 

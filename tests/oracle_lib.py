@@ -60,7 +60,14 @@ def dtype_of(prec):
 
 
 def make_shot(spec, allnx, allnz, pml, nt, dx, dz, dt, *, rr=1e-3, rb_seed=1234, rb_amp=0.3, rho_const=0.0):
-    """OrcShot from a ds2d_amd ShotSpec (keeps rec_ix alive on the struct)."""
+    """OrcShot from a ds2d_amd ShotSpec (keeps rec_ix alive on the struct).
+
+    dx, dz and dt are rounded to FP32 first: the library's geometry holds
+    them as floats, as the reference's does (FD2DGPU.cu:25 takes float dx1,
+    dz1, dt1), and the FP64 oracle then steps with exactly those values.  A
+    normalised image over 2500 steps moves by 1.5e-4 between dt = 0.0008 and
+    its FP32 rounding (2e-11 apart), so this matters for the parity bar."""
+    dx, dz, dt = (float(np.float32(v)) for v in (dx, dz, dt))
     rix = np.ascontiguousarray(spec.rec_ix, np.int32)
     s = OrcShot(allnx, allnz, spec.ngx_min, spec.nx, pml, nt, dx, dz, dt, spec.boundary, rr, rb_seed,
                 spec.shot_id, rb_amp, rho_const, spec.isx, spec.isz, spec.igz, rix.size,
