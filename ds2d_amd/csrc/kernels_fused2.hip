@@ -80,16 +80,22 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
     const SlotDesc S = slots[C.slot];
     const int nzp = S.nzp, nxp = S.nxp;
     // Lane -> local row r of the block (LDS index).  With damping strips the
-    // rows of the z strips (and those outside the window) go to the first
-    // lanes, so one wave instead of three carries the damped rows of the
-    // top and bottom strips; the other waves hold contiguous interior rows.
+    // rows of the z strips (and those outside the window) are gathered into
+    // consecutive ranks, so one wave instead of three carries the damped rows
+    // of the top and bottom strips; the other waves hold contiguous interior
+    // rows.  Rank = lane - 192 (mod T): the damping wave is the block's fourth
+    // wave.  Its place matters, with the results bit-identical: on the C3
+    // batch (704 lanes) a launch takes 1.485 ms with it in the fourth wave,
+    // 1.519 ms in the first, 1.52-1.55 ms in waves 2-8 and 1.66 ms in the
+    // last three.
     const int zb = C.z0 - FUSED2_HALO;
     int r = threadIdx.x;
     if (PML && S.boundary == 0) {
         const int a0 = min(max(S.pml - zb, 0), T);                // rows with z < pml
         const int b0 = min(max(nzp - S.pml + 1 - zb, a0), T);      // first row with z > nzp - pml
         const int nb = T - b0;
-        const int l = threadIdx.x;
+        int l = (int)threadIdx.x - 192;                             // T >= 512
+        if (l < 0) l += T;
         r = l < a0 ? l : (l < a0 + nb ? b0 + (l - a0) : a0 + (l - a0 - nb));
     }
     const int L = r;
