@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--graph", action="store_true",
                     help="capture each time loop in one hipGraph (always on for c3, whose configuration names it)")
     ap.add_argument("--no-graph", action="store_true", help="c3: launch the time loop's kernels directly")
+    ap.add_argument("--no-chunk-plan", action="store_true",
+                    help="two steps per launch: fixed-width chunks instead of the balanced chunk plan")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=5.0, help="CPU baseline: seconds per timed run (3 runs)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "k_fused_step_c3_traffic.json"),
@@ -195,6 +197,7 @@ def run_forward(a, cfg):
                pml=cfg["pml"], nx_max=nx_max, wavelet=wav, pml_rr=1e-3)
     s.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
     s.set_graph(a.graph)
+    s.set_chunk_plan(not a.no_chunk_plan)
     chunk = threads = depth = None
     if a.mode == 2 and a.tile:
         chunk, threads, depth = (int(v) for v in a.tile.split(","))
@@ -212,6 +215,7 @@ def run_forward(a, cfg):
         _, _, st = s.batch_forward(cfg["shots"], mode=a.mode, fetch=False)
         stats.append(st)
     elapsed = time.perf_counter() - t0
+    plan = tblock == 2 and s.chunk_plan_used()  # the timed launches' chunk list (two steps per launch)
     rz, rx, _ = s.batch_forward(cfg["shots"][:2], mode=a.mode, fetch=True)  # untimed validation
     assert np.isfinite(rz).all() and np.isfinite(rx).all() and np.abs(rz).max() > 0
     cu = stats[0]["cell_updates"]
@@ -227,7 +231,8 @@ def run_forward(a, cfg):
                        (" (autotuned)" if a.mode == 2 and not a.tile else ""), "parallelism": "shots batched on one GPU",
             "autotuned_tile": None if chunk is None else {"x_width": chunk, "threads": threads,
                                                           "prefetch_columns": depth,
-                                                          "time_steps_per_launch": tblock},
+                                                          "time_steps_per_launch": tblock,
+                                                          "chunk_plan": "balanced" if plan else "fixed width"},
             "hipgraph": a.graph},
         "roofline": roofline(stats),
     }
@@ -773,11 +778,15 @@ def main():
         if (tr.get("workload_prefix") and str(cfgd.get("workload", "")).startswith(tr["workload_prefix"])
                 and tr.get("shots") == cfgd.get("shots")
                 and bool(tr.get("random_boundary", False)) == bool(getattr(a, "random_boundary", False))):
+            balanced = tile.get("chunk_plan") == "balanced"
             for e in tr.get("entries", [tr]):
                 t = list(e.get("tile") or [])
                 if len(t) == 3:  # entries without a time block are one step per launch
                     t.append(1)
-                if t == key:
+                if (e.get("chunk_plan") == "balanced") != balanced:
+                    continue
+                # the balanced plan does not depend on the width
+                if (t[1:] == key[1:]) if balanced else (t == key):
                     out["roofline"]["traffic"] = e.get("traffic_bytes_per_launch")
     print(json.dumps(out), flush=True)
 

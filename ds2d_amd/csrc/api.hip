@@ -20,6 +20,7 @@
 #include "../../include/ds2d.h"
 #include "common.hpp"
 #include "kernels.hpp"
+#include "chunk_plan.hpp"
 #include "unet.hpp"
 
 using namespace ds2d;
@@ -79,6 +80,11 @@ struct ds2d_ctx {
     int chunk_t = 0;    // fused-kernel workgroup size (256/512/704/768; 0: default)
     int chunk_d = 0;    // fused-kernel prefetch depth in columns (2 or 4; 0: default 2)
     int tblock = 1;     // fused elastic forward: time steps per launch (1 or 2)
+    int chunk_plan = 1; // two steps per launch: balanced chunk plan when it simulates faster (0: fixed width)
+    int last_plan = 0;  // the last two-step launch list: 1 balanced, 0 fixed width
+    std::vector<int> plan_key;         // batch shape of the cached two-step chunk list
+    std::vector<ChunkDesc> plan_list;  // its chunk list
+    int plan_list_bal = 0;             // whether that list is the balanced plan
     double stop_resid = 0;  // CG solvers stop once the relative data residual is at most this (0: never)
     int log_every = 0;      // CG solvers print the residual to stderr every this many iterations (0: never)
     int use_graph = 0;  // capture time loops into one hipGraph per loop
@@ -843,10 +849,35 @@ static int batch_forward_n(ds2d_t* h, int nshots, const ds2d_shot* shots, float*
             const int D2 = 2;  // two columns in flight
             std::vector<ChunkDesc> ch2;
             const int rows2 = T2 - 2 * FUSED2_HALO;
-            for (int s = 0; s < nshots; ++s)
-                for (int x0 = 0; x0 < B.slots[s].nxp; x0 += W)
-                    for (int z0 = 0; z0 < B.slots[s].nzp; z0 += rows2)
-                        ch2.push_back(ChunkDesc{s, x0, std::min(x0 + W, B.slots[s].nxp), z0});
+            {
+                // fixed-width chunks, or the balanced plan when its simulated
+                // launch time is shorter (chunk_plan.hpp)
+                std::vector<int> nxv(nshots), nzv(nshots);
+                for (int s = 0; s < nshots; ++s) {
+                    nxv[s] = B.slots[s].nxp;
+                    nzv[s] = B.slots[s].nzp;
+                }
+                int ncu = 0;
+                if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess ||
+                    ncu < kPlanXcds)
+                    ncu = 256;
+                const int cpx = ncu / kPlanXcds;
+                std::vector<int> key{W, rows2, cpx, h->chunk_plan};
+                key.insert(key.end(), nxv.begin(), nxv.end());
+                key.insert(key.end(), nzv.begin(), nzv.end());
+                if (key != h->plan_key) {  // plan once per batch shape (the search takes milliseconds)
+                    std::vector<PlanChunk> fixed, bal;
+                    const long long span_f = plan_fixed(nxv, nzv, rows2, W, cpx, fixed);
+                    const long long span_b = h->chunk_plan ? plan_balanced(nxv, nzv, rows2, cpx, bal) : -1;
+                    h->plan_list_bal = span_b >= 0 && span_b < span_f;
+                    h->plan_list.clear();
+                    for (const PlanChunk& c : h->plan_list_bal ? bal : fixed)
+                        h->plan_list.push_back(ChunkDesc{c.slot, c.x0, c.x1, c.z0});
+                    h->plan_key = key;
+                }
+                h->last_plan = h->plan_list_bal;
+                ch2 = h->plan_list;
+            }
             ChunkDesc* d_ch2;
             if ((rc = h->fbuf("chunks2", ch2.size(), &d_ch2))) return rc;
             HIP_TRY(hipMemcpyAsync(d_ch2, ch2.data(), ch2.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice, st));
@@ -977,6 +1008,21 @@ int ds2d_set_time_block(ds2d_t* h, int steps) {
     if ((rc = check_h(h))) return rc;
     if (steps != 1 && steps != 2) return fail(DS2D_EINVAL, "time steps per launch must be 1 or 2, got %d", steps);
     h->tblock = steps;
+    return 0;
+}
+
+int ds2d_set_chunk_plan(ds2d_t* h, int on) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    h->chunk_plan = on ? 1 : 0;
+    return 0;
+}
+
+int ds2d_get_chunk_plan(ds2d_t* h, int* used) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (!used) return fail(DS2D_EINVAL, "null argument");
+    *used = h->last_plan;
     return 0;
 }
 

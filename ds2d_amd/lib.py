@@ -78,6 +78,8 @@ SIGNATURES = {
     "ds2d_take_gradients": (C.c_int, [_H, _F, _F, _F, _F, _F, _F, _F]),
     "ds2d_shot_status": (C.c_int, [_H, C.c_int, C.POINTER(C.c_int)]),
     "ds2d_set_time_block": (C.c_int, [_H, C.c_int]),
+    "ds2d_set_chunk_plan": (C.c_int, [_H, C.c_int]),
+    "ds2d_get_chunk_plan": (C.c_int, [_H, C.POINTER(C.c_int)]),
     "ds2d_set_stop": (C.c_int, [_H, C.c_double]),
     "ds2d_set_log": (C.c_int, [_H, C.c_int]),
     "ds2d_get_time_block": (C.c_int, [_H, C.POINTER(C.c_int)]),
@@ -254,6 +256,16 @@ class Solver:
     def set_time_block(self, steps):
         """Fused elastic forward: 1 or 2 time steps per launch (temporal blocking)."""
         _check(self.lib.ds2d_set_time_block(self.h, int(steps)))
+
+    def set_chunk_plan(self, on):
+        """Two steps per launch: balanced chunk plan across XCDs/CUs (True, default) or fixed-width chunks."""
+        _check(self.lib.ds2d_set_chunk_plan(self.h, 1 if on else 0))
+
+    def chunk_plan_used(self):
+        """Whether the last two-step launch list was the balanced plan."""
+        v = C.c_int(0)
+        _check(self.lib.ds2d_get_chunk_plan(self.h, C.byref(v)))
+        return bool(v.value)
 
     def time_block(self):
         v = C.c_int(0)

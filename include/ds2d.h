@@ -176,6 +176,17 @@ int ds2d_set_chunk(ds2d_t* h, int width, int threads, int depth);
  * No reference counterpart (new capability). */
 int ds2d_set_time_block(ds2d_t* h, int steps);
 int ds2d_get_time_block(ds2d_t* h, int* steps);
+/* Chunk list of two steps per launch: on (1, the default) the library plans
+ * the chunks of the batch for balance across the eight XCDs and their CUs
+ * (each XCD gets the same number of chunks, a multiple of its CU count,
+ * windows split into equal-width chunks, widest first) whenever a simulated
+ * list schedule says that finishes sooner than fixed-width chunks of the set
+ * width; off (0) always uses the fixed width.  Results are identical either
+ * way.  ds2d_get_chunk_plan reports whether the last two-step launch list
+ * was the balanced plan (1) or fixed width (0).  No reference counterpart
+ * (new capability). */
+int ds2d_set_chunk_plan(ds2d_t* h, int on);
+int ds2d_get_chunk_plan(ds2d_t* h, int* used);
 /* Capture each time loop into one hipGraph (1) or launch step by step (0). */
 int ds2d_set_graph(ds2d_t* h, int on);
 int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nsteps, int* best_width,
