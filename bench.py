@@ -71,6 +71,9 @@ def parse():
     ap.add_argument("--graph", action="store_true",
                     help="capture each time loop in one hipGraph (always on for c3, whose configuration names it)")
     ap.add_argument("--no-graph", action="store_true", help="c3: launch the time loop's kernels directly")
+    ap.add_argument("--no-fused-born", action="store_true",
+                    help="c5: the elastic Born forward pass with two kernels per step instead of one fused launch")
+    ap.add_argument("--born-tile", default=None, help="c5: x-chunk width,lanes of the fused Born step (e.g. 64,256)")
     ap.add_argument("--no-chunk-plan", action="store_true",
                     help="two steps per launch: fixed-width chunks instead of the balanced chunk plan")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -561,6 +564,8 @@ def run_c5(a):
     s = Solver(dx=cfg["dx"], dz=cfg["dz"], dt=cfg["dt"], nt=cfg["nt"], allnx=cfg["nx"], allnz=cfg["nz"],
                pml=cfg["pml"], nx_max=max(x.nx for x in shots), wavelet=wav, physics=0)
     s.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
+    bw, bt = (int(v) for v in a.born_tile.split(",")) if a.born_tile else (0, 0)
+    s.set_fused_born(not a.no_fused_born, bw, bt)
     s.set_log(1)  # a C5 iteration takes minutes: the solver prints each one
     t0 = time.perf_counter()
     dz, dx, stf = s.eborn_forward(shots, cfg["refl_pp"], cfg["refl_ps"])

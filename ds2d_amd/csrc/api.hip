@@ -81,6 +81,8 @@ struct ds2d_ctx {
     int chunk_d = 0;    // fused-kernel prefetch depth in columns (2 or 4; 0: default 2)
     int tblock = 1;     // fused elastic forward: time steps per launch (1 or 2)
     int chunk_plan = 1; // two steps per launch: balanced chunk plan when it simulates faster (0: fixed width)
+    int eb_fused = 1;   // elastic Born forward pass: one fused launch per step (0: k_eb_v + k_eb_s)
+    int eb_w = 0, eb_t = 0;  // its x-chunk width and workgroup lanes (0: defaults 64, 256)
     int last_plan = 0;  // the last two-step launch list: 1 balanced, 0 fixed width
     std::vector<int> plan_key;         // batch shape of the cached two-step chunk list
     std::vector<ChunkDesc> plan_list;  // its chunk list
@@ -1018,6 +1020,17 @@ int ds2d_set_chunk_plan(ds2d_t* h, int on) {
     return 0;
 }
 
+int ds2d_set_fused_born(ds2d_t* h, int on, int width, int threads) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    if (width < 0 || (width > 0 && width < 8)) return fail(DS2D_EINVAL, "chunk width must be 0 or >= 8");
+    if (threads != 0 && !eb_fused_threads_ok(threads)) return fail(DS2D_EINVAL, "threads must be 0, 256 or 512");
+    h->eb_fused = on ? 1 : 0;
+    h->eb_w = width;
+    h->eb_t = threads;
+    return 0;
+}
+
 int ds2d_get_chunk_plan(ds2d_t* h, int* used) {
     int rc;
     if ((rc = check_h(h))) return rc;
@@ -1722,6 +1735,11 @@ struct EbWS {
     bool store = false;
     int* rmap = nullptr;
     long long n = 0, R = 0, W = 0, H = 0;
+    // fused forward step: second state (10 n) and chunk lists per launch group
+    float* alt = nullptr;
+    ChunkDesc* chunks = nullptr;
+    std::vector<std::pair<int, int>> gchunks;
+    int threads = 256;
 };
 
 EbField eb_field(float* base, long long n) {
@@ -1782,6 +1800,28 @@ int eb_prepare(ds2d_ctx* h, const Batch& B, const float* pp_dev, const float* ps
     if ((rc = h->fbuf("ebrmap", std::max<long long>(B.total_rmap, 1), &w.rmap))) return rc;
     HIP_TRY(hipMemsetAsync(w.rmap, 0xff, B.total_rmap * 4, h->st));
     launch_fast_rmap(h->st, B.shape, w.rmap);
+    // fused forward step (32-bit byte offsets within a field: n < 2^30 cells)
+    w.alt = nullptr;
+    if (h->eb_fused && n < (1LL << 30)) {
+        const int W = h->eb_w > 0 ? h->eb_w : 64;
+        w.threads = eb_fused_threads_ok(h->eb_t) ? h->eb_t : 256;
+        const int rows = w.threads - 2 * FUSED_HALO;
+        std::vector<ChunkDesc> ch;
+        w.gchunks.clear();
+        const int ns = (int)B.slots.size();
+        const int gs = h->per_launch > 0 ? std::min(h->per_launch, ns) : ns;  // the groups of launch_groups
+        for (int s0 = 0; s0 < ns; s0 += gs) {
+            const int first = (int)ch.size();
+            for (int s = s0; s < std::min(ns, s0 + gs); ++s)
+                for (int x0 = 0; x0 < B.slots[s].nxp; x0 += W)
+                    for (int z0 = 0; z0 < B.slots[s].nzp; z0 += rows)
+                        ch.push_back(ChunkDesc{s - s0, x0, std::min(x0 + W, B.slots[s].nxp), z0});
+            w.gchunks.push_back({first, (int)ch.size() - first});
+        }
+        if ((rc = h->fbuf("ebchunks", ch.size(), &w.chunks))) return rc;
+        HIP_TRY(hipMemcpyAsync(w.chunks, ch.data(), ch.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice, h->st));
+        if ((rc = h->fbuf("eborn2", (size_t)n * 10, &w.alt))) return rc;
+    }
     return 0;
 }
 
@@ -1792,6 +1832,25 @@ void eb_fwd(ds2d_ctx* h, const Batch& B, EbWS& w, bool do_bg) {
     hipStream_t st = h->st;
     (void)hipMemsetAsync(w.bg.vx, 0, (size_t)w.n * 10 * 4, st);  // bg + bn
     const int hmode = do_bg ? (w.store ? 1 : 0) : 2;
+    if (do_bg && w.alt) {
+        // one fused launch per step, state ping-ponged between the workspace
+        // (bg then bn, 10 n floats) and w.alt; the final state goes back to
+        // the workspace, where the adjoint pass expects it
+        const int nt = h->g.nt;
+        int g = 0;
+        for (const BatchShape& sh : launch_groups(h, B)) {
+            const std::pair<int, int> gc = w.gchunks[g++];
+            float *a = w.bg.vx, *b = w.alt;
+            for (int it = 0; it < nt; ++it) {
+                EbFusedArgs q{a, b, w.n, w.m, w.hist, w.H, w.rec, w.rec + w.R};
+                launch_eb_fused(st, sh, q, w.chunks + gc.first, gc.second, w.rmap, h->wav[it], 1.f, it, hmode,
+                                w.threads);
+                std::swap(a, b);
+            }
+        }
+        if (nt & 1) (void)hipMemcpyAsync(w.bg.vx, w.alt, (size_t)w.n * 10 * 4, hipMemcpyDeviceToDevice, st);
+        return;
+    }
     for (const BatchShape& sh : launch_groups(h, B))
         for (int it = 0; it < h->g.nt; ++it) {
             launch_eb_v(st, sh, w.bg, w.bn, w.m.rho, h->wav[it], 1.f, do_bg, 1);

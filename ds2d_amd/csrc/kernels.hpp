@@ -165,6 +165,19 @@ struct EbMed {
 struct EbAux {
     float *wa, *wb, *wc, *ux, *uz;
 };
+// One fused forward step of the pair (kernels_eborn_fused.hip): A and B hold
+// 10 fields of n floats each (background vx vz txx tzz txz, then the
+// scattered five); A is read, B written.  hist_mode 1 stores the background
+// strains as k_eb_s does (0: none).
+struct EbFusedArgs {
+    const float* A;
+    float* B;
+    long long n;
+    EbMed m;
+    float* hist;
+    long long hstride;
+    float *rec_z, *rec_x;
+};
 // Forward step: k_eb_v (background with source sample w and/or scattered
 // velocity), then k_eb_s (stresses, Born source, recorder).  sign = -1 undoes a
 // background step.  hist_mode: 1 store the background strains (3 arrays of
@@ -174,6 +187,12 @@ void launch_eb_v(hipStream_t st, const BatchShape& b, EbField bg, EbField born, 
 void launch_eb_s(hipStream_t st, const BatchShape& b, EbField bg, EbField born, EbMed m, float* hist,
                  long long hstride, int it, float sign, int do_bg, int do_born, int hist_mode, const int* rmap,
                  float* rec_z, float* rec_x);
+// Chunks as the fused elastic step's (rows of a block: threads - 2*FUSED_HALO);
+// threads 256 or 512.  The result equals launch_eb_v + launch_eb_s with
+// do_bg = do_born = 1 up to FP32 rounding (FMA contraction).
+bool eb_fused_threads_ok(int threads);
+void launch_eb_fused(hipStream_t st, const BatchShape& b, const EbFusedArgs& a, const struct ChunkDesc* chunks,
+                     int nchunks, const int* rmap, float w, float sign, int it, int hist_mode, int threads);
 // Adjoint step: k_eb_adj_v (receiver injection of step it_inj, -1 none, and the
 // transpose of the stress update), then k_eb_adj_s (transpose of the velocity
 // update and the gradient of step it; strains from hist, or from bg when

@@ -187,6 +187,15 @@ int ds2d_get_time_block(ds2d_t* h, int* steps);
  * (new capability). */
 int ds2d_set_chunk_plan(ds2d_t* h, int on);
 int ds2d_get_chunk_plan(ds2d_t* h, int* used);
+/* Forward pass of the plain-L2 elastic Born pair (ds2d_eborn_forward and the
+ * Born passes of ds2d_elsrtm_cg / ds2d_elsrtm_learned): on (1, the default)
+ * one fused launch per time step advances the background and the scattered
+ * field (velocity, stress, Born source, recorder) with the arithmetic of the
+ * two per-step kernels (equal to FP32 rounding); off (0) launches those two
+ * kernels.
+ * width: x-chunk width of the fused step (0 = 64); threads: 256 or 512 lanes
+ * (0 = 256).  No reference counterpart (new capability). */
+int ds2d_set_fused_born(ds2d_t* h, int on, int width, int threads);
 /* Capture each time loop into one hipGraph (1) or launch step by step (0). */
 int ds2d_set_graph(ds2d_t* h, int on);
 int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nsteps, int* best_width,

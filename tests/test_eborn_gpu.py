@@ -206,3 +206,38 @@ def test_eborn_line_search_recovers_known_step():
     num, den = s.eborn_line_search(shots, pp, ps, gpp, gps, mz - c * gz, mx - c * gx)
     assert den > 0
     assert abs(num / den - c) <= 1e-4 * c
+
+
+@pytest.mark.parametrize("bg_mode", [1, 2])
+@pytest.mark.parametrize("threads,width", [(256, 64), (512, 24)])
+def test_fused_born_step_matches_the_two_kernels(bg_mode, threads, width):
+    """Three uneven windows (one of 12 columns) on an 80 x 300 model, so a
+    256-lane block covers the 340 padded rows in two row blocks; an odd step
+    count, so the fused pass ends in its second state and copies it back.
+    The records of the Born forward map and the PP/PS CG images (whose
+    adjoint passes rebuild the background from the forward pass's final
+    state when bg_mode = 2) agree with those of k_eb_v + k_eb_s to FP32
+    rounding (relative L2 <= 1e-6: the expressions are the same, but the
+    compiler may contract different products into FMAs in the two codes)."""
+    allnx, allnz, pml, nt = 80, 300, 20, 151
+    vp, vs, rho = elastic_model(allnx, allnz, 41)
+    wav = wavelet(nt)
+    shots = [ShotSpec(0, 60, pml + 30, pml + 2, pml + 1, pml + np.arange(0, 60, 3, dtype=np.int32), 0, 1),
+             ShotSpec(15, 50, pml + 7, pml + 223, pml + 221, pml + np.arange(50, dtype=np.int32), 1, 1),
+             ShotSpec(64, 12, pml + 5, pml + 3, pml + 2, pml + np.arange(12, dtype=np.int32), 2, 1)]
+    rng = np.random.default_rng(8)
+    pp = (0.05 * rng.standard_normal((allnx, allnz))).astype(np.float32)
+    ps = (0.03 * rng.standard_normal((allnx, allnz))).astype(np.float32)
+    out = {}
+    for fused in (False, True):
+        s = _solver(allnx, allnz, pml, nt, wav, bg_mode=bg_mode)
+        s.set_model(vp, vs, rho)
+        s.set_fused_born(fused, width, threads)
+        rz, rx, _ = s.eborn_forward(shots, pp, ps)
+        ipp, ips, hist, _ = s.elsrtm_cg(shots, rz, rx, 2)
+        out[fused] = (rz, rx, ipp, ips, hist)
+    for name, a, b in zip(("rec z", "rec x", "PP", "PS", "residuals"), out[False], out[True]):
+        e = orc.rel_l2(np.ravel(b), np.ravel(a))
+        print(f"{name}: rel {e:.2e}, identical {np.array_equal(a, b)}")
+        assert e <= 1e-6, name
+    assert np.abs(out[True][0]).max() > 0
