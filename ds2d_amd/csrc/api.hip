@@ -1876,12 +1876,28 @@ void eb_bwd(ds2d_ctx* h, const Batch& B, EbWS& w) {
     hipStream_t st = h->st;
     (void)hipMemsetAsync(w.lm.vx, 0, (size_t)w.n * 12 * 4, st);  // lm + gpp + gps + adjoint products
     const int nt = h->g.nt, undo = w.store ? 0 : 1;
+    int g = 0;
     for (const BatchShape& sh : launch_groups(h, B)) {
         // Without the history the background is rebuilt alongside: the stress of
         // step nt-1 comes off here, then each iteration takes the velocity of
         // step it+1 (with k_eb_adj_v) and the stress of step it (with k_eb_adj_s).
         if (undo && nt >= 2)
             launch_eb_s(st, sh, w.bg, w.bn, w.m, nullptr, w.H, nt - 1, -1.f, 1, 0, 0, w.rmap, nullptr, nullptr);
+        if (w.alt) {
+            // one fused launch per step: adjoint state (and the background when
+            // it is reversed) ping-ponged between the workspace and w.alt
+            const std::pair<int, int> gc = w.gchunks[g++];
+            float *alm = w.lm.vx, *abg = w.bg.vx, *blm = w.alt, *bbg = w.alt + 5 * w.n;
+            for (int it = nt - 2; it >= 0; --it) {
+                EbAdjArgs q{alm, abg, blm, bbg, w.n, w.m, w.store ? w.hist : nullptr, w.H, w.gpp, w.gps,
+                            w.res, w.res + w.R};
+                launch_eb_adj_fused(st, sh, q, w.chunks + gc.first, gc.second, w.rmap, h->wav[it + 1], it, it + 1,
+                                    undo != 0, w.threads);
+                std::swap(alm, blm);
+                if (undo) std::swap(abg, bbg);
+            }
+            continue;
+        }
         for (int it = nt - 2; it >= 0; --it) {
             launch_eb_adj_v(st, sh, w.lm, w.aux, w.bg, w.m, w.rmap, w.res, w.res + w.R, it + 1, undo,
                             h->wav[it + 1]);

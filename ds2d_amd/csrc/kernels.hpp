@@ -191,6 +191,23 @@ void launch_eb_s(hipStream_t st, const BatchShape& b, EbField bg, EbField born, 
 // threads 256 or 512.  The result equals launch_eb_v + launch_eb_s with
 // do_bg = do_born = 1 up to FP32 rounding (FMA contraction).
 bool eb_fused_threads_ok(int threads);
+// One fused adjoint step (k_eb_adj_v with the residual of step it_inj, then
+// k_eb_adj_s of step it): adjoint state and background read from A, written
+// to B (5 fields each: vx vz txx tzz txz); gradients accumulated in place.
+// undo: no strain history, the background steps back alongside; otherwise
+// the strains come from hist (3 arrays of hstride floats).
+struct EbAdjArgs {
+    const float *Alm, *Abg;
+    float *Blm, *Bbg;
+    long long n;
+    EbMed m;
+    const float* hist;
+    long long hstride;
+    float *gpp, *gps;
+    const float *res_z, *res_x;
+};
+void launch_eb_adj_fused(hipStream_t st, const BatchShape& b, const EbAdjArgs& a, const struct ChunkDesc* chunks,
+                         int nchunks, const int* rmap, float w, int it, int it_inj, bool undo, int threads);
 void launch_eb_fused(hipStream_t st, const BatchShape& b, const EbFusedArgs& a, const struct ChunkDesc* chunks,
                      int nchunks, const int* rmap, float w, float sign, int it, int hist_mode, int threads);
 // Adjoint step: k_eb_adj_v (receiver injection of step it_inj, -1 none, and the

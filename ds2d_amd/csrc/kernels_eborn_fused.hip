@@ -279,6 +279,265 @@ __global__ __launch_bounds__(T) void k_eb_fused(EbFusedArgs a, const ChunkDesc* 
     }
 }
 
+// One adjoint step of the pair (k_eb_adj_v + k_eb_adj_s of step it, with the
+// receiver residual of step it+1) in a single x-marching launch.  The products
+// C s* (wa, wb, wc) and (1/rho) v* (ux, uz) that the two per-cell kernels hand
+// each other through memory are formed in registers here: wa, wb, wc of the
+// adjoint stress of column j+4 as it is loaded (zero outside the update
+// region, as the per-cell kernels leave them), ux, uz of column j as its
+// velocity is updated.  UNDO (no strain history): the background is taken one
+// step back alongside, velocity at column j from its stress (source sample w),
+// stress at column j-4 from the new velocity, whose differences are also the
+// strains of the gradient; otherwise the strains come from the history.  The
+// adjoint state (lm) and the background are ping-ponged (A read, B written);
+// the gradients accumulate in place (each cell has one owner).
+template <int T, bool UNDO>
+__global__ __launch_bounds__(T) void k_eb_adj_fused(EbAdjArgs a, const ChunkDesc* __restrict__ chunks,
+                                                    int nchunks, int per_xcd, const int* __restrict__ rmap,
+                                                    const SlotDesc* __restrict__ slots, float cax, float caz,
+                                                    float idx, float idz, float w, int it, int it_inj, int nt) {
+    const int b = blockIdx.x;
+    const int ch = (b % 8) * per_xcd + b / 8;
+    if (ch >= nchunks) return;
+    const ChunkDesc C = chunks[ch];
+    const SlotDesc S = slots[C.slot];
+    const int L = threadIdx.x;
+    const int z = C.z0 - FUSED_HALO + L;
+    const int nzp = S.nzp, nxp = S.nxp;
+    const bool zin = z >= 0 && z < nzp;
+    const bool zown = zin && L >= FUSED_HALO && L < T - FUSED_HALO;
+    const bool vlane = L >= 4 && L < T - 4;
+    const bool zupd = z >= NOP && z < nzp - NOP;
+    const bool zphys = z >= S.pml && z < S.pml + S.nz;
+    const int x0 = C.x0, x1 = C.x1;
+
+    // z-tap columns: background txz, tzz (column j), wc, wb (j), ux, uz (j-4),
+    // new background vx, vz (j-4)
+    __shared__ float lds[2][8][T];
+
+    const long long n = a.n;
+    const uint32_t colb = (uint32_t)nzp * 4u;
+    const uint32_t zo4 = (uint32_t)(S.off + z) * 4u;
+    const uint32_t zc4 = (uint32_t)(S.off + min(max(z, 0), nzp - 1)) * 4u;
+    auto at = [](const float* p, uint32_t boff) -> float {
+        return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + boff);
+    };
+    auto st = [](float* p, uint32_t boff, float v) {
+        *reinterpret_cast<float*>(reinterpret_cast<char*>(p) + boff) = v;
+    };
+    auto ldc = [&](const float* p, int col) -> float {
+        return at(p, zc4 + (uint32_t)min(max(col, 0), nxp - 1) * colb);
+    };
+    auto inu = [&](int col) { return zupd && col >= NOP && col < nxp - NOP; };
+    const float* Alm = a.Alm;  // vx vz txx tzz txz
+    const float* Abg = a.Abg;
+
+    struct AdjCol {
+        float wa4, wb4, wc4;            // products C s* of column j+4
+        float lvx, lvz, rho;            // adjoint velocity and rho of column j
+        float ix, iz;                   // residual injected at column j (receiver-row lane)
+        float gxx4, gxz4, gzz, gvx, gvz;  // background: txx, txz of column j+4; tzz, vx, vz of column j
+    };
+    auto fetch = [&](int j) {
+        AdjCol o;
+        const int c4 = j + 4;
+        const float txx = ldc(Alm + 2 * n, c4), tzz = ldc(Alm + 3 * n, c4), txz = ldc(Alm + 4 * n, c4);
+        const float l = ldc(a.m.lam, c4), mu = ldc(a.m.mu, c4), l2m = l + 2 * mu;
+        const bool u4 = inu(c4);
+        o.wa4 = u4 ? l2m * txx + l * tzz : 0.f;
+        o.wb4 = u4 ? l * txx + l2m * tzz : 0.f;
+        o.wc4 = u4 ? mu * txz : 0.f;
+        o.lvx = ldc(Alm, j);
+        o.lvz = ldc(Alm + n, j);
+        o.rho = ldc(a.m.rho, j);
+        o.ix = o.iz = 0.f;
+        // every computed column, halo included: the stress taps of owned columns read them
+        if (it_inj >= 0 && z == S.igz && inu(j)) {
+            const int r = rmap[S.rmap_off + j];
+            if (r >= 0) {
+                const size_t q = (size_t)S.rec_off + (size_t)r * nt + it_inj;
+                o.ix = a.res_x[q];
+                o.iz = a.res_z[q];
+            }
+        }
+        if (UNDO) {
+            o.gxx4 = ldc(Abg + 2 * n, c4);
+            o.gxz4 = ldc(Abg + 4 * n, c4);
+            o.gzz = ldc(Abg + 3 * n, j);
+            o.gvx = ldc(Abg, j);
+            o.gvz = ldc(Abg + n, j);
+        } else {
+            o.gxx4 = o.gxz4 = o.gzz = o.gvx = o.gvz = 0.f;
+        }
+        return o;
+    };
+
+    // queues: wa, wc (x taps, column j-4+k), wb (column j+k: z taps only),
+    // ux, uz (columns j-8+k, j-7+k); background as in the forward step
+    float qwa[9], qwc[9], qwb[5], qux[9], quz[8];
+    float gxx[9], gxz[9], gzz[5], gvx[9], gvz[8];
+    const int j0 = x0 - 4;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int c = j0 - 4 + k;
+        const bool u = zin && c >= 0 && c < nxp && inu(c);
+        const float txx = u ? ldc(Alm + 2 * n, c) : 0.f, tzz = u ? ldc(Alm + 3 * n, c) : 0.f;
+        const float txz = u ? ldc(Alm + 4 * n, c) : 0.f;
+        const float l = ldc(a.m.lam, c), mu = ldc(a.m.mu, c), l2m = l + 2 * mu;
+        qwa[k + 1] = u ? l2m * txx + l * tzz : 0.f;
+        qwc[k + 1] = u ? mu * txz : 0.f;
+        if (k >= 4) qwb[k - 3] = u ? l * txx + l2m * tzz : 0.f;
+        gxx[k + 1] = UNDO && zin && c >= 0 && c < nxp ? ldc(Abg + 2 * n, c) : 0.f;
+        gxz[k + 1] = UNDO && zin && c >= 0 && c < nxp ? ldc(Abg + 4 * n, c) : 0.f;
+        if (k < 4) gzz[k + 1] = UNDO && zin && c >= 0 && c < nxp ? ldc(Abg + 3 * n, c) : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) qux[k] = gvx[k] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) quz[k] = gvz[k] = 0.f;
+
+    const int lsrc = S.isz - (C.z0 - FUSED_HALO);
+    const bool srow = z == S.isz;
+
+    auto column = [&](const int j, const AdjCol& o, const int p) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            qwa[k] = qwa[k + 1];
+            qwc[k] = qwc[k + 1];
+            qux[k] = qux[k + 1];
+            gxx[k] = gxx[k + 1];
+            gxz[k] = gxz[k + 1];
+            gvx[k] = gvx[k + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            qwb[k] = qwb[k + 1];
+            gzz[k] = gzz[k + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            quz[k] = quz[k + 1];
+            gvz[k] = gvz[k + 1];
+        }
+        qwa[8] = fresh_eb(o.wa4);
+        qwc[8] = fresh_eb(o.wc4);
+        qwb[4] = fresh_eb(o.wb4);
+        gxx[8] = fresh_eb(o.gxx4);
+        gxz[8] = fresh_eb(o.gxz4);
+        gzz[4] = fresh_eb(o.gzz);
+        lds[p][0][L] = gxz[4];
+        lds[p][1][L] = gzz[4];
+        lds[p][2][L] = qwc[4];
+        lds[p][3][L] = qwb[0];  // wb of column j (qwb[k]: column j+k)
+        lds[p][4][L] = qux[4];  // column j-4
+        lds[p][5][L] = quz[3];
+        lds[p][6][L] = gvx[4];
+        lds[p][7][L] = gvz[3];
+        __syncthreads();
+
+        // ---- velocity of column j (k_eb_adj_v)
+        float ngvx = o.gvx, ngvz = o.gvz, nlvx = o.lvx, nlvz = o.lvz, ux = 0.f, uz = 0.f;
+        if (vlane && inu(j)) {
+            const float* L0 = lds[p][0];
+            const float* L1 = lds[p][1];
+            const float* L2 = lds[p][2];
+            const float* L3 = lds[p][3];
+            const float br = 1.f / o.rho;
+            if (UNDO) {
+                const bool sx = srow && (unsigned)(S.isx - j + 3) < 8u;
+                auto txx = [&](int k) { return (sx && j + k == S.isx) ? gxx[4 + k] + w : gxx[4 + k]; };
+                auto tzz = [&](int k) { return (j == S.isx && L + k == lsrc) ? L1[L + k] + w : L1[L + k]; };
+                ngvx += -(br * cax * ebp(txx) + br * caz * ebm([&](int k) { return L0[L + k]; }));
+                ngvz += -(br * cax * ebm([&](int k) { return gxz[4 + k]; }) + br * caz * ebp(tzz));
+            }
+            nlvx = o.lvx + o.ix - (cax * ebp([&](int k) { return qwa[4 + k]; }) +
+                                   caz * ebm([&](int k) { return L2[L + k]; }));
+            nlvz = o.lvz + o.iz - (caz * ebp([&](int k) { return L3[L + k]; }) +
+                                   cax * ebm([&](int k) { return qwc[4 + k]; }));
+            ux = br * nlvx;
+            uz = br * nlvz;
+        }
+        qux[8] = ux;
+        quz[7] = uz;
+        gvx[8] = ngvx;
+        gvz[7] = ngvz;
+        if (zown && j >= x0 && j < x1) {
+            const uint32_t c = zo4 + (uint32_t)j * colb;
+            st(a.Blm, c, nlvx);
+            st(a.Blm + n, c, nlvz);
+            if (UNDO) {
+                st(a.Bbg, c, ngvx);
+                st(a.Bbg + n, c, ngvz);
+            }
+        }
+
+        // ---- stress of column cs = j - 4 (k_eb_adj_s)
+        const int cs = j - 4;
+        if (zown && cs >= x0 && cs < x1) {
+            const uint32_t c = zo4 + (uint32_t)cs * colb;
+            float ltxx = at(Alm + 2 * n, c), ltzz = at(Alm + 3 * n, c), ltxz = at(Alm + 4 * n, c);
+            float btxx = gxx[0], btzz = gzz[0], btxz = gxz[0];
+            if (inu(cs)) {
+                const float* L4 = lds[p][4];
+                const float* L5 = lds[p][5];
+                const float* L6 = lds[p][6];
+                const float* L7 = lds[p][7];
+                const float txx = ltxx - cax * ebm([&](int k) { return qux[4 + k]; });
+                const float tzz = ltzz - caz * ebm([&](int k) { return L5[L + k]; });
+                const float txz = ltxz - (caz * ebp([&](int k) { return L4[L + k]; }) +
+                                          cax * ebp([&](int k) { return quz[3 + k]; }));
+                ltxx = txx;
+                ltzz = tzz;
+                ltxz = txz;
+                const float l = at(a.m.lam, c), mu = at(a.m.mu, c), l2m = l + 2 * mu;
+                const bool inside = zphys && cs >= S.pml && cs < S.pml + S.nx;
+                float exx = 0.f, ezz = 0.f, exz = 0.f;
+                if (UNDO) {
+                    const float dvx = ebm([&](int k) { return gvx[4 + k]; }), dvz = ebm([&](int k) { return L7[L + k]; });
+                    const float dvxz = ebp([&](int k) { return L6[L + k]; }), dvzx = ebp([&](int k) { return gvz[3 + k]; });
+                    exx = dvx * idx;
+                    ezz = dvz * idz;
+                    exz = dvzx * idx + dvxz * idz;
+                    btxx += -(cax * l2m * dvx + caz * l * dvz);
+                    btzz += -(cax * l * dvx + caz * l2m * dvz);
+                    btxz += -(mu * (cax * dvzx + caz * dvxz));
+                } else if (inside) {
+                    const size_t h =
+                        (size_t)S.hist_off + ((size_t)it * S.nx + (cs - S.pml)) * S.nz + (size_t)(z - S.pml);
+                    exx = a.hist[h];
+                    ezz = a.hist[h + a.hstride];
+                    exz = a.hist[h + 2 * a.hstride];
+                }
+                if (inside) {
+                    float gpp = at(a.gpp, c), gps = at(a.gps, c);
+                    gpp += -2 * l2m * (exx + ezz) * (txx + tzz);
+                    gps += 4 * mu * (ezz * txx + exx * tzz) - 2 * mu * exz * txz;
+                    st(a.gpp, c, gpp);
+                    st(a.gps, c, gps);
+                }
+            }
+            st(a.Blm + 2 * n, c, ltxx);
+            st(a.Blm + 3 * n, c, ltzz);
+            st(a.Blm + 4 * n, c, ltxz);
+            if (UNDO) {
+                st(a.Bbg + 2 * n, c, btxx);
+                st(a.Bbg + 3 * n, c, btzz);
+                st(a.Bbg + 4 * n, c, btxz);
+            }
+        }
+    };
+
+    const int jend = j0 + (x1 + 4 - j0 + 1) / 2 * 2;
+    AdjCol cA = fetch(j0), cB = fetch(j0 + 1);
+    for (int j = j0; j < jend; j += 2) {
+        const bool more = j + 2 < jend;
+        column(j, cA, 0);
+        if (more) cA = fetch(j + 2);
+        column(j + 1, cB, 1);
+        if (more) cB = fetch(j + 3);
+    }
+}
+
 bool eb_fused_threads_ok(int threads) { return threads == 256 || threads == 512; }
 
 void launch_eb_fused(hipStream_t st, const BatchShape& b, const EbFusedArgs& a, const ChunkDesc* chunks,
@@ -292,6 +551,28 @@ void launch_eb_fused(hipStream_t st, const BatchShape& b, const EbFusedArgs& a, 
     else
         hipLaunchKernelGGL(k_eb_fused<256>, grid, dim3(256), 0, st, a, chunks, nchunks, per_xcd, rmap, b.slots, cax,
                            caz, idx, idz, w, sign, it, b.nt, hist_mode);
+}
+
+void launch_eb_adj_fused(hipStream_t st, const BatchShape& b, const EbAdjArgs& a, const ChunkDesc* chunks,
+                         int nchunks, const int* rmap, float w, int it, int it_inj, bool undo, int threads) {
+    const int per_xcd = (nchunks + 7) / 8;
+    const dim3 grid(per_xcd * 8);
+    const float cax = b.dt / b.dx, caz = b.dt / b.dz, idx = 1.f / b.dx, idz = 1.f / b.dz;
+#define DS2D_EBA_LAUNCH(TT, UU)                                                                                \
+    hipLaunchKernelGGL((k_eb_adj_fused<TT, UU>), grid, dim3(TT), 0, st, a, chunks, nchunks, per_xcd, rmap, b.slots, \
+                       cax, caz, idx, idz, w, it, it_inj, b.nt)
+    if (threads == 512) {
+        if (undo)
+            DS2D_EBA_LAUNCH(512, true);
+        else
+            DS2D_EBA_LAUNCH(512, false);
+    } else {
+        if (undo)
+            DS2D_EBA_LAUNCH(256, true);
+        else
+            DS2D_EBA_LAUNCH(256, false);
+    }
+#undef DS2D_EBA_LAUNCH
 }
 
 }  // namespace ds2d

@@ -214,9 +214,10 @@ def test_fused_born_step_matches_the_two_kernels(bg_mode, threads, width):
     """Three uneven windows (one of 12 columns) on an 80 x 300 model, so a
     256-lane block covers the 340 padded rows in two row blocks; an odd step
     count, so the fused pass ends in its second state and copies it back.
-    The records of the Born forward map and the PP/PS CG images (whose
-    adjoint passes rebuild the background from the forward pass's final
-    state when bg_mode = 2) agree with those of k_eb_v + k_eb_s to FP32
+    The records of the Born forward map and the PP/PS CG images (forward
+    and adjoint passes; with bg_mode = 2 the adjoint rebuilds the background
+    from the forward pass's final state) of the fused steps agree with those
+    of the per-cell kernels (k_eb_v + k_eb_s, k_eb_adj_v + k_eb_adj_s) to FP32
     rounding (relative L2 <= 1e-6: the expressions are the same, but the
     compiler may contract different products into FMAs in the two codes)."""
     allnx, allnz, pml, nt = 80, 300, 20, 151
