@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--no-graph", action="store_true", help="c3: launch the time loop's kernels directly")
     ap.add_argument("--no-fused-born", action="store_true",
                     help="c5: the elastic Born forward pass with two kernels per step instead of one fused launch")
+    ap.add_argument("--no-fused-rot", action="store_true",
+                    help="rot: k_rot_v + k_rot_p per step instead of the fused rotation step")
     ap.add_argument("--born-tile", default=None, help="c5: x-chunk width,lanes of the fused Born step (e.g. 64,256)")
     ap.add_argument("--no-chunk-plan", action="store_true",
                     help="two steps per launch: fixed-width chunks instead of the balanced chunk plan")
@@ -252,8 +254,9 @@ def run_forward(a, cfg):
 
 
 def run_rot(a, cfg):
-    """P/S-decoupled (theta/omega) forward modelling of the C3 windows: two
-    per-cell kernels per step (velocity parts, then dilatation/rotation)."""
+    """P/S-decoupled (theta/omega) forward modelling of the C3 windows: one
+    fused launch per step (velocity parts, then dilatation/rotation, in one
+    x-march), or with --no-fused-rot the two per-cell kernels per step."""
     from ds2d_amd import Solver
     from ds2d_amd.lib import ricker
     wav = ricker(cfg["nt"], cfg["f0"], cfg["dt"])
@@ -261,6 +264,7 @@ def run_rot(a, cfg):
     s = Solver(dx=cfg["dx"], dz=cfg["dz"], dt=cfg["dt"], nt=cfg["nt"], allnx=cfg["nx"], allnz=cfg["nz"],
                pml=cfg["pml"], nx_max=nx_max, wavelet=wav, pml_rr=1e-3)
     s.set_model(cfg["vp"], cfg["vs"], cfg["rho"])
+    s.set_fused_rotation(not a.no_fused_rot)
     for _ in range(a.warmup):
         s.rot_forward(cfg["shots"], fetch=False)
     stats = []
@@ -278,7 +282,9 @@ def run_rot(a, cfg):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": f"synthetic: seeded layered+faulted elastic model (seed {cfg['seed']}), vs = vp/1.73",
         "config": {"workload": cfg["desc"], "shots": len(cfg["shots"]), "nt": cfg["nt"], "cell_updates_per_step": cu,
-                   "kernels": "k_rot_v + k_rot_p per step (per-cell, reference layout of the rotation scheme)",
+                   "kernels": ("k_rot_v + k_rot_p per step (per-cell, reference layout of the rotation scheme)"
+                               if a.no_fused_rot else "k_rot_fused: one x-marching launch per step (unsplit "
+                               "theta/omega outside the damping strips)"),
                    "parallelism": "shots batched on one GPU"},
         "roofline": roofline(stats),
     }

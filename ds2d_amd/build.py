@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libds2d.so")
 SOURCES = ["api.hip", "kernels_ref.hip", "kernels_setup.hip", "kernels_laplace.hip", "kernels_acoustic.hip", "kernels_fast.hip",
-           "kernels_fused.hip", "kernels_fused2.hip", "kernels_acoustic_fused.hip", "kernels_eborn.hip", "kernels_eborn_fused.hip", "kernels_rot.hip", "unet_api.hip",
+           "kernels_fused.hip", "kernels_fused2.hip", "kernels_acoustic_fused.hip", "kernels_eborn.hip", "kernels_eborn_fused.hip", "kernels_rot.hip", "kernels_rot_fused.hip", "unet_api.hip",
            "unet_kernels.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DS2D_ARCH", "gfx950")

@@ -83,6 +83,7 @@ struct ds2d_ctx {
     int chunk_plan = 1; // two steps per launch: balanced chunk plan when it simulates faster (0: fixed width)
     int eb_fused = 1;   // elastic Born forward pass: one fused launch per step (0: k_eb_v + k_eb_s)
     int eb_w = 0, eb_t = 0;  // its x-chunk width and workgroup lanes (0: defaults 64, 256)
+    int rot_fused = 1;  // rotation forward: one fused launch per step (0: k_rot_v + k_rot_p)
     int last_plan = 0;  // the last two-step launch list: 1 balanced, 0 fixed width
     std::vector<int> plan_key;         // batch shape of the cached two-step chunk list
     std::vector<ChunkDesc> plan_list;  // its chunk list
@@ -1028,6 +1029,13 @@ int ds2d_set_fused_born(ds2d_t* h, int on, int width, int threads) {
     h->eb_fused = on ? 1 : 0;
     h->eb_w = width;
     h->eb_t = threads;
+    return 0;
+}
+
+int ds2d_set_fused_rotation(ds2d_t* h, int on) {
+    int rc;
+    if ((rc = check_h(h))) return rc;
+    h->rot_fused = on ? 1 : 0;
     return 0;
 }
 
@@ -2462,13 +2470,70 @@ int ds2d_rot_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec_z
     const BatchShape& sh = B.shape;
     hipStream_t st = h->st;
     const int nt = h->g.nt;
-    RotField f;
-    if ((rc = alloc_rot(h, "f", B.total_cells, f))) return rc;
     const size_t R = std::max<long long>(B.total_rec, 1);
     float* rs;
     if ((rc = h->fbuf("frec", R * 2, &rs))) return rc;
     StepSampler smp;
     if ((rc = smp.init(nt))) return rc;
+    const long long n = B.total_cells;
+    if (h->rot_fused && n < (1LL << 30)) {
+        // one fused launch per step (64-column chunks, 256 lanes), state
+        // ping-ponged between two sets of 8 fields
+        float* fa;
+        int* rmap;
+        if ((rc = h->fbuf("rotf", (size_t)n * 16, &fa)) || (rc = h->fbuf("rmap", B.total_rmap, &rmap))) return rc;
+        HIP_TRY(hipMemsetAsync(fa, 0, (size_t)n * 16 * 4, st));
+        HIP_TRY(hipMemsetAsync(rmap, 0xff, B.total_rmap * 4, st));
+        HIP_TRY(hipMemsetAsync(rs, 0, R * 2 * 4, st));
+        launch_fast_rmap(st, sh, rmap);
+        const int W = 64, T = 256, rows = T - 2 * FUSED_HALO;
+        std::vector<ChunkDesc> ch;
+        for (int s = 0; s < nshots; ++s)
+            for (int x0 = 0; x0 < B.slots[s].nxp; x0 += W)
+                for (int z0 = 0; z0 < B.slots[s].nzp; z0 += rows)
+                    ch.push_back(ChunkDesc{s, x0, std::min(x0 + W, B.slots[s].nxp), z0});
+        ChunkDesc* d_ch;
+        if ((rc = h->fbuf("rotchunks", ch.size(), &d_ch))) return rc;
+        HIP_TRY(hipMemcpyAsync(d_ch, ch.data(), ch.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice, st));
+        stats_begin(h);
+        stats_loop_begin(h);
+        float *a = fa, *b = fa + 8 * n;
+        for (int it = 0; it < nt; ++it) {
+            const bool tk = smp.take(it);
+            if (tk) smp.mark(st, 0);
+            RotFusedArgs q{a, b, n, m.vp, m.vs, m.dampx, m.dampz, h->d_wav, rs, rs + R};
+            launch_rot_fused(st, sh, q, d_ch, (int)ch.size(), rmap, it, T, B.any_pml);
+            if (tk) {
+                smp.mark(st, 1);
+                smp.mark(st, 2);
+            }
+            std::swap(a, b);
+        }
+        stats_loop_end(h);
+        if ((rc = stats_end(h, stats, padded_cells(B) * nt, nt, 0))) return rc;
+        double kf, unused;
+        smp.averages(&kf, &unused);
+        // per padded cell: theta, omega, 4 velocity parts, vp, vs in, 6 fields out (56 B); the damping cells
+        // add dampx, dampz and theta_x, omega_x read and written (24 B)
+        h->last.kernel_avg_ms = kf;
+        double pml_cells = 0;  // update-region cells of the damping strips (PML windows)
+        for (auto& S : B.slots)
+            if (S.boundary == 0 && S.pml >= NOP)
+                pml_cells += (double)(S.nxp - 2 * NOP) * (S.nzp - 2 * NOP) -
+                             (double)(S.nxp - 2 * S.pml + 1) * (S.nzp - 2 * S.pml + 1);
+        h->last.alg_bytes = (double)n * 14 * 4 + pml_cells * 6 * 4;
+        h->last.aux_kernel_avg_ms = 0;
+        h->last.aux_alg_bytes = 0;
+        h->last.kernel_samples = smp.n;
+        if (stats) *stats = h->last;
+        if (rec_z) HIP_TRY(hipMemcpyAsync(rec_z, rs, B.total_rec * 4, hipMemcpyDeviceToHost, st));
+        if (rec_x) HIP_TRY(hipMemcpyAsync(rec_x, rs + R, B.total_rec * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
+    RotField f;
+    if ((rc = alloc_rot(h, "f", B.total_cells, f))) return rc;
     stats_begin(h);
     stats_loop_begin(h);
     for (int it = 0; it < nt; ++it) {

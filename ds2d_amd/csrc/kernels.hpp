@@ -191,6 +191,18 @@ void launch_eb_s(hipStream_t st, const BatchShape& b, EbField bg, EbField born, 
 // threads 256 or 512.  The result equals launch_eb_v + launch_eb_s with
 // do_bg = do_born = 1 up to FP32 rounding (FMA contraction).
 bool eb_fused_threads_ok(int threads);
+// One fused step of the rotation scheme (kernels_rot_fused.hip): A and B hold
+// 8 fields of n floats (theta omega vpx vpz vsx vsz, then theta_x omega_x of
+// the damping cells); records of vz, vx through the column -> trace map.
+struct RotFusedArgs {
+    const float* A;
+    float* B;
+    long long n;
+    const float *vp, *vs, *dampx, *dampz, *wav;
+    float *rec_z, *rec_x;
+};
+void launch_rot_fused(hipStream_t st, const BatchShape& b, const RotFusedArgs& a, const struct ChunkDesc* chunks,
+                      int nchunks, const int* rmap, int it, int threads, bool pml);
 // One fused adjoint step (k_eb_adj_v with the residual of step it_inj, then
 // k_eb_adj_s of step it): adjoint state and background read from A, written
 // to B (5 fields each: vx vz txx tzz txz); gradients accumulated in place.

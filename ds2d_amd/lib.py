@@ -80,6 +80,7 @@ SIGNATURES = {
     "ds2d_set_time_block": (C.c_int, [_H, C.c_int]),
     "ds2d_set_chunk_plan": (C.c_int, [_H, C.c_int]),
     "ds2d_set_fused_born": (C.c_int, [_H, C.c_int, C.c_int, C.c_int]),
+    "ds2d_set_fused_rotation": (C.c_int, [_H, C.c_int]),
     "ds2d_get_chunk_plan": (C.c_int, [_H, C.POINTER(C.c_int)]),
     "ds2d_set_stop": (C.c_int, [_H, C.c_double]),
     "ds2d_set_log": (C.c_int, [_H, C.c_int]),
@@ -265,6 +266,10 @@ class Solver:
     def set_fused_born(self, on, width=0, threads=0):
         """Elastic Born forward pass: one fused launch per step (True, default) or the two per-step kernels."""
         _check(self.lib.ds2d_set_fused_born(self.h, 1 if on else 0, int(width), int(threads)))
+
+    def set_fused_rotation(self, on):
+        """Rotation forward: one fused launch per step (True, default) or k_rot_v + k_rot_p."""
+        _check(self.lib.ds2d_set_fused_rotation(self.h, 1 if on else 0))
 
     def chunk_plan_used(self):
         """Whether the last two-step launch list was the balanced plan."""

@@ -196,6 +196,12 @@ int ds2d_get_chunk_plan(ds2d_t* h, int* used);
  * width: x-chunk width of the fused step (0 = 64); threads: 256 or 512 lanes
  * (0 = 256).  No reference counterpart (new capability). */
 int ds2d_set_fused_born(ds2d_t* h, int on, int width, int threads);
+/* Rotation-scheme forward modelling (ds2d_rot_forward): on (1, the default)
+ * one fused launch per time step updates the velocity parts and theta /
+ * omega (unsplit outside the damping strips); off (0) launches the per-cell
+ * k_rot_v and k_rot_p of the reference layout.  No reference counterpart
+ * (new capability). */
+int ds2d_set_fused_rotation(ds2d_t* h, int on);
 /* Capture each time loop into one hipGraph (1) or launch step by step (0). */
 int ds2d_set_graph(ds2d_t* h, int on);
 int ds2d_autotune_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, int nsteps, int* best_width,

@@ -153,3 +153,35 @@ def test_rotation_and_decomposed_rtm_at_c1_size():
         e = orc.rel_l2(img.ravel(), oimg)
         print(f"C1 size decomposed RTM kind {kind}: rel {e:.2e}")
         assert np.abs(oimg).max() > 0 and e <= TOL
+
+
+def test_fused_rotation_step_matches_per_cell_kernels():
+    """The fused rotation step (unsplit theta/omega outside the damping
+    strips) against k_rot_v + k_rot_p on a PML and a random-boundary window
+    plus a 12-column window, over rows that need two 256-lane row blocks:
+    records agree to FP32 rounding of the two arithmetic orders."""
+    allnx, allnz, pml, nt = 90, 260, 20, 300
+    vp, vs, rho = elastic_model(allnx, allnz, 37)
+    wav = wavelet(nt)
+    shots = [ShotSpec(0, 60, pml + 30, pml + 3, pml + 1, pml + np.arange(0, 60, 2, dtype=np.int32), 0, 0),
+             ShotSpec(30, 60, pml + 20, pml + 228, pml + 226, pml + np.arange(60, dtype=np.int32), 1, 1),
+             ShotSpec(70, 12, pml + 6, pml + 4, pml + 2, pml + np.arange(12, dtype=np.int32), 2, 0)]
+    out = {}
+    for fused in (False, True):
+        s = _solver(allnx, allnz, pml, nt, wav)
+        s.set_model(vp, vs, rho)
+        s.set_fused_rotation(fused)
+        rz, rx, _ = s.rot_forward(shots)
+        out[fused] = (rz, rx)
+    ez, ex = orc.rel_l2(out[True][0], out[False][0]), orc.rel_l2(out[True][1], out[False][1])
+    print(f"fused vs per-cell rotation forward: rel z {ez:.2e} x {ex:.2e}")
+    assert np.abs(out[False][0]).max() > 0
+    assert ez <= 2e-6 and ex <= 2e-6
+    off = 0
+    for sh in shots:
+        n = sh.ntr * nt
+        oz, ox = orc.rot_forward(_oshot(sh, allnx, allnz, pml, nt), vp, vs, wav)
+        ez, ex = orc.rel_l2(out[True][0][off:off + n], oz), orc.rel_l2(out[True][1][off:off + n], ox)
+        print(f"shot {sh.shot_id} (boundary {sh.boundary}, {sh.nx} columns): fused vs oracle rel z {ez:.2e} x {ex:.2e}")
+        assert ez <= TOL and ex <= TOL
+        off += n
