@@ -634,7 +634,9 @@ def run_c5(a):
                                                              for x in shots)),
                        "born_data_s": round(t_fwd, 2), "solvers": runs, "elastic_unet_training": train,
                        "note": "a step = one CG iteration over all shots; 1e-2 is not reached in this many "
-                               "iterations (see resid)"}}
+                               "iterations (see resid)"},
+            # the fused forward step of the Born data pass (k_eb_fused, 100 B per padded cell and step)
+            "roofline": roofline([stf]) if stf.get("kernel_avg_ms", 0) > 0 else None}
 
 
 def run_compat(a):

@@ -1748,6 +1748,7 @@ struct EbWS {
     ChunkDesc* chunks = nullptr;
     std::vector<std::pair<int, int>> gchunks;
     int threads = 256;
+    StepSampler* smp = nullptr;  // when set, the fused forward launches are timed (sampled steps)
 };
 
 EbField eb_field(float* base, long long n) {
@@ -1851,8 +1852,14 @@ void eb_fwd(ds2d_ctx* h, const Batch& B, EbWS& w, bool do_bg) {
             float *a = w.bg.vx, *b = w.alt;
             for (int it = 0; it < nt; ++it) {
                 EbFusedArgs q{a, b, w.n, w.m, w.hist, w.H, w.rec, w.rec + w.R};
+                const bool tk = w.smp && g == 1 && w.smp->take(it);
+                if (tk) w.smp->mark(st, 0);
                 launch_eb_fused(st, sh, q, w.chunks + gc.first, gc.second, w.rmap, h->wav[it], 1.f, it, hmode,
                                 w.threads);
+                if (tk) {
+                    w.smp->mark(st, 1);
+                    w.smp->mark(st, 2);
+                }
                 std::swap(a, b);
             }
         }
@@ -1954,11 +1961,27 @@ int ds2d_eborn_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, const floa
     HIP_TRY(hipMemcpyAsync(d_img + A, img_ps, A * 4, hipMemcpyHostToDevice, h->st));
     EbWS w;
     if ((rc = eb_prepare(h, B, d_img, d_img + A, false, w))) return rc;
+    StepSampler smp;
+    if ((rc = smp.init(h->g.nt))) return rc;
+    w.smp = &smp;
     stats_begin(h);
     stats_loop_begin(h);
     eb_fwd(h, B, w, true);
     stats_loop_end(h);
     if ((rc = stats_end(h, stats, 2 * eb_cells(B, h->g.nt), 2LL * h->g.nt, 0))) return rc;
+    // the fused forward step (first launch group): 100 B per padded cell (both states in and out, rho,
+    // lambda, mu, the two image samples)
+    double kf = 0, unused = 0;
+    smp.averages(&kf, &unused);
+    h->last.kernel_avg_ms = w.alt ? kf : 0;
+    h->last.kernel_samples = w.alt ? smp.n : 0;
+    long long cells0 = 0;  // padded cells of the first launch group
+    {
+        const std::vector<BatchShape> gr = launch_groups(h, B);
+        for (int s = 0; s < gr[0].nslots; ++s) cells0 += (long long)B.slots[s].nxp * B.slots[s].nzp;
+    }
+    h->last.alg_bytes = w.alt ? (double)cells0 * 25 * 4 : 0;
+    if (stats) *stats = h->last;
     HIP_TRY(hipMemcpyAsync(rec_z, w.rec, B.total_rec * 4, hipMemcpyDeviceToHost, h->st));
     HIP_TRY(hipMemcpyAsync(rec_x, w.rec + w.R, B.total_rec * 4, hipMemcpyDeviceToHost, h->st));
     HIP_TRY(hipStreamSynchronize(h->st));
