@@ -2,7 +2,7 @@
 //
 // A workgroup of k_fused2_step marches one chunk: columns [x0, x1) of one
 // window and one block of rows.  Its time is close to linear in the width:
-// on the C3 batch (128 windows of 440-1240 padded columns, 704 lanes) a chunk
+// on 128 windows of 440-1240 padded columns (2000x600 model, 704 lanes) a chunk
 // of w columns took 63 us + 2.80 us per column with the PML and 38 us + 1.90 us
 // with the random boundary, i.e. w plus 20-23 columns, the 12 + 12 columns of
 // the march outside the chunk.  One workgroup fits a CU (112 KB of LDS), so a
@@ -12,7 +12,7 @@
 // when its current one ends.
 //
 // Fixed-width chunks leave CUs idle at the end of a launch: with 240-column
-// chunks on C3 every XCD gets 64 chunks, most of them 240 wide, and the CUs
+// chunks on that batch every XCD gets 64 chunks, most of them 240 wide, and the CUs
 // that draw two of those end 12 % after the mean.  The balanced plan instead
 //   (1) deals whole row-block units (window, row block) to the eight XCDs,
 //       largest first, each to the least-loaded XCD (neighbouring chunks of a

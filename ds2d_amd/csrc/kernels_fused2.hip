@@ -84,10 +84,10 @@ __global__ __launch_bounds__(T) void k_fused2_step(FusedArgs a, float* __restric
     // consecutive ranks, so one wave instead of three carries the damped rows
     // of the top and bottom strips; the other waves hold contiguous interior
     // rows.  Rank = lane - 192 (mod T): the damping wave is the block's fourth
-    // wave.  Its place matters, with the results bit-identical: on the C3
-    // batch (704 lanes) a launch takes 1.485 ms with it in the fourth wave,
-    // 1.519 ms in the first, 1.52-1.55 ms in waves 2-8 and 1.66 ms in the
-    // last three.
+    // wave.  Its place matters, with the results bit-identical: on 128 uneven
+    // windows of a 2000x600 model with a 20-cell PML (704 lanes) a launch
+    // takes 1.485 ms with it in the fourth wave, 1.519 ms in the first,
+    // 1.52-1.55 ms in waves 2-8 and 1.66 ms in the last three.
     const int zb = C.z0 - FUSED2_HALO;
     int r = threadIdx.x;
     if (PML && S.boundary == 0) {
