@@ -789,10 +789,12 @@ def main():
                tile.get("time_steps_per_launch", 1)]
         # per-launch traffic depends on the batch and the tile, not on the number of steps
         if (tr.get("workload_prefix") and str(cfgd.get("workload", "")).startswith(tr["workload_prefix"])
-                and tr.get("shots") == cfgd.get("shots")
-                and bool(tr.get("random_boundary", False)) == bool(getattr(a, "random_boundary", False))):
+                and tr.get("shots") == cfgd.get("shots")):
             balanced = tile.get("chunk_plan") == "balanced"
+            rb = bool(getattr(a, "random_boundary", False))
             for e in tr.get("entries", [tr]):
+                if bool(e.get("random_boundary", tr.get("random_boundary", False))) != rb:
+                    continue
                 t = list(e.get("tile") or [])
                 if len(t) == 3:  # entries without a time block are one step per launch
                     t.append(1)
