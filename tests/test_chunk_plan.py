@@ -73,8 +73,10 @@ def _c3_windows():
     return np.array([sh.nx + 2 * cfg["pml"] for sh in cfg["shots"]]), cfg["nz"] + 2 * cfg["pml"]
 
 
+@pytest.mark.parametrize("cpx", [32, 16, 38])
 @pytest.mark.parametrize("case", ["c3", "two_row_blocks", "random"])
-def test_balanced_plan_invariants(planner, case):
+def test_balanced_plan_invariants(planner, case, cpx):
+    """cpx: CUs per XCD (32 on MI355X; other counts for parts with fewer or more CUs)."""
     rows = 672
     if case == "c3":
         nxp, nz = _c3_windows()
@@ -87,7 +89,6 @@ def test_balanced_plan_invariants(planner, case):
         rng = np.random.default_rng(11)
         nxp = rng.integers(200, 2000, 40)
         nzp = rng.integers(100, 1500, 40)
-    cpx = 32
     span, ch = planner(nxp, nzp, rows, cpx)
     assert span > 0
     n = len(ch)
