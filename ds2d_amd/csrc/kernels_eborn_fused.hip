@@ -419,6 +419,23 @@ __global__ __launch_bounds__(T) void k_eb_adj_fused(EbAdjArgs a, const ChunkDesc
             quz[k] = quz[k + 1];
             gvz[k] = gvz[k + 1];
         }
+        // operands of the stress part of column j-4, issued before the barrier
+        const int cs = j - 4;
+        const bool own_s = zown && cs >= x0 && cs < x1;
+        const uint32_t cS = zo4 + (uint32_t)cs * colb;
+        const bool ins = own_s && inu(cs) && zphys && cs >= S.pml && cs < S.pml + S.nx;
+        float e_ltxx = 0.f, e_ltzz = 0.f, e_ltxz = 0.f, e_l = 0.f, e_mu = 0.f, e_gpp = 0.f, e_gps = 0.f;
+        if (own_s) {
+            e_ltxx = at(Alm + 2 * n, cS);
+            e_ltzz = at(Alm + 3 * n, cS);
+            e_ltxz = at(Alm + 4 * n, cS);
+            e_l = at(a.m.lam, cS);
+            e_mu = at(a.m.mu, cS);
+        }
+        if (ins) {
+            e_gpp = at(a.gpp, cS);
+            e_gps = at(a.gps, cS);
+        }
         qwa[8] = fresh_eb(o.wa4);
         qwc[8] = fresh_eb(o.wc4);
         qwb[4] = fresh_eb(o.wb4);
@@ -472,10 +489,9 @@ __global__ __launch_bounds__(T) void k_eb_adj_fused(EbAdjArgs a, const ChunkDesc
         }
 
         // ---- stress of column cs = j - 4 (k_eb_adj_s)
-        const int cs = j - 4;
-        if (zown && cs >= x0 && cs < x1) {
-            const uint32_t c = zo4 + (uint32_t)cs * colb;
-            float ltxx = at(Alm + 2 * n, c), ltzz = at(Alm + 3 * n, c), ltxz = at(Alm + 4 * n, c);
+        if (own_s) {
+            const uint32_t c = cS;
+            float ltxx = e_ltxx, ltzz = e_ltzz, ltxz = e_ltxz;
             float btxx = gxx[0], btzz = gzz[0], btxz = gxz[0];
             if (inu(cs)) {
                 const float* L4 = lds[p][4];
@@ -489,7 +505,7 @@ __global__ __launch_bounds__(T) void k_eb_adj_fused(EbAdjArgs a, const ChunkDesc
                 ltxx = txx;
                 ltzz = tzz;
                 ltxz = txz;
-                const float l = at(a.m.lam, c), mu = at(a.m.mu, c), l2m = l + 2 * mu;
+                const float l = e_l, mu = e_mu, l2m = l + 2 * mu;
                 const bool inside = zphys && cs >= S.pml && cs < S.pml + S.nx;
                 float exx = 0.f, ezz = 0.f, exz = 0.f;
                 if (UNDO) {
@@ -509,7 +525,7 @@ __global__ __launch_bounds__(T) void k_eb_adj_fused(EbAdjArgs a, const ChunkDesc
                     exz = a.hist[h + 2 * a.hstride];
                 }
                 if (inside) {
-                    float gpp = at(a.gpp, c), gps = at(a.gps, c);
+                    float gpp = e_gpp, gps = e_gps;
                     gpp += -2 * l2m * (exx + ezz) * (txx + tzz);
                     gps += 4 * mu * (ezz * txx + exx * tzz) - 2 * mu * exz * txz;
                     st(a.gpp, c, gpp);
