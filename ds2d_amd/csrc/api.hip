@@ -1829,7 +1829,11 @@ int eb_prepare(ds2d_ctx* h, const Batch& B, const float* pp_dev, const float* ps
         }
         if ((rc = h->fbuf("ebchunks", ch.size(), &w.chunks))) return rc;
         HIP_TRY(hipMemcpyAsync(w.chunks, ch.data(), ch.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice, h->st));
-        if ((rc = h->fbuf("eborn2", (size_t)n * 10, &w.alt))) return rc;
+        // the second state; when it does not fit, the per-cell kernels (one state) run instead
+        if (h->fbuf("eborn2", (size_t)n * 10, &w.alt)) {
+            w.alt = nullptr;
+            (void)hipGetLastError();
+        }
     }
     return 0;
 }
@@ -2499,12 +2503,18 @@ int ds2d_rot_forward(ds2d_t* h, int nshots, const ds2d_shot* shots, float* rec_z
     StepSampler smp;
     if ((rc = smp.init(nt))) return rc;
     const long long n = B.total_cells;
-    if (h->rot_fused && n < (1LL << 30)) {
+    float* fa = nullptr;
+    // one fused launch per step needs two sets of 8 fields; when they do not
+    // fit, the per-cell kernels (12 fields) run instead
+    if (h->rot_fused && n < (1LL << 30) && h->fbuf("rotf", (size_t)n * 16, &fa)) {
+        fa = nullptr;
+        (void)hipGetLastError();
+    }
+    if (fa) {
         // one fused launch per step (64-column chunks, 256 lanes), state
         // ping-ponged between two sets of 8 fields
-        float* fa;
         int* rmap;
-        if ((rc = h->fbuf("rotf", (size_t)n * 16, &fa)) || (rc = h->fbuf("rmap", B.total_rmap, &rmap))) return rc;
+        if ((rc = h->fbuf("rmap", B.total_rmap, &rmap))) return rc;
         HIP_TRY(hipMemsetAsync(fa, 0, (size_t)n * 16 * 4, st));
         HIP_TRY(hipMemsetAsync(rmap, 0xff, B.total_rmap * 4, st));
         HIP_TRY(hipMemsetAsync(rs, 0, R * 2 * 4, st));
