@@ -220,7 +220,9 @@ def run_forward(a, cfg):
         _, _, st = s.batch_forward(cfg["shots"], mode=a.mode, fetch=False)
         stats.append(st)
     elapsed = time.perf_counter() - t0
-    plan = tblock == 2 and s.chunk_plan_used()  # the timed launches' chunk list (two steps per launch)
+    # the timed launches' chunk list (two steps per launch): with the balanced plan the chunk widths come from
+    # the plan, so the tile's width is not reported
+    plan = tblock == 2 and s.chunk_plan_used()
     rz, rx, _ = s.batch_forward(cfg["shots"][:2], mode=a.mode, fetch=True)  # untimed validation
     assert np.isfinite(rz).all() and np.isfinite(rx).all() and np.abs(rz).max() > 0
     cu = stats[0]["cell_updates"]
@@ -234,7 +236,7 @@ def run_forward(a, cfg):
             "kernels": KERNELS[1 if (a.mode == 2 and chunk == 0) else a.mode] +
                        (", two time steps per launch (temporal blocking)" if tblock == 2 else "") +
                        (" (autotuned)" if a.mode == 2 and not a.tile else ""), "parallelism": "shots batched on one GPU",
-            "autotuned_tile": None if chunk is None else {"x_width": chunk, "threads": threads,
+            "autotuned_tile": None if chunk is None else {"x_width": None if plan else chunk, "threads": threads,
                                                           "prefetch_columns": depth,
                                                           "time_steps_per_launch": tblock,
                                                           "chunk_plan": "balanced" if plan else "fixed width"},
