@@ -41,6 +41,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "k_fused_step_c3_traffic.json")  # rocprof FETCH/WRITE per launch
 
 
 def parse():
@@ -80,7 +81,7 @@ def parse():
                     help="two steps per launch: fixed-width chunks instead of the balanced chunk plan")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=5.0, help="CPU baseline: seconds per timed run (3 runs)")
-    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "k_fused_step_c3_traffic.json"),
+    ap.add_argument("--traffic-json", default=TRAFFIC_JSON,
                     help="rocprof FETCH/WRITE summary (JSON) to embed as roofline.traffic when its kernel and "
                          "workload match the run")
     ap.add_argument("--random-boundary", action="store_true",
@@ -783,8 +784,11 @@ def main():
         out = run_lsrtm(a, cfg)
     name, ncu, mem = device_info()
     out["device"] = {"arch": name, "cus": ncu, "hbm_gb": round(mem, 1)}
-    if a.traffic_json and os.path.exists(a.traffic_json) and out.get("roofline"):
-        tr = json.load(open(a.traffic_json))
+    traffic_json = a.traffic_json
+    if a.workload == "rot" and traffic_json == TRAFFIC_JSON:  # the fused rotation step's own counters
+        traffic_json = None if a.no_fused_rot else os.path.join(ROOT, "profiles", "k_rot_fused_c3_traffic.json")
+    if traffic_json and os.path.exists(traffic_json) and out.get("roofline"):
+        tr = json.load(open(traffic_json))
         cfgd = out.get("config", {})
         tile = cfgd.get("autotuned_tile") or {}
         key = [tile.get("x_width"), tile.get("threads"), tile.get("prefetch_columns"),
@@ -796,6 +800,10 @@ def main():
             rb = bool(getattr(a, "random_boundary", False))
             for e in tr.get("entries", [tr]):
                 if bool(e.get("random_boundary", tr.get("random_boundary", False))) != rb:
+                    continue
+                if e.get("tile") is None:  # a workload without tiles: one kernel, one figure
+                    if not tile:
+                        out["roofline"]["traffic"] = e.get("traffic_bytes_per_launch")
                     continue
                 t = list(e.get("tile") or [])
                 if len(t) == 3:  # entries without a time block are one step per launch
