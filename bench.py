@@ -630,7 +630,8 @@ def run_c5(a):
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
             "data": "synthetic: seeded 4000x1000 layered+faulted elastic model, Born data of the true PP/PS "
                     "reflectivities",
-            "config": {"workload": f"c5_combined: 4000x1000, dx=dz=10 m, {len(shots)} uneven windows (400-1200 "
+            "config": {"shots": len(shots),
+                       "workload": f"c5_combined: 4000x1000, dx=dz=10 m, {len(shots)} uneven windows (400-1200 "
                                    f"cols), random boundary {cfg['pml']}, dt {cfg['dt'] * 1e3:.1f} ms, "
                                    f"{cfg['nt']} steps, PP/PS elastic Born CG LSRTM",
                        "padded_cells_per_wavefield": int(sum((x.nx + 2 * cfg["pml"]) * (cfg["nz"] + 2 * cfg["pml"])
@@ -787,6 +788,8 @@ def main():
     traffic_json = a.traffic_json
     if a.workload == "rot" and traffic_json == TRAFFIC_JSON:  # the fused rotation step's own counters
         traffic_json = None if a.no_fused_rot else os.path.join(ROOT, "profiles", "k_rot_fused_c3_traffic.json")
+    if a.workload == "c5" and traffic_json == TRAFFIC_JSON:  # the fused Born forward step's counters
+        traffic_json = None if a.no_fused_born else os.path.join(ROOT, "profiles", "k_eb_fused_c5_traffic.json")
     if traffic_json and os.path.exists(traffic_json) and out.get("roofline"):
         tr = json.load(open(traffic_json))
         cfgd = out.get("config", {})
